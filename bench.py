@@ -1,0 +1,211 @@
+"""H-matvec benchmark on one MI355X.
+
+Workload (the single-GPU benchmark configuration): two disjoint icospheres of
+radius 1, centres 3 apart, 81,920 flat triangles each (163,840 unknowns),
+piecewise-constant collocation at centroids, kernel 1/(4 pi |x-y|) (centroid
+quadrature off the diagonal, analytic self terms), bisection cluster tree
+with leaf size 20, admissibility eta = 2, ACA+ tolerance 1e-5, x ~ U[-1,1]
+from seed 1.  All data synthetic (generated on the host by libhmvgen).
+
+One step = one FP64 H-matvec y = H x over the whole H-matrix with x, y and
+the arena resident in HBM.  value = algorithmic bytes per matvec
+(8*E + 16*N, E = stored entries) x steps / wall time of the timed steps.
+
+Also reported: per-phase HIP-event timings and the roofline of the dominant
+kernel, BiCGSTAB time-to-1e-8 in FP64 and with FP32 / BF16 storage under
+FP64 iterative refinement, the CPU baseline (the oracle's OpenMP per-leaf
+GEMV restatement of the reference CPU path, timed on this host), and the
+leaf-rank / leaf-size histograms.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "H-matvec HBM GB/s + BiCGSTAB time-to-1e-8 residual, mixed vs FP64, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+PMC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_two_spheres_163840.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_traffic(kernel_phase):
+    """HBM bytes per launch of the given phase from the committed rocprofv3
+    PMC summary (lines 'traffic_bytes_per_launch phase<k> <bytes>'), or None."""
+    if not os.path.exists(PMC_FILE):
+        return None
+    for line in open(PMC_FILE):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel_phase:
+            return float(parts[2])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--subdiv", type=int, default=6, help="icosphere subdivisions per sphere")
+    ap.add_argument("--quick", action="store_true", help="matvec only (for profiler runs)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as g
+    H = g.load_package()
+    L = H.hmv_lib()
+    if L.hmv_init(0) != 0:
+        raise SystemExit("no usable GPU: " + L.hmv_last_error().decode())
+    dev_name, ncu = H.device_info()
+    log("device:", dev_name, "CUs", ncu)
+
+    # ---- workload ---------------------------------------------------------
+    t0 = time.time()
+    mesh = H.merge(H.icosphere(args.subdiv), H.icosphere(args.subdiv, 1.0, (3.0, 0.0, 0.0)))
+    hm = H.HMatrix(*mesh, H.baseline_params(aca_eps=1e-5))
+    t_build = time.time() - t0
+    nd = hm.nd
+    tab = hm.leaf_table()
+    lr, de = tab[tab[:, 0] == 1], tab[tab[:, 0] == 2]
+    E_lr = int(((lr[:, 3] + lr[:, 5]) * lr[:, 1]).sum())
+    E_d = int((de[:, 3] * de[:, 5]).sum())
+    log("H-matrix: nd=%d leaves=%d (LR %d, dense %d) E=%d built in %.1fs"
+        % (nd, hm.nlf, len(lr), len(de), E_lr + E_d, t_build))
+
+    t0 = time.time()
+    a64 = H.Arena(hm, "f64")
+    t_arena = time.time() - t0
+    st = a64.stats
+    alg_bytes = 8 * (E_lr + E_d) + 16 * nd
+    assert alg_bytes == st["alg_bytes"]
+    log("arena: %.2f GB (payload %.2f GB), %d+%d workgroups, built in %.1fs"
+        % (st["arena_bytes"] / 1e9, st["payload_bytes"] / 1e9, st["nwork_phase1"], st["nwork_phase2"], t_arena))
+
+    x = H.uniform(1, nd)
+    dx = H.DeviceBuffer.from_array(x)
+    dy = H.DeviceBuffer(8 * nd)
+
+    # ---- timed matvecs ----------------------------------------------------
+    for _ in range(args.warmup):
+        a64.apply(dx.ptr, dy.ptr, 0.0)
+    L.hmv_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a64.apply(dx.ptr, dy.ptr, 0.0)
+    L.hmv_device_synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = alg_bytes * args.steps / elapsed / 1e9
+    log("H-matvec FP64: %.3f ms/step, %.1f GB/s algorithmic" % (ms_per_step, value))
+
+    # per-phase HIP-event timing (events on the stream the kernels run on)
+    ms1, ms2, msa = a64.apply_timed(dx.ptr, dy.ptr, max(20, args.steps))
+    tslots = int(st["tslots"])
+    bytes_p1 = 8 * int((lr[:, 5] * lr[:, 1]).sum()) + 8 * nd + 8 * tslots
+    bytes_p2 = 8 * int((lr[:, 3] * lr[:, 1]).sum()) + 8 * E_d + 8 * tslots + 16 * nd
+    phases = {"phase1": (ms1, bytes_p1, "hmv_stream_kernel<double, 1> (t = V^T x)"),
+              "phase2": (ms2, bytes_p2, "hmv_stream_kernel<double, 2> (y = [A|U][x;t])")}
+    dom = max(phases, key=lambda k: phases[k][0])
+    dms, dbytes, dname = phases[dom]
+    achieved = dbytes / (dms * 1e-3) / 1e9
+    traffic = load_traffic(dom)
+    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "alg_bytes_per_launch": dbytes, "avg_launch_ms": round(dms, 5),
+                "phases_ms": {"phase1": round(ms1, 5), "phase2": round(ms2, 5), "apply": round(msa, 5)},
+                "apply_event_GBs": round(alg_bytes / (msa * 1e-3) / 1e9, 1)}
+    log("phases: p1 %.3f ms (%.1f GB/s), p2 %.3f ms (%.1f GB/s); dominant %s %.1f GB/s = %.1f%% of peak"
+        % (ms1, bytes_p1 / ms1 / 1e6, ms2, bytes_p2 / ms2 / 1e6, dom, achieved, 100 * achieved / HBM_PEAK_GBS))
+
+    # full-size parity of this very apply against the oracle (checker only)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    y = dy.download()
+    nthr = max(1, min(16, os.cpu_count() or 1))
+    y_ref = O.adot_lfmtx_hyp(hm, x, nthr)
+    parity = float(np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref))
+    log("parity vs oracle: %.3e" % parity)
+
+    result = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (icosphere meshes generated on the host, x ~ U[-1,1] seed 1)",
+        "config": {"workload": "two icospheres s=%d (r=1, centres 3 apart), %d unknowns, leaf 20, "
+                               "eta 2, ACA+ 1e-5, 1/(4 pi r) centroid kernel" % (args.subdiv, nd),
+                   "nd": nd, "leaves": hm.nlf, "lowrank_leaves": int(len(lr)), "dense_leaves": int(len(de)),
+                   "entries": E_lr + E_d, "alg_bytes_per_matvec": alg_bytes,
+                   "arena_bytes": int(st["arena_bytes"]), "device": dev_name},
+        "roofline": roofline,
+        "parity_rel_err_vs_oracle": parity,
+        "construction_s": round(t_build, 2), "arena_build_s": round(t_arena, 2),
+    }
+
+    if not args.quick:
+        # ---- BiCGSTAB time to 1e-8: FP64 vs mixed precision -------------------
+        b = H.uniform(1, nd)
+        solves = {}
+        u64, i64, h64 = a64.bicgstab(b, eps=1e-8, maxit=2000, check_every=8)   # warm (graph capture)
+        u64, i64, h64 = a64.bicgstab(b, eps=1e-8, maxit=2000, check_every=8)
+        solves["fp64"] = {"seconds": round(i64["seconds"], 4), "iterations": i64["nstp"],
+                          "true_relres": i64["true_relres"], "status": i64["status_name"]}
+        log("BiCGSTAB FP64: %d it, %.3f s, true relres %.2e" % (i64["nstp"], i64["seconds"], i64["true_relres"]))
+        for prec in ("f32", "bf16"):
+            alow = H.Arena(hm, prec)
+            H.bicgstab_ir(a64, alow, b, eps=1e-8, inner_eps=1e-4, max_outer=40, max_inner=2000)  # warm
+            u, inf, hist = H.bicgstab_ir(a64, alow, b, eps=1e-8, inner_eps=1e-4, max_outer=40, max_inner=2000)
+            err = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+            solves["mixed_" + prec] = {
+                "seconds": round(inf["seconds"], 4), "outer": inf["outer"], "inner_iterations": inf["inner_total"],
+                "true_relres": inf["relres"], "status": inf["status_name"],
+                "rel_err_vs_fp64_solution": err, "outer_residuals": [float(v) for v in hist],
+                "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3),
+                "arena_bytes": int(alow.stats["arena_bytes"])}
+            log("BiCGSTAB-IR %s: outer %d inner %d, %.3f s (%.2fx FP64), relres %.2e, err vs FP64 sol %.2e"
+                % (prec, inf["outer"], inf["inner_total"], inf["seconds"], inf["seconds"] / i64["seconds"],
+                   inf["relres"], err))
+            alow.free()
+        result["bicgstab"] = solves
+
+        # ---- CPU baseline: bounded sample on this host ------------------------
+        O.adot_lfmtx_hyp(hm, x, nthr)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            O.adot_lfmtx_hyp(hm, x, nthr)
+            n += 1
+            el = time.perf_counter() - t0
+            if el > args.cpu_seconds or n >= 400:
+                break
+        cpu_gbs = alg_bytes * n / el / 1e9
+        result["cpu_baseline"] = {"value": round(cpu_gbs, 2), "unit": "GB/s", "cores": nthr, "kind": "port",
+                                  "sample": "%d FP64 H-matvecs of the same %d-unknown H-matrix "
+                                            "(oracle OpenMP per-leaf GEMV, %.1f s)" % (n, nd, el),
+                                  "ms_per_matvec": round(1e3 * el / n, 3)}
+        log("CPU baseline: %d threads, %.1f GB/s (%.2f ms/matvec)" % (nthr, cpu_gbs, 1e3 * el / n))
+
+        # ---- leaf histograms ---------------------------------------------------
+        result["histograms"] = {
+            "lowrank_rank": np.bincount(lr[:, 1]).tolist(),
+            "leaf_rows_log2_bins": {"lowrank": np.bincount(np.log2(lr[:, 3]).astype(int)).tolist(),
+                                    "dense": np.bincount(np.log2(de[:, 3]).astype(int)).tolist()},
+        }
+    else:
+        result["cpu_baseline"] = None
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,582 @@
+"""Host-side Python mirror of the H-matvec engine (ctypes over the C ABI).
+
+The compute path is libhmv.so (HIP, gfx950); this module only loads it,
+describes its structs and moves host arrays in and out.  There is no CPU
+fallback: every compute call raises HmvError when the library or the GPU is
+missing.
+
+Layers exposed here:
+  * mesh helpers            icosphere(), merge(), read_pbf()          (libhmvgen)
+  * H-matrix construction   HParams, HMatrix                          (libhmvgen)
+  * engine                  Arena (arena build, apply, BiCGSTAB, IR)  (libhmv)
+  * reference-named calls   adot_body_lfcpy_batch_sorted, adot_body_lfmtx_batch,
+                            adot_body_lfdel_batch, bicgstab_cax_lfmtx_gpu
+                            (HACApK_BATCH_SORT.c:86, HACApK_BATCH.c:257,812,
+                             HACApK_SOLVER.c:421) and solve() (m_HACApK_use.f90:190-262)
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_LIBDIR = os.path.join(_PKG, "lib")
+
+PREC = {"f64": 0, "f32": 1, "bf16": 2}
+KERNEL = {"ppohbem": 0, "centroid": 1}
+SOLVE_STATUS = {0: "converged", 1: "maxiter", 2: "breakdown", 3: "nonfinite"}
+
+
+class HmvError(RuntimeError):
+    pass
+
+
+# ---------------------------------------------------------------------------
+# structs
+# ---------------------------------------------------------------------------
+class LeafRec(C.Structure):
+    """stc_HACApK_leafmtx (HACApK_MAGMA.h:12-20)."""
+    _fields_ = [("ltmtx", C.c_int), ("kt", C.c_int), ("nstrtl", C.c_int), ("ndl", C.c_int),
+                ("nstrtt", C.c_int), ("ndt", C.c_int), ("a1size", C.c_longlong),
+                ("a1", C.POINTER(C.c_double)), ("a2", C.POINTER(C.c_double))]
+
+
+class _GenParams(C.Structure):
+    _fields_ = [("leaf_size", C.c_int), ("max_leaf_frac", C.c_double), ("eta", C.c_double),
+                ("aca_kind", C.c_int), ("aca_norm", C.c_int), ("kt_predict", C.c_int),
+                ("kmax", C.c_int), ("kmin", C.c_int), ("aca_eps", C.c_double),
+                ("aca_eps_factor", C.c_double), ("kernel", C.c_int), ("nthreads", C.c_int)]
+
+
+class _GenStats(C.Structure):
+    _fields_ = [("nd", C.c_int), ("nclst", C.c_int), ("nlf", C.c_int), ("nlfkt", C.c_int),
+                ("nlfdense", C.c_int), ("nhblocks", C.c_int), ("ktmax", C.c_int), ("pad0", C.c_int),
+                ("ncpc", C.c_int64), ("mem_entries", C.c_int64), ("mem_mbyte", C.c_double),
+                ("time_tree", C.c_double), ("time_fill", C.c_double)]
+
+
+class _ArenaStats(C.Structure):
+    _fields_ = [("nd", C.c_int), ("precision", C.c_int), ("nlf", C.c_int64), ("nlr", C.c_int64),
+                ("ndense", C.c_int64), ("entries_lr", C.c_int64), ("entries_dense", C.c_int64),
+                ("tslots", C.c_int64), ("arena_bytes", C.c_int64), ("payload_bytes", C.c_int64),
+                ("alg_bytes", C.c_int64), ("col_groups", C.c_int64), ("row_groups", C.c_int64),
+                ("atoms", C.c_int64), ("nwork_phase1", C.c_int64), ("nwork_phase2", C.c_int64),
+                ("ktmax", C.c_int), ("pad", C.c_int), ("build_seconds", C.c_double)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("nstp", C.c_int), ("status", C.c_int), ("relres", C.c_double),
+                ("true_relres", C.c_double), ("seconds", C.c_double), ("outer", C.c_int),
+                ("inner_total", C.c_int)]
+
+    def as_dict(self):
+        d = {f[0]: getattr(self, f[0]) for f in self._fields_}
+        d["status_name"] = SOLVE_STATUS.get(self.status, "?")
+        return d
+
+
+class _PlanTables(C.Structure):
+    _fields_ = [("arena", C.c_void_p), ("arena_bytes", C.c_int64), ("seg1", C.c_void_p),
+                ("nseg1", C.c_int64), ("work1", C.c_void_p), ("nwork1", C.c_int64),
+                ("seg2", C.c_void_p), ("nseg2", C.c_int64), ("work2", C.c_void_p),
+                ("nwork2", C.c_int64), ("slotmap", C.c_void_p), ("nslot", C.c_int64),
+                ("zidx", C.c_void_p), ("nzidx", C.c_int64), ("tsize", C.c_int64),
+                ("seg_bytes", C.c_int), ("work_bytes", C.c_int)]
+
+
+SEG_DTYPE = np.dtype([("data_off", "<i8"), ("ld", "<i4"), ("ncols", "<i4"), ("nrows", "<i4"),
+                      ("acc_row", "<i4"), ("nx", "<i4"), ("xsrc", "<i4"), ("tsrc", "<i4"),
+                      ("dtype", "<i2"), ("lpr", "<i2"), ("pad0", "<i4"), ("pad1", "<i4")])
+WORK_DTYPE = np.dtype([("seg_begin", "<i4"), ("seg_end", "<i4"), ("out_base", "<i4"), ("nout", "<i4")])
+
+
+class RefLeafmtxp(C.Structure):
+    """C view of st_HACApK_leafmtxp built without HAVE_MAGMA (m_HACApK_base.f90:64-76)."""
+    _fields_ = [("nd", C.c_int), ("nlf", C.c_int), ("nlfkt", C.c_int), ("ktmax", C.c_int),
+                ("st_lf_stride", C.c_int), ("st_lf", C.c_void_p)]
+
+
+class RefLcontrol(C.Structure):
+    """stc_HACApK_lcontrol (HACApK_MAGMA.h:108-121)."""
+    _fields_ = [("lf_umpi", C.c_int), ("lpmd_offset", C.c_int), ("lod_offset", C.c_int),
+                ("lsp_offset", C.c_int), ("lnp_offset", C.c_int), ("lthr_offset", C.c_int),
+                ("param", C.POINTER(C.c_double))]
+
+
+# ---------------------------------------------------------------------------
+# library loading
+# ---------------------------------------------------------------------------
+_gen = None
+_hmv = None
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+def _load(name):
+    path = os.path.join(_LIBDIR, name)
+    if not os.path.exists(path):
+        raise HmvError("%s is not built (run hacapk-magma_amd/build.py)" % path)
+    return C.CDLL(path)
+
+
+def gen_lib():
+    global _gen
+    if _gen is None:
+        L = _load("libhmvgen.so")
+        L.hmvgen_default_params.argtypes = [C.POINTER(_GenParams)]
+        L.hmvgen_icosphere.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       _I, _I, _D, _I]
+        L.hmvgen_read_pbf.argtypes = [C.c_char_p, _I, _I, _D, _I, _D]
+        L.hmvgen_build.argtypes = [C.c_int, _D, C.c_int, _I, C.POINTER(_GenParams),
+                                   C.POINTER(C.c_void_p)]
+        L.hmvgen_free.argtypes = [C.c_void_p]
+        L.hmvgen_get_stats.argtypes = [C.c_void_p, C.POINTER(_GenStats)]
+        L.hmvgen_leaves.argtypes = [C.c_void_p, _I, _I]
+        L.hmvgen_leaves.restype = C.c_void_p
+        L.hmvgen_lod.argtypes = [C.c_void_p]
+        L.hmvgen_lod.restype = _I
+        L.hmvgen_centroids.argtypes = [C.c_void_p]
+        L.hmvgen_centroids.restype = _D
+        L.hmvgen_entry.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.hmvgen_entry.restype = C.c_double
+        L.hmvgen_face_integral.argtypes = [C.c_int, _D, _D, _D, C.c_double, C.c_double, C.c_double]
+        L.hmvgen_face_integral.restype = C.c_double
+        L.hmvgen_uniform.argtypes = [C.c_uint64, C.c_int64, _D]
+        L.hmvgen_util_unrm.argtypes = [C.c_int, _D]
+        L.hmvgen_util_unrm.restype = C.c_double
+        L.hmvgen_util_maxabs.argtypes = [C.c_int, _D, _D]
+        L.hmvgen_util_minabs.argtypes = [C.c_int, _D, _D]
+        L.hmvgen_last_error.restype = C.c_char_p
+        _gen = L
+    return _gen
+
+
+def hmv_lib():
+    global _hmv
+    if _hmv is None:
+        L = _load("libhmv.so")
+        vp = C.c_void_p
+        L.hmv_init.argtypes = [C.c_int]
+        L.hmv_arena_build.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.POINTER(vp)]
+        L.hmv_arena_build_tuned.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int64,
+                                            C.c_int64, C.c_int, C.POINTER(vp)]
+        L.hmv_arena_stats_get.argtypes = [vp, C.POINTER(_ArenaStats)]
+        L.hmv_arena_free.argtypes = [vp]
+        L.hmv_arena_tbytes.argtypes = [vp]
+        L.hmv_arena_tbytes.restype = C.c_int64
+        L.hmv_apply.argtypes = [vp, vp, vp, C.c_double, vp]
+        L.hmv_apply_host.argtypes = [vp, _D, _D]
+        L.hmv_bicgstab.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, _D,
+                                   C.POINTER(SolveInfo)]
+        L.hmv_bicgstab_ir.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      _D, C.POINTER(SolveInfo)]
+        L.hmv_last_error.restype = C.c_char_p
+        L.hmv_device_malloc.argtypes = [C.POINTER(vp), C.c_size_t]
+        L.hmv_device_free.argtypes = [vp]
+        L.hmv_memcpy_htod.argtypes = [vp, vp, C.c_size_t]
+        L.hmv_memcpy_dtoh.argtypes = [vp, vp, C.c_size_t]
+        L.hmv_device_info.argtypes = [C.c_char_p, C.c_int, _I]
+        L.hmv_apply_timed.argtypes = [vp, vp, vp, C.c_int, _D, _D, _D]
+        L.hmv_plan_create.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.POINTER(vp)]
+        L.hmv_plan_view.argtypes = [vp, C.POINTER(_PlanTables)]
+        L.hmv_plan_free.argtypes = [vp]
+        L.c_hacapk_adot_body_lfcpy_batch_sorted_.argtypes = [_I, C.POINTER(RefLeafmtxp)]
+        L.c_hacapk_adot_body_lfcpy_batch_sorted_.restype = None
+        L.c_hacapk_adot_body_lfmtx_batch_.argtypes = [_D, C.POINTER(RefLeafmtxp), _D, _D, _D, _D, _D]
+        L.c_hacapk_adot_body_lfmtx_batch_.restype = None
+        L.c_hacapk_adot_body_lfdel_batch_.argtypes = [C.POINTER(RefLeafmtxp)]
+        L.c_hacapk_adot_body_lfdel_batch_.restype = None
+        L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes = [C.POINTER(RefLeafmtxp),
+                                                       C.POINTER(RefLcontrol), _D, _D, _D, _I, _I, _I]
+        L.c_hacapk_bicgstab_cax_lfmtx_gpu_.restype = None
+        _hmv = L
+    return _hmv
+
+
+def _check_gen(rc, what):
+    if rc != 0:
+        raise HmvError("%s: %s" % (what, gen_lib().hmvgen_last_error().decode()))
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise HmvError("%s: %s" % (what, hmv_lib().hmv_last_error().decode()))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+# ---------------------------------------------------------------------------
+# meshes
+# ---------------------------------------------------------------------------
+def icosphere(subdiv, radius=1.0, center=(0.0, 0.0, 0.0)):
+    """Icosahedron subdivided `subdiv` times onto a sphere: (nodes (n,3), faces (m,3) 0-based)."""
+    L = gen_lib()
+    nn, nf = C.c_int(), C.c_int()
+    _check_gen(L.hmvgen_icosphere(subdiv, radius, *center, C.byref(nn), C.byref(nf), None, None),
+               "icosphere")
+    nodes = np.empty((nn.value, 3), np.float64)
+    faces = np.empty((nf.value, 3), np.int32)
+    _check_gen(L.hmvgen_icosphere(subdiv, radius, *center, C.byref(nn), C.byref(nf),
+                                  _dptr(nodes), faces.ctypes.data_as(_I)), "icosphere")
+    return nodes, faces
+
+
+def merge(*meshes):
+    """Concatenates (nodes, faces) meshes into one."""
+    nodes, faces, off = [], [], 0
+    for n, f in meshes:
+        nodes.append(n)
+        faces.append(f + off)
+        off += n.shape[0]
+    return np.ascontiguousarray(np.vstack(nodes)), np.ascontiguousarray(np.vstack(faces).astype(np.int32))
+
+
+def read_pbf(path):
+    """ppohBEM mesh file -> (nodes, faces, first double parameter per face)."""
+    L = gen_lib()
+    nn, nf = C.c_int(), C.c_int()
+    _check_gen(L.hmvgen_read_pbf(path.encode(), C.byref(nn), C.byref(nf), None, None, None), "read_pbf")
+    nodes = np.empty((nn.value, 3), np.float64)
+    faces = np.empty((nf.value, 3), np.int32)
+    para = np.empty(nf.value, np.float64)
+    _check_gen(L.hmvgen_read_pbf(path.encode(), C.byref(nn), C.byref(nf), _dptr(nodes),
+                                 faces.ctypes.data_as(_I), _dptr(para)), "read_pbf")
+    return nodes, faces, para
+
+
+def uniform(seed, n):
+    """Deterministic i.i.d. uniform[-1,1) vector (splitmix64 stream)."""
+    out = np.empty(int(n), np.float64)
+    gen_lib().hmvgen_uniform(C.c_uint64(seed), int(n), _dptr(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# H-matrix construction
+# ---------------------------------------------------------------------------
+class HParams:
+    """Construction parameters; names follow the reference's param() slots
+    (defaults m_HACApK_base.f90:294-313)."""
+
+    def __init__(self, **kw):
+        p = _GenParams()
+        gen_lib().hmvgen_default_params(C.byref(p))
+        self._p = p
+        for k, v in kw.items():
+            self[k] = v
+
+    def __setitem__(self, k, v):
+        if k == "kernel" and isinstance(v, str):
+            v = KERNEL[v]
+        if not hasattr(self._p, k):
+            raise KeyError(k)
+        setattr(self._p, k, v)
+
+    def __getitem__(self, k):
+        return getattr(self._p, k)
+
+    def as_dict(self):
+        return {f[0]: getattr(self._p, f[0]) for f in _GenParams._fields_}
+
+
+def baseline_params(aca_eps=1e-4, leaf_size=20, eta=2.0, kernel="centroid", nthreads=0):
+    """BEM configuration of the H-matvec benchmarks: 1/(4 pi r) kernel with
+    centroid quadrature off the diagonal, analytic self terms."""
+    return HParams(leaf_size=leaf_size, eta=eta, aca_eps=aca_eps, kernel=kernel, nthreads=nthreads)
+
+
+class HMatrix:
+    """An H-matrix built on the host: leaves in the reference's record layout."""
+
+    def __init__(self, nodes, faces, params=None):
+        L = gen_lib()
+        nodes = np.ascontiguousarray(nodes, np.float64)
+        faces = np.ascontiguousarray(faces, np.int32)
+        params = params or HParams()
+        h = C.c_void_p()
+        _check_gen(L.hmvgen_build(nodes.shape[0], _dptr(nodes), faces.shape[0],
+                                  faces.ctypes.data_as(_I), C.byref(params._p), C.byref(h)),
+                   "hmvgen_build")
+        self._h = h
+        self.params = params
+        st = _GenStats()
+        _check_gen(L.hmvgen_get_stats(h, C.byref(st)), "stats")
+        self.stats = {f[0]: getattr(st, f[0]) for f in _GenStats._fields_ if not f[0].startswith("pad")}
+        self.nd = st.nd
+        nlf, stride = C.c_int(), C.c_int()
+        self.leaves_ptr = L.hmvgen_leaves(h, C.byref(nlf), C.byref(stride))
+        self.nlf, self.stride = nlf.value, stride.value
+        lod = L.hmvgen_lod(h)
+        self.lod = np.ctypeslib.as_array(lod, shape=(self.nd,)).copy()
+        cen = L.hmvgen_centroids(h)
+        self.centroids = np.ctypeslib.as_array(cen, shape=(self.nd, 3)).copy()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            gen_lib().hmvgen_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def entry(self, i, j):
+        """A(i,j) for original 1-based indices (HACApK_entry_ij)."""
+        return gen_lib().hmvgen_entry(self._h, i, j)
+
+    def leaves(self):
+        """Leaf records as a ctypes array view (no copy)."""
+        return (LeafRec * self.nlf).from_address(self.leaves_ptr)
+
+    def leaf_table(self):
+        """numpy table of (ltmtx, kt, nstrtl, ndl, nstrtt, ndt) per leaf."""
+        arr = self.leaves()
+        return np.array([(l.ltmtx, l.kt, l.nstrtl, l.ndl, l.nstrtt, l.ndt) for l in arr], np.int64)
+
+    def to_cluster_order(self, v):
+        """Original order -> cluster order (b(:nd)=rhs(lod(:nd)), m_HACApK_use.f90:285)."""
+        return np.ascontiguousarray(np.asarray(v)[self.lod - 1])
+
+    def to_original_order(self, v):
+        """Cluster order -> original order (sol(lod(:nd))=u(:nd), m_HACApK_use.f90:469)."""
+        out = np.empty_like(v)
+        out[self.lod - 1] = v
+        return out
+
+
+# ---------------------------------------------------------------------------
+# device buffers
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    def __init__(self, nbytes):
+        L = hmv_lib()
+        p = C.c_void_p()
+        _check(L.hmv_device_malloc(C.byref(p), int(nbytes)), "device malloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a, np.float64)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        _check(hmv_lib().hmv_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "htod")
+
+    def download(self, n=None):
+        n = self.nbytes // 8 if n is None else n
+        out = np.empty(n, np.float64)
+        _check(hmv_lib().hmv_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), "dtoh")
+        return out
+
+    def free(self):
+        if self.ptr:
+            hmv_lib().hmv_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_info():
+    L = hmv_lib()
+    buf = C.create_string_buffer(256)
+    cu = C.c_int()
+    _check(L.hmv_device_info(buf, 256, C.byref(cu)), "device info")
+    return buf.value.decode(), cu.value
+
+
+# ---------------------------------------------------------------------------
+# engine
+# ---------------------------------------------------------------------------
+class Arena:
+    """Device-resident arena of an H-matrix (hmv_arena_build)."""
+
+    def __init__(self, hm, precision="f64", nd=None, nlf=None, leaves_ptr=None, stride=None,
+                 tuning=None):
+        L = hmv_lib()
+        if hm is not None:
+            nd, nlf, leaves_ptr, stride = hm.nd, hm.nlf, hm.leaves_ptr, hm.stride
+        self.nd = nd
+        self.precision = precision
+        a = C.c_void_p()
+        if tuning:
+            _check(L.hmv_arena_build_tuned(nd, nlf, leaves_ptr, stride, PREC[precision],
+                                           int(tuning.get("p1_target_bytes", 0)),
+                                           int(tuning.get("p2_target_bytes", 0)),
+                                           int(tuning.get("lpr_bias", 0)), C.byref(a)),
+                   "hmv_arena_build_tuned")
+        else:
+            _check(L.hmv_arena_build(nd, nlf, leaves_ptr, stride, PREC[precision], C.byref(a)),
+                   "hmv_arena_build")
+        self._a = a
+        st = _ArenaStats()
+        _check(L.hmv_arena_stats_get(a, C.byref(st)), "arena stats")
+        self.stats = {f[0]: getattr(st, f[0]) for f in _ArenaStats._fields_ if f[0] != "pad"}
+
+    @property
+    def handle(self):
+        return self._a
+
+    def free(self):
+        if getattr(self, "_a", None):
+            hmv_lib().hmv_arena_free(self._a)
+            self._a = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def apply(self, x_ptr, y_ptr, beta=0.0, stream=None):
+        """y = H x + beta y on device pointers (asynchronous)."""
+        _check(hmv_lib().hmv_apply(self._a, x_ptr, y_ptr, float(beta), stream), "hmv_apply")
+
+    def matvec(self, x):
+        """Host convenience: returns H x (x, result in cluster order)."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros_like(x)
+        _check(hmv_lib().hmv_apply_host(self._a, _dptr(x), _dptr(y)), "hmv_apply_host")
+        return y
+
+    def apply_timed(self, x_ptr, y_ptr, reps):
+        m1, m2, ma = C.c_double(), C.c_double(), C.c_double()
+        _check(hmv_lib().hmv_apply_timed(self._a, x_ptr, y_ptr, int(reps), C.byref(m1), C.byref(m2),
+                                         C.byref(ma)), "hmv_apply_timed")
+        return m1.value, m2.value, ma.value
+
+    def bicgstab(self, b, u0=None, eps=1e-8, maxit=500, check_every=1, history=True):
+        """BiCGSTAB on device; host b/u0 in cluster order.  Returns (u, info, hist)."""
+        b = np.ascontiguousarray(b, np.float64)
+        u0 = np.zeros_like(b) if u0 is None else np.ascontiguousarray(u0, np.float64)
+        db, du = DeviceBuffer.from_array(b), DeviceBuffer.from_array(u0)
+        hist = np.full(maxit + 1, np.nan) if history else None
+        info = SolveInfo()
+        _check(hmv_lib().hmv_bicgstab(self._a, db.ptr, du.ptr, eps, maxit, check_every,
+                                      _dptr(hist) if history else None, C.byref(info)), "hmv_bicgstab")
+        u = du.download()
+        db.free()
+        du.free()
+        if history:
+            hist = hist[: info.nstp + 1]
+        return u, info.as_dict(), hist
+
+
+def plan_tables(hm, precision="f64"):
+    """Host-side repack of an HMatrix (hmv_plan_create): numpy copies of the
+    arena image and the engine's segment/work tables.  Needs no device."""
+    L = hmv_lib()
+    p = C.c_void_p()
+    _check(L.hmv_plan_create(hm.nd, hm.nlf, hm.leaves_ptr, hm.stride, PREC[precision], C.byref(p)),
+           "hmv_plan_create")
+    try:
+        t = _PlanTables()
+        _check(L.hmv_plan_view(p, C.byref(t)), "hmv_plan_view")
+        assert t.seg_bytes == SEG_DTYPE.itemsize and t.work_bytes == WORK_DTYPE.itemsize
+
+        def arr(ptr, n, dt):
+            if n == 0:
+                return np.zeros(0, dt)
+            return np.frombuffer((C.c_char * (n * np.dtype(dt).itemsize)).from_address(ptr), dt).copy()
+
+        return {
+            "arena": arr(t.arena, t.arena_bytes, np.uint8),
+            "seg1": arr(t.seg1, t.nseg1, SEG_DTYPE), "work1": arr(t.work1, t.nwork1, WORK_DTYPE),
+            "seg2": arr(t.seg2, t.nseg2, SEG_DTYPE), "work2": arr(t.work2, t.nwork2, WORK_DTYPE),
+            "slotmap": arr(t.slotmap, t.nslot, np.int32), "zidx": arr(t.zidx, t.nzidx, np.int32),
+            "tsize": t.tsize,
+        }
+    finally:
+        L.hmv_plan_free(p)
+
+
+def bicgstab_ir(a64, alow, b, u0=None, eps=1e-8, inner_eps=1e-4, max_outer=30, max_inner=1000):
+    """Mixed-precision refinement (FP64 residual, low-precision inner BiCGSTAB)."""
+    b = np.ascontiguousarray(b, np.float64)
+    u0 = np.zeros_like(b) if u0 is None else np.ascontiguousarray(u0, np.float64)
+    db, du = DeviceBuffer.from_array(b), DeviceBuffer.from_array(u0)
+    hist = np.full(max_outer + 1, np.nan)
+    info = SolveInfo()
+    _check(hmv_lib().hmv_bicgstab_ir(a64.handle, alow.handle, db.ptr, du.ptr, eps, inner_eps,
+                                     max_outer, max_inner, _dptr(hist), C.byref(info)),
+           "hmv_bicgstab_ir")
+    u = du.download()
+    return u, info.as_dict(), hist[: info.outer + 1]
+
+
+# ---------------------------------------------------------------------------
+# reference-named entry points
+# ---------------------------------------------------------------------------
+def make_leafmtxp(hm):
+    """st_HACApK_leafmtxp view of an HMatrix for the reference-named calls."""
+    t = hm.leaf_table()
+    lp = RefLeafmtxp()
+    lp.nd = hm.nd
+    lp.nlf = hm.nlf
+    lp.nlfkt = int((t[:, 0] == 1).sum()) if len(t) else 0
+    lp.ktmax = int(t[t[:, 0] == 1, 1].max()) if lp.nlfkt else 0
+    lp.st_lf_stride = hm.stride
+    lp.st_lf = hm.leaves_ptr
+    lp._owner = hm
+    return lp
+
+
+def adot_body_lfcpy_batch_sorted(nd, leafmtxp):
+    """c_hacapk_adot_body_lfcpy_batch_sorted_ (HACApK_BATCH_SORT.c:86)."""
+    hmv_lib().c_hacapk_adot_body_lfcpy_batch_sorted_(C.byref(C.c_int(nd)), C.byref(leafmtxp))
+
+
+def adot_body_lfmtx_batch(zau, leafmtxp, zu):
+    """c_hacapk_adot_body_lfmtx_batch_ (HACApK_BATCH.c:257): zau += H zu (in place)."""
+    assert zau.dtype == np.float64 and zau.flags.c_contiguous
+    zu = np.ascontiguousarray(zu, np.float64)
+    tb, ts, tc = C.c_double(0), C.c_double(0), C.c_double(0)
+    hmv_lib().c_hacapk_adot_body_lfmtx_batch_(_dptr(zau), C.byref(leafmtxp), _dptr(zu), None,
+                                              C.byref(tb), C.byref(ts), C.byref(tc))
+    return tb.value
+
+
+def adot_body_lfdel_batch(leafmtxp):
+    """c_hacapk_adot_body_lfdel_batch_ (HACApK_BATCH.c:812)."""
+    hmv_lib().c_hacapk_adot_body_lfdel_batch_(C.byref(leafmtxp))
+
+
+def bicgstab_cax_lfmtx_gpu(leafmtxp, u, b, param):
+    """c_hacapk_bicgstab_cax_lfmtx_gpu_ (HACApK_SOLVER.c:421): u updated in place.
+    param is the 100-slot Fortran param array seen 0-based (param[82] max
+    steps, param[90] tolerance).  Returns (nstp, lrtrn)."""
+    assert u.dtype == np.float64 and u.flags.c_contiguous
+    b = np.ascontiguousarray(b, np.float64)
+    param = np.ascontiguousarray(param, np.float64)
+    ctl = RefLcontrol()
+    ctl.param = _dptr(param)
+    nd, nstp, lrtrn = C.c_int(len(u)), C.c_int(0), C.c_int(0)
+    hmv_lib().c_hacapk_bicgstab_cax_lfmtx_gpu_(C.byref(leafmtxp), C.byref(ctl), _dptr(u), _dptr(b),
+                                               _dptr(param), C.byref(nd), C.byref(nstp),
+                                               C.byref(lrtrn))
+    return nstp.value, lrtrn.value
+
+
+def solve(hm, rhs, sol0=None, ztol=1e-8, mstep=500, arena=None):
+    """HACApK_solve (m_HACApK_use.f90:190-262) on the engine: permute with lod,
+    BiCGSTAB on the device, un-permute.  Returns (sol, info, hist)."""
+    own = arena is None
+    a = arena or Arena(hm, "f64")
+    b = hm.to_cluster_order(rhs)
+    u0 = None if sol0 is None else hm.to_cluster_order(sol0)
+    u, info, hist = a.bicgstab(b, u0, eps=ztol, maxit=mstep)
+    if own:
+        a.free()
+    return hm.to_original_order(u), info, hist

@@ -1,0 +1,498 @@
+// hmv_api.cpp -- C ABI of the H-matvec engine: arena lifecycle, apply, and
+// the reference-named entry points a Fortran driver links against.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hmv.h"
+#include "hmv_engine.h"
+#include "hmv_plan.h"
+
+namespace hmv {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char *what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return (e == hipErrorOutOfMemory) ? HMV_ERR_MEMORY : HMV_ERR_DEVICE;
+}
+
+hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta, const int *skip,
+                         hipStream_t s) {
+  return hmv_launch_apply(a->acc64, a->d_arena, a->d_seg1, a->d_work1, a->nwork1, a->d_seg2,
+                          a->d_work2, a->nwork2, a->d_slotmap, a->d_zidx, a->d_t, x, y, beta, skip,
+                          s);
+}
+
+}  // namespace hmv
+
+using hmv::hip_fail;
+using hmv::set_error;
+
+namespace {
+
+int check_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) {
+    set_error("no HIP device available (the engine has no CPU path)");
+    return HMV_ERR_DEVICE;
+  }
+  return HMV_OK;
+}
+
+template <typename T>
+hipError_t upload(T **dst, const std::vector<T> &src) {
+  *dst = nullptr;
+  if (src.empty()) return hipSuccess;
+  hipError_t e = hipMalloc((void **)dst, sizeof(T) * src.size());
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice);
+}
+
+void release(hmv_arena_s *a) {
+  if (!a) return;
+  hmv::free_solve_work(a);
+  (void)hipFree(a->d_arena);
+  (void)hipFree(a->d_seg1);
+  (void)hipFree(a->d_seg2);
+  (void)hipFree(a->d_work1);
+  (void)hipFree(a->d_work2);
+  (void)hipFree(a->d_slotmap);
+  (void)hipFree(a->d_zidx);
+  (void)hipFree(a->d_t);
+  (void)hipFree(a->d_hx);
+  (void)hipFree(a->d_hy);
+  delete a;
+}
+
+int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::PlanTuning &tune,
+          hmv_arena *out) {
+  if (!out) {
+    set_error("hmv_arena_build: null output handle");
+    return HMV_ERR_ARG;
+  }
+  *out = nullptr;
+  int rc = check_device();
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  hmv::Plan P;
+  std::string err;
+  if (!hmv::build_plan(nd, nlf, leaves, stride, prec, tune, P, err)) {
+    set_error("hmv_arena_build: " + err);
+    return HMV_ERR_ARG;
+  }
+  hmv_arena_s *a = new hmv_arena_s();
+  hipGetDevice(&a->device);
+  a->nd = nd;
+  a->prec = prec;
+  a->acc64 = prec == HMV_PREC_F64 ? 1 : 0;
+  a->tsize = P.tsize;
+  a->nwork1 = (int)P.work1.size();
+  a->nwork2 = (int)P.work2.size();
+  hipError_t e;
+  if ((e = upload(&a->d_arena, P.arena)) != hipSuccess ||
+      (e = upload(&a->d_seg1, P.seg1)) != hipSuccess ||
+      (e = upload(&a->d_seg2, P.seg2)) != hipSuccess ||
+      (e = upload(&a->d_work1, P.work1)) != hipSuccess ||
+      (e = upload(&a->d_work2, P.work2)) != hipSuccess ||
+      (e = upload(&a->d_slotmap, P.slotmap)) != hipSuccess ||
+      (e = upload(&a->d_zidx, P.zidx)) != hipSuccess) {
+    release(a);
+    return hip_fail(e, "hmv_arena_build: upload");
+  }
+  size_t tb = (size_t)std::max<int64_t>(1, P.tsize) * (a->acc64 ? 8 : 4);
+  if ((e = hipMalloc(&a->d_t, tb)) != hipSuccess) {
+    release(a);
+    return hip_fail(e, "hmv_arena_build: t vector");
+  }
+  if ((e = hipDeviceSynchronize()) != hipSuccess) {
+    release(a);
+    return hip_fail(e, "hmv_arena_build: sync");
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  hmv_arena_stats &s = a->stats;
+  s.nd = nd;
+  s.precision = prec;
+  s.nlf = P.st.nlf;
+  s.nlr = P.st.nlr;
+  s.ndense = P.st.ndense;
+  s.entries_lr = P.st.entries_lr;
+  s.entries_dense = P.st.entries_dense;
+  s.tslots = P.st.tslots;
+  s.arena_bytes = P.st.arena_bytes;
+  s.payload_bytes = P.st.payload_bytes;
+  s.alg_bytes = P.st.alg_bytes;
+  s.col_groups = P.st.col_groups;
+  s.row_groups = P.st.row_groups;
+  s.atoms = P.st.atoms;
+  s.nwork_phase1 = a->nwork1;
+  s.nwork_phase2 = a->nwork2;
+  s.ktmax = P.st.ktmax;
+  s.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+  *out = a;
+  return HMV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hmv_last_error(void) { return hmv::g_err.c_str(); }
+
+int hmv_init(int device) {
+  int rc = check_device();
+  if (rc) return rc;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(e, "hmv_init");
+  return HMV_OK;
+}
+
+int hmv_arena_build(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
+                    hmv_arena *out) {
+  return build(nd, nlf, leaves, leaf_stride, precision, hmv::PlanTuning(), out);
+}
+
+int hmv_arena_build_tuned(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
+                          int64_t p1_target_bytes, int64_t p2_target_bytes, int lpr_bias,
+                          hmv_arena *out) {
+  hmv::PlanTuning t;
+  if (p1_target_bytes > 0) t.p1_target_bytes = p1_target_bytes;
+  if (p2_target_bytes > 0) t.p2_target_bytes = p2_target_bytes;
+  if (lpr_bias < 0 || lpr_bias > 4) {
+    set_error("hmv_arena_build_tuned: lpr_bias must be in [0,4]");
+    return HMV_ERR_ARG;
+  }
+  t.lpr_bias = lpr_bias;
+  return build(nd, nlf, leaves, leaf_stride, precision, t, out);
+}
+
+int hmv_arena_stats_get(hmv_arena a, hmv_arena_stats *s) {
+  if (!a || !s) {
+    set_error("hmv_arena_stats_get: null argument");
+    return HMV_ERR_ARG;
+  }
+  *s = a->stats;
+  return HMV_OK;
+}
+
+int hmv_arena_free(hmv_arena a) {
+  if (!a) return HMV_OK;
+  release(a);
+  return HMV_OK;
+}
+
+int64_t hmv_arena_tbytes(hmv_arena a) {
+  if (!a) return -1;
+  return std::max<int64_t>(1, a->tsize) * (a->acc64 ? 8 : 4);
+}
+
+int hmv_apply(hmv_arena a, const double *x, double *y, double beta, void *stream) {
+  if (!a || !x || !y) {
+    set_error("hmv_apply: null argument");
+    return HMV_ERR_ARG;
+  }
+  if (beta != 0.0 && beta != 1.0) {
+    set_error("hmv_apply: beta must be 0 or 1");
+    return HMV_ERR_ARG;
+  }
+  hipError_t e = hmv::enqueue_apply(a, x, y, beta, nullptr, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "hmv_apply: launch");
+  return HMV_OK;
+}
+
+int hmv_apply_host(hmv_arena a, const double *zu, double *zau) {
+  if (!a || !zu || !zau) {
+    set_error("hmv_apply_host: null argument");
+    return HMV_ERR_ARG;
+  }
+  hipError_t e;
+  const size_t nb = sizeof(double) * (size_t)a->nd;
+  if (!a->d_hx) {
+    if ((e = hipMalloc(&a->d_hx, nb)) != hipSuccess) return hip_fail(e, "hmv_apply_host: alloc");
+    if ((e = hipMalloc(&a->d_hy, nb)) != hipSuccess) return hip_fail(e, "hmv_apply_host: alloc");
+  }
+  if ((e = hipMemcpy(a->d_hx, zu, nb, hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "hmv_apply_host: copy x");
+  if ((e = hipMemcpy(a->d_hy, zau, nb, hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(e, "hmv_apply_host: copy y");
+  if ((e = hmv::enqueue_apply(a, a->d_hx, a->d_hy, 1.0, nullptr, nullptr)) != hipSuccess)
+    return hip_fail(e, "hmv_apply_host: launch");
+  if ((e = hipMemcpy(zau, a->d_hy, nb, hipMemcpyDeviceToHost)) != hipSuccess)
+    return hip_fail(e, "hmv_apply_host: copy back");
+  return HMV_OK;
+}
+
+int hmv_device_malloc(void **p, size_t bytes) {
+  if (!p) {
+    set_error("hmv_device_malloc: null pointer");
+    return HMV_ERR_ARG;
+  }
+  *p = nullptr;
+  int rc = check_device();
+  if (rc) return rc;
+  hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+  if (e != hipSuccess) return hip_fail(e, "hmv_device_malloc");
+  return HMV_OK;
+}
+
+int hmv_device_free(void *p) {
+  hipError_t e = hipFree(p);
+  if (e != hipSuccess) return hip_fail(e, "hmv_device_free");
+  return HMV_OK;
+}
+
+int hmv_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(e, "hmv_memcpy_htod");
+  return HMV_OK;
+}
+
+int hmv_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "hmv_memcpy_dtoh");
+  return HMV_OK;
+}
+
+int hmv_device_synchronize(void) {
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "hmv_device_synchronize");
+  return HMV_OK;
+}
+
+int hmv_device_info(char *name, int name_len, int *compute_units) {
+  int rc = check_device();
+  if (rc) return rc;
+  int dev = 0;
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, dev);
+  if (e != hipSuccess) return hip_fail(e, "hmv_device_info");
+  if (name && name_len > 0) {
+    std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (compute_units) *compute_units = prop.multiProcessorCount;
+  return HMV_OK;
+}
+
+int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *ms_phase1,
+                    double *ms_phase2, double *ms_apply) {
+  if (!a || !x || !y || reps < 1) {
+    set_error("hmv_apply_timed: invalid argument");
+    return HMV_ERR_ARG;
+  }
+  hipStream_t s = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "hmv_apply_timed: stream");
+  std::vector<hipEvent_t> ev(3 * (size_t)reps, nullptr);
+  for (auto &v : ev)
+    if ((e = hipEventCreate(&v)) != hipSuccess) break;
+  double t1 = 0.0, t2 = 0.0, ta = 0.0;
+  if (e == hipSuccess) {
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+      hipEvent_t e0 = ev[3 * r], e1 = ev[3 * r + 1], e2 = ev[3 * r + 2];
+      hipEventRecord(e0, s);
+      if (a->nwork1 > 0)
+        e = hmv_launch_apply(a->acc64, a->d_arena, a->d_seg1, a->d_work1, a->nwork1, a->d_seg2,
+                             a->d_work2, 0, a->d_slotmap, a->d_zidx, a->d_t, x, y, 0.0, nullptr, s);
+      hipEventRecord(e1, s);
+      if (e == hipSuccess)
+        e = hmv_launch_apply(a->acc64, a->d_arena, a->d_seg1, a->d_work1, 0, a->d_seg2,
+                             a->d_work2, a->nwork2, a->d_slotmap, a->d_zidx, a->d_t, x, y, 0.0,
+                             nullptr, s);
+      hipEventRecord(e2, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+      float m1 = 0.f, m2 = 0.f;
+      hipEventElapsedTime(&m1, ev[3 * r], ev[3 * r + 1]);
+      hipEventElapsedTime(&m2, ev[3 * r + 1], ev[3 * r + 2]);
+      t1 += m1;
+      t2 += m2;
+      ta += (double)m1 + (double)m2;
+    }
+  }
+  for (auto &v : ev)
+    if (v) hipEventDestroy(v);
+  hipStreamDestroy(s);
+  if (e != hipSuccess) return hip_fail(e, "hmv_apply_timed");
+  if (ms_phase1) *ms_phase1 = a->nwork1 > 0 ? t1 / reps : 0.0;
+  if (ms_phase2) *ms_phase2 = t2 / reps;
+  if (ms_apply) *ms_apply = ta / reps;
+  return HMV_OK;
+}
+
+struct hmv_plan_s {
+  hmv::Plan P;
+};
+
+int hmv_plan_create(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
+                    hmv_plan *out) {
+  if (!out) {
+    set_error("hmv_plan_create: null output");
+    return HMV_ERR_ARG;
+  }
+  *out = nullptr;
+  hmv_plan p = new hmv_plan_s();
+  std::string err;
+  if (!hmv::build_plan(nd, nlf, leaves, leaf_stride, precision, hmv::PlanTuning(), p->P, err)) {
+    delete p;
+    set_error("hmv_plan_create: " + err);
+    return HMV_ERR_ARG;
+  }
+  *out = p;
+  return HMV_OK;
+}
+
+int hmv_plan_view(hmv_plan p, hmv_plan_tables *t) {
+  if (!p || !t) {
+    set_error("hmv_plan_view: null argument");
+    return HMV_ERR_ARG;
+  }
+  const hmv::Plan &P = p->P;
+  t->arena = P.arena.data();
+  t->arena_bytes = (int64_t)P.arena.size();
+  t->seg1 = P.seg1.data();
+  t->nseg1 = (int64_t)P.seg1.size();
+  t->work1 = P.work1.data();
+  t->nwork1 = (int64_t)P.work1.size();
+  t->seg2 = P.seg2.data();
+  t->nseg2 = (int64_t)P.seg2.size();
+  t->work2 = P.work2.data();
+  t->nwork2 = (int64_t)P.work2.size();
+  t->slotmap = P.slotmap.data();
+  t->nslot = (int64_t)P.slotmap.size();
+  t->zidx = P.zidx.data();
+  t->nzidx = (int64_t)P.zidx.size();
+  t->tsize = P.tsize;
+  t->seg_bytes = (int)sizeof(HmvSeg);
+  t->work_bytes = (int)sizeof(HmvWork);
+  return HMV_OK;
+}
+
+int hmv_plan_free(hmv_plan p) {
+  delete p;
+  return HMV_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reference-named entry points.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_tab_mu;
+std::map<const void *, hmv_arena> g_tab;
+
+hmv_arena lookup(const void *key) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  auto it = g_tab.find(key);
+  return it == g_tab.end() ? nullptr : it->second;
+}
+
+void report(const char *fn) {
+  std::fprintf(stderr, "%s: %s\n", fn, hmv_last_error());
+  std::fflush(stderr);
+}
+}  // namespace
+
+extern "C" {
+
+void c_hacapk_adot_body_lfcpy_batch_sorted_(int *nd, hmv_ref_leafmtxp *H) {
+  if (!nd || !H) return;
+  c_hacapk_adot_body_lfdel_batch_(H);
+  hmv_arena a = nullptr;
+  if (hmv_arena_build(*nd, H->nlf, H->st_lf, H->st_lf_stride, HMV_PREC_F64, &a) != HMV_OK) {
+    report("c_hacapk_adot_body_lfcpy_batch_sorted_");
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  g_tab[H] = a;
+}
+
+void c_hacapk_adot_body_lfmtx_batch_(double *zau, hmv_ref_leafmtxp *H, double *zu, double *zbu,
+                                     double *time_batch, double *time_set, double *time_copy) {
+  (void)zbu;
+  (void)time_set;
+  (void)time_copy;
+  hmv_arena a = lookup(H);
+  if (!a) {
+    set_error("c_hacapk_adot_body_lfmtx_batch_: arena not built (call lfcpy_batch_sorted first)");
+    report("c_hacapk_adot_body_lfmtx_batch_");
+    return;
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  if (hmv_apply_host(a, zu, zau) != HMV_OK) report("c_hacapk_adot_body_lfmtx_batch_");
+  auto t1 = std::chrono::steady_clock::now();
+  if (time_batch) *time_batch += std::chrono::duration<double>(t1 - t0).count();
+}
+
+void c_hacapk_adot_body_lfdel_batch_(hmv_ref_leafmtxp *H) {
+  hmv_arena a = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto it = g_tab.find(H);
+    if (it != g_tab.end()) {
+      a = it->second;
+      g_tab.erase(it);
+    }
+  }
+  if (a) hmv_arena_free(a);
+}
+
+void c_hacapk_bicgstab_cax_lfmtx_gpu_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C, double *u_cpu,
+                                      double *b_cpu, double *param, int *nd, int *nstp,
+                                      int *lrtrn) {
+  if (lrtrn) *lrtrn = HMV_ERR_ARG;
+  if (!H || !param || !nd || !u_cpu || !b_cpu) return;
+  const int mstep = (int)param[82];
+  const double eps = param[90];
+  const int verbose = (C && C->param) ? (int)C->param[0] : (int)param[0];
+  // The reference rebuilds the arena per call (HACApK_SOLVER.c:521,777); an
+  // arena built earlier through lfcpy_batch_sorted_ is reused instead.
+  hmv_arena a = lookup(H);
+  bool own = false;
+  if (!a) {
+    if (hmv_arena_build(*nd, H->nlf, H->st_lf, H->st_lf_stride, HMV_PREC_F64, &a) != HMV_OK) {
+      report("c_hacapk_bicgstab_cax_lfmtx_gpu_");
+      return;
+    }
+    own = true;
+  }
+  const size_t nb = sizeof(double) * (size_t)*nd;
+  double *du = nullptr, *db = nullptr;
+  int rc = HMV_OK;
+  hmv_solve_info info{};
+  if (hipMalloc(&du, nb) != hipSuccess || hipMalloc(&db, nb) != hipSuccess ||
+      hipMemcpy(du, u_cpu, nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(db, b_cpu, nb, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("c_hacapk_bicgstab_cax_lfmtx_gpu_: device vectors");
+    rc = HMV_ERR_DEVICE;
+  }
+  if (rc == HMV_OK) rc = hmv_bicgstab(a, db, du, eps, mstep, 1, nullptr, &info);
+  if (rc == HMV_OK && hipMemcpy(u_cpu, du, nb, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("c_hacapk_bicgstab_cax_lfmtx_gpu_: copy back");
+    rc = HMV_ERR_DEVICE;
+  }
+  if (rc != HMV_OK) report("c_hacapk_bicgstab_cax_lfmtx_gpu_");
+  if (nstp) *nstp = info.nstp;
+  if (lrtrn) *lrtrn = rc == HMV_OK ? info.status : rc;
+  if (verbose > 0 && rc == HMV_OK) {
+    std::printf(" BiCGSTAB (hmv): steps=%d relres=%.2e true relres=%.2e time=%.5e s\n", info.nstp,
+                info.relres, info.true_relres, info.seconds);
+    std::fflush(stdout);
+  }
+  (void)hipFree(du);
+  (void)hipFree(db);
+  if (own) hmv_arena_free(a);
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// hmv_plan.h -- host-side repack of a leaf list into the engine's arena.
+#ifndef HMV_PLAN_H
+#define HMV_PLAN_H
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hmv_internal.h"
+
+namespace hmv {
+
+// A leaf in the reference's record layout (stc_HACApK_leafmtx,
+// HACApK_MAGMA.h:12-20); a2 is located at (char*)a1 + a1size.
+struct RefLeaf {
+  int ltmtx;
+  int kt;
+  int nstrtl, ndl;
+  int nstrtt, ndt;
+  long long a1size;
+  double *a1;
+  double *a2;
+};
+
+// Storage precision of an arena.
+enum Precision { kF64 = 0, kF32 = 1, kBF16 = 2 };
+
+struct PlanStats {
+  int64_t nlf = 0, nlr = 0, ndense = 0, nkt0 = 0;
+  int64_t entries_lr = 0;     // sum_LR kt*(ndl+ndt)
+  int64_t entries_dense = 0;  // sum_D ndl*ndt
+  int64_t tslots = 0;         // sum_LR kt
+  int64_t col_groups = 0, row_groups = 0, atoms = 0;
+  int64_t arena_bytes = 0;    // bytes of the arena including padding
+  int64_t payload_bytes = 0;  // bytes of factor/dense entries at storage precision
+  int64_t alg_bytes = 0;      // payload + x read once + y written once (8 B each)
+  int ktmax = 0;
+  int max_ndt = 0;
+};
+
+struct Plan {
+  int nd = 0;
+  int prec = kF64;
+  int64_t tsize = 0;             // length of the t vector
+  std::vector<char> arena;       // device image of the arena
+  std::vector<HmvSeg> seg1, seg2;
+  std::vector<HmvWork> work1, work2;
+  std::vector<int> slotmap;      // phase-1 output row -> t index
+  std::vector<int> zidx;         // phase-2 gather indices into x
+  PlanStats st;
+};
+
+// Tunables of the repack (chosen by the autotuner or left at defaults).
+struct PlanTuning {
+  int64_t p1_target_bytes = 128 * 1024;  // payload per phase-1 workgroup
+  int64_t p2_target_bytes = 192 * 1024;  // payload per phase-2 workgroup
+  int p2_max_rows = HMV_MAX_ROWS;
+  int lpr_bias = 0;                      // +1 doubles the lanes per row chosen
+};
+
+// Builds the plan.  `leaves` points at nlf records spaced `stride` bytes.
+// Returns false and fills `err` on invalid input.
+bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
+                const PlanTuning &tune, Plan &out, std::string &err);
+
+int dtype_size(int dt);
+
+}  // namespace hmv
+
+#endif

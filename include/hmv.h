@@ -1,0 +1,220 @@
+/*
+ * hmv.h -- MI355X H-matrix matvec engine and BiCGSTAB solver (C ABI).
+ *
+ * Plain pointers and sizes only.  Device pointers are hipMalloc'd memory of
+ * the current device; `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  Every function returns 0 on success and a negative code on
+ * failure; hmv_last_error() gives the message of the calling thread's last
+ * failure.  There is no CPU fallback: without a usable gfx950 device every
+ * compute entry point fails with HMV_ERR_DEVICE.
+ *
+ * Vectors are in the cluster (permuted) order of the leaves, as in the
+ * reference (m_HACApK_use.f90:285,469 permute with lod on the host side).
+ *
+ * Two groups of entry points:
+ *   1. an explicit-array API (hmv_*), and
+ *   2. reference-named entry points with the reference's argument lists, for
+ *      a Fortran driver that links this library in place of the MAGMA build
+ *      (declared at the end of this file).
+ */
+#ifndef HMV_H
+#define HMV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HMV_OK 0
+#define HMV_ERR_ARG (-1)
+#define HMV_ERR_DEVICE (-2)
+#define HMV_ERR_MEMORY (-3)
+#define HMV_ERR_STATE (-4)
+
+/* Storage precision of an arena. */
+#define HMV_PREC_F64 0   /* factors and dense blocks FP64, FP64 accumulation       */
+#define HMV_PREC_F32 1   /* factors and dense blocks FP32, FP32 accumulation       */
+#define HMV_PREC_BF16 2  /* low-rank factors BF16, dense blocks FP32, FP32 accum.  */
+
+/* Leaf record, identical to stc_HACApK_leafmtx (HACApK_MAGMA.h:12-20) and to
+ * the Fortran st_HACApK_leafmtx as seen from C.  ltmtx 1 = low rank
+ * (a1 = V: ndt x kt, a2 = U: ndl x kt, column-major, block = U V^T),
+ * 2 = dense (a1 = ndt x ndl column-major, i.e. the block row-major).
+ * nstrtl/nstrtt are 1-based.  The engine reads a2 at (char*)a1 + a1size,
+ * like HACApK_MAGMA.c:27. */
+typedef struct {
+  int ltmtx;
+  int kt;
+  int nstrtl, ndl;
+  int nstrtt, ndt;
+  long long a1size;
+  double *a1;
+  double *a2;
+} hmv_leaf;
+
+typedef struct hmv_arena_s *hmv_arena;
+
+typedef struct {
+  int nd;                 /* unknowns                                          */
+  int precision;          /* HMV_PREC_*                                        */
+  int64_t nlf, nlr, ndense;
+  int64_t entries_lr;     /* sum_LR kt*(ndl+ndt)                               */
+  int64_t entries_dense;  /* sum_D ndl*ndt                                     */
+  int64_t tslots;         /* sum_LR kt (length of the t vector)                */
+  int64_t arena_bytes;    /* device bytes of factors incl. alignment padding   */
+  int64_t payload_bytes;  /* bytes of the stored entries                        */
+  int64_t alg_bytes;      /* payload + 16*nd (x read once, y written once)      */
+  int64_t col_groups, row_groups, atoms;
+  int64_t nwork_phase1, nwork_phase2;
+  int ktmax;
+  int pad;
+  double build_seconds;   /* host repack + upload                               */
+} hmv_arena_stats;
+
+/* Selects the device for the calling thread (hipSetDevice). */
+int hmv_init(int device);
+
+/* Repacks nlf leaves (records spaced leaf_stride bytes apart, stc_HACApK_leafmtx
+ * layout) of an nd x nd H-matrix into a device-resident arena. */
+int hmv_arena_build(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
+                    hmv_arena *out);
+
+/* Same, with explicit repack tunables (used by the autotuner):
+ * p1_target_bytes / p2_target_bytes: payload per workgroup of each phase
+ * (<=0: default), lpr_bias: 0 default, k doubles lanes-per-row k times. */
+int hmv_arena_build_tuned(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
+                          int64_t p1_target_bytes, int64_t p2_target_bytes, int lpr_bias,
+                          hmv_arena *out);
+
+int hmv_arena_stats_get(hmv_arena a, hmv_arena_stats *s);
+int hmv_arena_free(hmv_arena a);
+
+/* y = H*x + beta*y on device vectors (beta 0: y is overwritten, NaNs in y
+ * ignored).  Asynchronous on `stream`. */
+int hmv_apply(hmv_arena a, const double *x, double *y, double beta, void *stream);
+
+/* zau += H*zu on host vectors (the accumulate semantics of
+ * c_hacapk_adot_body_lfmtx_batch_, HACApK_BATCH.c:257-271); synchronous. */
+int hmv_apply_host(hmv_arena a, const double *zu, double *zau);
+
+/* Device work buffers needed by hmv_apply (t vector).  Returns bytes. */
+int64_t hmv_arena_tbytes(hmv_arena a);
+
+/* ---- BiCGSTAB ---------------------------------------------------------- */
+
+#define HMV_SOLVE_CONVERGED 0
+#define HMV_SOLVE_MAXITER 1
+#define HMV_SOLVE_BREAKDOWN 2
+#define HMV_SOLVE_NONFINITE 3
+
+typedef struct {
+  int nstp;            /* iterations taken (nstp of the reference)          */
+  int status;          /* HMV_SOLVE_*                                       */
+  double relres;       /* recursive ||r||/||b|| at exit                     */
+  double true_relres;  /* ||b - A u||/||b|| recomputed after the solve      */
+  double seconds;      /* wall time of the solve (device-synchronised)      */
+  int outer;           /* refinement steps (mixed-precision solve), else 0  */
+  int inner_total;     /* inner iterations summed over refinement steps     */
+} hmv_solve_info;
+
+/* Unpreconditioned BiCGSTAB (m_HACApK_solve.f90:342-449) on device vectors:
+ * u (initial guess in, solution out), b.  Stops when ||r||/||b|| < eps or
+ * after maxit steps.  hist (host, may be NULL) receives maxit+1 relative
+ * residuals (hist[0] initial).  check_every: residual read-back period in
+ * iterations (1 = every iteration, like the reference). */
+int hmv_bicgstab(hmv_arena a, const double *b, double *u, double eps, int maxit, int check_every,
+                 double *hist, hmv_solve_info *info);
+
+/* Mixed-precision iterative refinement: the FP64 residual r = b - A64 u, an
+ * inner BiCGSTAB with the low-precision arena `alow` to inner_eps, u += d;
+ * repeats until ||b - A64 u||/||b|| < eps or max_outer steps.
+ * hist (host, may be NULL) receives max_outer+1 outer relative residuals. */
+int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
+                    double inner_eps, int max_outer, int max_inner, double *hist,
+                    hmv_solve_info *info);
+
+const char *hmv_last_error(void);
+
+/* ---- device plumbing (buffers, copies, timing) ------------------------- */
+int hmv_device_malloc(void **p, size_t bytes);
+int hmv_device_free(void *p);
+int hmv_memcpy_htod(void *dst, const void *src, size_t bytes);
+int hmv_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int hmv_device_synchronize(void);
+/* Device name and compute units of the current device. */
+int hmv_device_info(char *name, int name_len, int *compute_units);
+/* Runs `reps` applies y = H x on an internal stream and returns the average
+ * duration of each phase launch and of the whole apply, measured with HIP
+ * events recorded on that stream around every launch (milliseconds).
+ * ms_phase1 is 0 when the arena has no low-rank leaves. */
+int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *ms_phase1,
+                    double *ms_phase2, double *ms_apply);
+
+/* ---- host-side plan inspection (no device needed) ----------------------
+ * The repack that hmv_arena_build uploads, exposed for inspection and for
+ * CPU-side validation of the arena's indexing.  Tables use the engine's
+ * internal record layouts: segment = {int64 data_off (bytes); int ld, ncols,
+ * nrows, acc_row, nx, xsrc, tsrc; int16 dtype, lpr; int pad[2]} (48 bytes),
+ * work = {int seg_begin, seg_end, out_base, nout} (16 bytes). */
+typedef struct hmv_plan_s *hmv_plan;
+typedef struct {
+  const void *arena;   int64_t arena_bytes;
+  const void *seg1;    int64_t nseg1;
+  const void *work1;   int64_t nwork1;
+  const void *seg2;    int64_t nseg2;
+  const void *work2;   int64_t nwork2;
+  const int *slotmap;  int64_t nslot;
+  const int *zidx;     int64_t nzidx;
+  int64_t tsize;
+  int seg_bytes, work_bytes;
+} hmv_plan_tables;
+int hmv_plan_create(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
+                    hmv_plan *out);
+int hmv_plan_view(hmv_plan p, hmv_plan_tables *t);
+int hmv_plan_free(hmv_plan p);
+
+/* ---- reference-named entry points --------------------------------------
+ * For a Fortran driver built WITHOUT HAVE_MAGMA: the C view of its
+ * st_HACApK_leafmtxp is then the five integers followed by the st_lf
+ * pointer (m_HACApK_base.f90:64-76 with the MAGMA block compiled out), and
+ * the device state lives in a side table keyed by the struct's address. */
+typedef struct {
+  int nd;
+  int nlf;
+  int nlfkt;
+  int ktmax;
+  int st_lf_stride;
+  hmv_leaf *st_lf;
+} hmv_ref_leafmtxp;
+
+typedef struct {
+  int lf_umpi;
+  int lpmd_offset;
+  int lod_offset;
+  int lsp_offset;
+  int lnp_offset;
+  int lthr_offset;
+  double *param;
+} hmv_ref_lcontrol;
+
+/* Replaces c_hacapk_adot_body_lfcpy_batch_sorted_ (HACApK_BATCH_SORT.c:86). */
+void c_hacapk_adot_body_lfcpy_batch_sorted_(int *nd, hmv_ref_leafmtxp *st_leafmtxp);
+/* Replaces c_hacapk_adot_body_lfmtx_batch_ (HACApK_BATCH.c:257): zau += H*zu
+ * on host vectors; zbu is unused; the timers are incremented (seconds). */
+void c_hacapk_adot_body_lfmtx_batch_(double *zau, hmv_ref_leafmtxp *st_leafmtxp, double *zu,
+                                     double *zbu, double *time_batch, double *time_set,
+                                     double *time_copy);
+/* Replaces c_hacapk_adot_body_lfdel_batch_ (HACApK_BATCH.c:812). */
+void c_hacapk_adot_body_lfdel_batch_(hmv_ref_leafmtxp *st_leafmtxp);
+/* Replaces c_hacapk_bicgstab_cax_lfmtx_gpu_ (HACApK_SOLVER.c:421): host u, b;
+ * param[82] = max steps, param[90] = tolerance (Fortran param(83), param(91)). */
+void c_hacapk_bicgstab_cax_lfmtx_gpu_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lcontrol *st_ctl,
+                                      double *u_cpu, double *b_cpu, double *param, int *nd,
+                                      int *nstp, int *lrtrn);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HMV_H */

@@ -1,0 +1,150 @@
+"""GPU parity of the H-matvec (libhmv.so through the C ABI) against the CPU oracle.
+
+Bar: FP64 relative error <= 1e-12 in the 2-norm against the oracle's C-loop
+apply (HACApK_MAGMA.c:5-57) on the same leaves and inputs; FP32 storage
+<= 1e-5 and BF16 low-rank storage <= 2e-2 (storage rounding 2^-24 / 2^-9 per
+entry, summed over long rows).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("case", ["sphere3", "sphere4", "two_spheres4", "ref_mesh_sample"])
+def test_f64_apply_matches_oracle(gpu, O, case, request):
+    hm = request.getfixturevalue(case)
+    a = gpu.Arena(hm, "f64")
+    for seed in (0, 1):
+        x = gpu.uniform(seed, hm.nd)
+        y = a.matvec(x)
+        y_ref = O.adot_body_lfmtx(hm, x)
+        print(case, "nd", hm.nd, "seed", seed, "rel err %.3e" % _rel(y, y_ref), flush=True)
+        assert _rel(y, y_ref) <= TOL64
+    a.free()
+
+
+def test_f64_device_apply_overwrite_and_accumulate(gpu, O, sphere4):
+    hm = sphere4
+    a = gpu.Arena(hm, "f64")
+    x = gpu.uniform(3, hm.nd)
+    y0 = gpu.uniform(4, hm.nd)
+    dx, dy = gpu.DeviceBuffer.from_array(x), gpu.DeviceBuffer.from_array(np.full(hm.nd, np.nan))
+    a.apply(dx.ptr, dy.ptr, 0.0)          # beta=0 ignores NaNs in y
+    gpu.hmv_lib().hmv_device_synchronize()
+    y_ref = O.adot_body_lfmtx(hm, x)
+    assert _rel(dy.download(), y_ref) <= TOL64
+    dy.upload(y0)
+    a.apply(dx.ptr, dy.ptr, 1.0)          # y += H x
+    gpu.hmv_lib().hmv_device_synchronize()
+    assert _rel(dy.download(), y0 + y_ref) <= TOL64
+    a.free()
+
+
+def test_apply_is_deterministic(gpu, sphere4):
+    a = gpu.Arena(sphere4, "f64")
+    x = gpu.uniform(5, sphere4.nd)
+    y1 = a.matvec(x)
+    y2 = a.matvec(x)
+    assert np.array_equal(y1, y2)
+    a.free()
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16", 2e-2)])
+def test_low_precision_apply(gpu, O, sphere4, prec, tol):
+    hm = sphere4
+    a = gpu.Arena(hm, prec)
+    x = gpu.uniform(6, hm.nd)
+    y = a.matvec(x)
+    y_ref = O.adot_body_lfmtx(hm, x)
+    err = _rel(y, y_ref)
+    print(prec, "rel err vs FP64 oracle %.3e" % err, flush=True)
+    assert err <= tol
+    a.free()
+
+
+def test_linearity(gpu, sphere4):
+    a = gpu.Arena(sphere4, "f64")
+    x1, x2 = gpu.uniform(10, sphere4.nd), gpu.uniform(11, sphere4.nd)
+    y = a.matvec(2.0 * x1 - 3.0 * x2)
+    y12 = 2.0 * a.matvec(x1) - 3.0 * a.matvec(x2)
+    assert _rel(y, y12) <= 1e-13
+    a.free()
+
+
+def test_reference_named_batch_apply(gpu, O, sphere3):
+    """c_hacapk_adot_body_lfcpy_batch_sorted_ / lfmtx_batch_ / lfdel_batch_:
+    zau += H zu on host vectors, the Fortran driver's calling sequence
+    (m_HACApK_solve.f90:711-712)."""
+    hm = sphere3
+    lp = gpu.make_leafmtxp(hm)
+    gpu.adot_body_lfcpy_batch_sorted(hm.nd, lp)
+    zu = np.ones(hm.nd)
+    zau = np.ones(hm.nd)               # the reference's v(:)=1 before the call
+    gpu.adot_body_lfmtx_batch(zau, lp, zu)
+    gpu.adot_body_lfdel_batch(lp)
+    ref = O.adot_body_lfmtx(hm, zu, np.ones(hm.nd))
+    assert _rel(zau, ref) <= TOL64
+
+
+def test_edge_leaves(gpu, O, H):
+    """Hand-made leaf lists: a rank-0 low-rank leaf, a 1x1 dense leaf, rows
+    covered by no leaf (y = 0 there), a long low-rank block (> 2048 columns,
+    several LDS passes) and blocks whose row stride needs padding."""
+    rng = np.random.default_rng(0)
+    nd = 5000
+    specs = [(1, 0, 1, 10, 11, 10), (2, 0, 21, 1, 22, 1), (1, 3, 1, 2500, 2501, 2499),
+             (2, 0, 2501, 7, 1, 13), (1, 5, 3001, 1000, 1, 2500), (2, 0, 4001, 999, 4001, 999)]
+    bufs = []
+    recs = (H.LeafRec * len(specs))()
+    for i, (lt, kt, r0, ndl, c0, ndt) in enumerate(specs):
+        if lt == 1:
+            a1 = rng.standard_normal(ndt * kt + ndl * kt)
+            recs[i].a1size = ndt * kt * 8
+        else:
+            a1 = rng.standard_normal(ndt * ndl)
+            recs[i].a1size = ndt * ndl * 8
+        a1 = np.ascontiguousarray(np.concatenate([a1, [0.0]]))
+        bufs.append(a1)
+        recs[i].ltmtx, recs[i].kt = lt, kt
+        recs[i].nstrtl, recs[i].ndl, recs[i].nstrtt, recs[i].ndt = r0, ndl, c0, ndt
+        recs[i].a1 = a1.ctypes.data_as(C.POINTER(C.c_double))
+        recs[i].a2 = C.cast(C.addressof(recs[i].a1.contents) + recs[i].a1size, C.POINTER(C.c_double)) \
+            if lt == 1 else None
+
+    ptr, stride = C.addressof(recs), C.sizeof(H.LeafRec)
+    a = H.Arena(None, "f64", nd=nd, nlf=len(specs), leaves_ptr=ptr, stride=stride)
+    x = rng.uniform(-1, 1, nd)
+    y = a.matvec(x)
+    zau = np.zeros(nd)
+    zbu = np.zeros(16)
+    O.lib().oracle_adot_body_lfmtx(zau.ctypes.data_as(O._D), ptr, len(specs), stride,
+                                   x.ctypes.data_as(O._D), zbu.ctypes.data_as(O._D))
+    assert _rel(y, zau) <= TOL64
+    covered = np.zeros(nd, bool)
+    for lt, kt, r0, ndl, c0, ndt in specs:
+        if lt == 2 or kt > 0:
+            covered[r0 - 1:r0 - 1 + ndl] = True
+    assert np.all(y[~covered] == 0.0)
+    a.free()
+
+
+def test_invalid_leaves_rejected(gpu, H):
+    recs = (H.LeafRec * 1)()
+    recs[0].ltmtx, recs[0].kt, recs[0].nstrtl, recs[0].ndl = 2, 0, 10, 5
+    recs[0].nstrtt, recs[0].ndt = 1, 5
+    buf = np.zeros(25)
+    recs[0].a1 = buf.ctypes.data_as(C.POINTER(C.c_double))
+    with pytest.raises(H.HmvError, match="outside"):
+        H.Arena(None, "f64", nd=12, nlf=1, leaves_ptr=C.addressof(recs), stride=C.sizeof(H.LeafRec))
+    recs[0].ltmtx = 3
+    with pytest.raises(H.HmvError, match="ltmtx"):
+        H.Arena(None, "f64", nd=20, nlf=1, leaves_ptr=C.addressof(recs), stride=C.sizeof(H.LeafRec))

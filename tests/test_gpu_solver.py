@@ -1,0 +1,99 @@
+"""GPU BiCGSTAB (hmv_bicgstab, hipGraph per iteration) against the oracle's
+restatement of HACApK_bicgstab_cax_lfmtx_hyp (m_HACApK_solve.f90:342-449)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_bicgstab_matches_oracle_history(gpu, O, sphere4):
+    hm = sphere4
+    b = np.ones(hm.nd)
+    a = gpu.Arena(hm, "f64")
+    u, info, hist = a.bicgstab(b, eps=1e-8, maxit=300)
+    u_ref, nstp_ref, hist_ref = O.bicgstab(hm, b, None, mstep=300, eps=1e-8)
+    print("gpu steps", info["nstp"], "oracle", nstp_ref, "true relres %.2e" % info["true_relres"],
+          flush=True)
+    assert info["status_name"] == "converged"
+    assert abs(info["nstp"] - nstp_ref) <= 1
+    n = min(len(hist), len(hist_ref))
+    # The dot products sum in a different order (GPU tree vs sequential);
+    # BiCGSTAB amplifies that rounding difference about tenfold per step, so
+    # the histories are compared while the residual is above 1e-5.
+    early = (hist_ref[:n] > 1e-5) & (hist[:n] > 1e-5)
+    assert early.sum() >= 4
+    assert np.allclose(np.log10(hist[:n][early]), np.log10(hist_ref[:n][early]), atol=1e-6)
+    assert info["true_relres"] < 2e-8
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-7
+    a.free()
+
+
+def test_check_every_gives_same_result(gpu, sphere4):
+    a = gpu.Arena(sphere4, "f64")
+    b = gpu.uniform(2, sphere4.nd)
+    u1, i1, _ = a.bicgstab(b, eps=1e-8, maxit=300, check_every=1)
+    u8, i8, _ = a.bicgstab(b, eps=1e-8, maxit=300, check_every=8)
+    assert i1["nstp"] == i8["nstp"]
+    assert np.array_equal(u1, u8)
+    a.free()
+
+
+def test_recorded_reference_run(gpu):
+    """The reference's recorded run on input_10ts.pbs (leaf 15, tol 1e-6, potential 1):
+    same step count and log10 residuals within 1e-3 (its ACA+ build stored more
+    entries, so the operator differs at the 1e-6 level)."""
+    H = gpu
+    rec = json.load(open(os.path.join(HERE, "golden", "recorded_run_10ts.json")))
+    d = np.load(os.path.join(HERE, "golden", "input_10ts_mesh.npz"))
+    hm = H.HMatrix(d["nodes"], d["faces"], H.HParams(leaf_size=15, aca_eps=1e-6, kernel="ppohbem"))
+    sol, info, hist = H.solve(hm, d["dble_para"], ztol=rec["tolerance"], mstep=5000)
+    got = np.log10(hist[1:])
+    print("gpu log10 history", got, "recorded", rec["log10_relres_history"], flush=True)
+    assert info["nstp"] == len(rec["log10_relres_history"])
+    assert np.allclose(got, rec["log10_relres_history"], atol=1e-3)
+
+
+def test_reference_named_bicgstab(gpu, O, sphere3):
+    hm = sphere3
+    lp = gpu.make_leafmtxp(hm)
+    param = np.zeros(100)
+    param[82] = 200      # param(83): max steps
+    param[90] = 1e-8     # param(91): tolerance
+    b = np.ones(hm.nd)
+    u = np.zeros(hm.nd)
+    nstp, lrtrn = gpu.bicgstab_cax_lfmtx_gpu(lp, u, b, param)
+    u_ref, nstp_ref, _ = O.bicgstab(hm, b, None, mstep=200, eps=1e-8)
+    assert lrtrn == 0
+    assert abs(nstp - nstp_ref) <= 1
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-7
+
+
+def test_maxiter_and_zero_rhs(gpu, sphere3):
+    a = gpu.Arena(sphere3, "f64")
+    b = gpu.uniform(1, sphere3.nd)
+    u, info, hist = a.bicgstab(b, eps=1e-14, maxit=3)
+    assert info["status_name"] == "maxiter" and info["nstp"] == 3
+    u, info, hist = a.bicgstab(np.zeros(sphere3.nd), eps=1e-8, maxit=10)
+    # b = 0: ||r||/||b|| is 0/0 -> the solver reports a non-finite residual
+    assert info["status_name"] == "nonfinite" and info["nstp"] == 0
+    a.free()
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_mixed_precision_refinement(gpu, O, sphere4, prec):
+    hm = sphere4
+    a64, alow = gpu.Arena(hm, "f64"), gpu.Arena(hm, prec)
+    b = gpu.uniform(3, hm.nd)
+    u, info, hist = gpu.bicgstab_ir(a64, alow, b, eps=1e-8, inner_eps=1e-4, max_outer=30, max_inner=500)
+    u_ref, nstp_ref, _ = O.bicgstab(hm, b, None, mstep=500, eps=1e-10)
+    print(prec, "outer", info["outer"], "inner", info["inner_total"], "relres %.2e" % info["relres"],
+          "err vs FP64 solution %.2e" % (np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)), flush=True)
+    assert info["status_name"] == "converged"
+    assert info["relres"] < 1e-8
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-5
+    a64.free()
+    alow.free()
