@@ -99,7 +99,8 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
   a->nwork1 = (int)P.work1.size();
   a->nwork2 = (int)P.work2.size();
   hipError_t e;
-  if ((e = upload(&a->d_arena, P.arena)) != hipSuccess ||
+  if ((e = hipMalloc(&a->d_arena, P.arena_size)) != hipSuccess ||
+      (e = hipMemcpy(a->d_arena, P.arena.get(), P.arena_size, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = upload(&a->d_seg1, P.seg1)) != hipSuccess ||
       (e = upload(&a->d_seg2, P.seg2)) != hipSuccess ||
       (e = upload(&a->d_work1, P.work1)) != hipSuccess ||
@@ -358,8 +359,8 @@ int hmv_plan_view(hmv_plan p, hmv_plan_tables *t) {
     return HMV_ERR_ARG;
   }
   const hmv::Plan &P = p->P;
-  t->arena = P.arena.data();
-  t->arena_bytes = (int64_t)P.arena.size();
+  t->arena = P.arena.get();
+  t->arena_bytes = (int64_t)P.arena_size;
   t->seg1 = P.seg1.data();
   t->nseg1 = (int64_t)P.seg1.size();
   t->work1 = P.work1.data();

@@ -9,6 +9,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -80,6 +83,14 @@ void parallel_for(int64_t n, F f) {
 bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
                 const PlanTuning &tune, Plan &P, std::string &err) {
   P = Plan();
+  const bool timing = std::getenv("HMV_PLAN_TIMING") != nullptr;
+  auto tp = std::chrono::steady_clock::now();
+  auto lap = [&](const char *what) {
+    if (!timing) return;
+    auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "plan %-12s %.3f s\n", what, std::chrono::duration<double>(now - tp).count());
+    tp = now;
+  };
   P.nd = nd;
   P.prec = prec;
   if (nd <= 0) {
@@ -158,13 +169,14 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     int64_t tbase = 0;
     int64_t zoff = 0;
     int64_t off_d = -1, off_u = -1;  // byte offsets (dense/merged panel, U panel)
+    int64_t beg = -1, end = -1;      // byte extent of both panels incl. padding
     int ld_d = 0, ld_u = 0;
   };
   struct ColGroup {
     int s, n;
     std::vector<int> lr;
     int r = 0;
-    int64_t off = -1;
+    int64_t off = -1, end = -1;  // byte extent incl. alignment padding
     int ld = 0;
     int64_t row0 = 0;  // first global V^T row
   };
@@ -202,6 +214,7 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
       cg[c].r += l->kt;
     }
   }
+  lap("groups");
   st.row_groups = (int64_t)rg.size();
   st.col_groups = (int64_t)cg.size();
 
@@ -231,12 +244,14 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     c.row0 = vrows;
     vrows += c.r;
     off = round_up(off + (int64_t)c.r * c.ld * es_l, HMV_ARENA_ALIGN);
+    c.end = off;
   }
   int64_t zcur = 0;
   for (int r : rg_order) {
     RowGroup &g = rg[r];
     g.zoff = zcur;
     zcur += g.wd;
+    g.beg = off;
     if (merged) {
       int nc = g.wd + g.k;
       if (nc > 0) {
@@ -256,6 +271,7 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
         off = round_up(off + (int64_t)g.n * g.ld_u * es_l, HMV_ARENA_ALIGN);
       }
     }
+    g.end = off;
   }
   if (zcur > INT32_MAX || vrows > INT32_MAX) {
     err = "problem too large for 32-bit gather indices";
@@ -265,18 +281,24 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   st.payload_bytes = st.entries_lr * es_l + st.entries_dense * es_d;
   st.alg_bytes = st.payload_bytes + 16LL * nd;
   try {
-    P.arena.assign((size_t)st.arena_bytes, 0);
+    // left uninitialised: every block (with its padding) is zeroed by the
+    // thread that fills it, so the image is first touched in parallel
+    P.arena.reset(new char[(size_t)st.arena_bytes]);
+    P.arena_size = (size_t)st.arena_bytes;
     P.slotmap.assign((size_t)vrows, 0);
     P.zidx.assign((size_t)zcur, 0);
   } catch (...) {
     err = "out of host memory for the arena image";
     return false;
   }
-  char *A = P.arena.data();
+  char *A = P.arena.get();
+  if (off < st.arena_bytes) std::memset(A + off, 0, (size_t)(st.arena_bytes - off));
+  lap("alloc");
 
   // ---- fill: V^T stacks ---------------------------------------------------
   parallel_for((int64_t)cg.size(), [&](int64_t ci) {
     ColGroup &c = cg[ci];
+    std::memset(A + c.off, 0, (size_t)(c.end - c.off));
     int64_t row = 0;
     for (int i : c.lr) {
       const RefLeaf *l = L[i];
@@ -289,9 +311,11 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     }
   });
 
+  lap("fill V");
   // ---- fill: row panels ---------------------------------------------------
   parallel_for((int64_t)rg.size(), [&](int64_t ri) {
     RowGroup &g = rg[ri];
+    if (g.end > g.beg) std::memset(A + g.beg, 0, (size_t)(g.end - g.beg));
     int64_t zc = g.zoff;
     for (int i : g.dense) {
       const RefLeaf *l = L[i];
@@ -333,6 +357,7 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     }
   });
 
+  lap("fill panels");
   // ---- phase-1 work: pieces of the V^T stacks -----------------------------
   struct Item {
     int64_t bytes;
@@ -487,6 +512,7 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     }
   }
 
+  lap("work lists");
   // largest workgroups first so the dispatcher's in-order launch ends balanced
   auto by_size = [](const Item &x, const Item &y) { return x.bytes > y.bytes; };
   std::stable_sort(it1.begin(), it1.end(), by_size);

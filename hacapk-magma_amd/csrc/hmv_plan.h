@@ -4,6 +4,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -43,7 +44,8 @@ struct Plan {
   int nd = 0;
   int prec = kF64;
   int64_t tsize = 0;             // length of the t vector
-  std::vector<char> arena;       // device image of the arena
+  std::unique_ptr<char[]> arena;  // device image of the arena (arena_size bytes)
+  size_t arena_size = 0;
   std::vector<HmvSeg> seg1, seg2;
   std::vector<HmvWork> work1, work2;
   std::vector<int> slotmap;      // phase-1 output row -> t index
