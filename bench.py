@@ -115,8 +115,8 @@ def main():
     tslots = int(st["tslots"])
     bytes_p1 = 8 * int((lr[:, 5] * lr[:, 1]).sum()) + 8 * nd + 8 * tslots
     bytes_p2 = 8 * int((lr[:, 3] * lr[:, 1]).sum()) + 8 * E_d + 8 * tslots + 16 * nd
-    phases = {"phase1": (ms1, bytes_p1, "hmv_stream_kernel<double, 1> (t = V^T x)"),
-              "phase2": (ms2, bytes_p2, "hmv_stream_kernel<double, 2> (y = [A|U][x;t])")}
+    phases = {"phase1": (ms1, bytes_p1, "hmv_stream_kernel<double, 1, F64, F64> (t = V^T x)"),
+              "phase2": (ms2, bytes_p2, "hmv_stream_kernel<double, 2, F64, F64> (y = [A|U][x;t])")}
     dom = max(phases, key=lambda k: phases[k][0])
     dms, dbytes, dname = phases[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9
@@ -138,6 +138,15 @@ def main():
     parity = float(np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref))
     log("parity vs oracle: %.3e" % parity)
 
+    # achievable streaming rates on this device (plain read / copy kernels)
+    cal_bytes = 2 << 30
+    ms_read = H.stream_test(0, cal_bytes, reps=10)
+    ms_copy = H.stream_test(1, cal_bytes // 2, reps=10)
+    calibration = {"read_GBs": round(cal_bytes / ms_read / 1e6, 1),
+                   "copy_GBs": round(2 * (cal_bytes // 2) / ms_copy / 1e6, 1), "bytes": cal_bytes}
+    log("calibration: plain read %.1f GB/s, plain copy %.1f GB/s"
+        % (calibration["read_GBs"], calibration["copy_GBs"]))
+
     result = {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
@@ -150,6 +159,7 @@ def main():
                    "arena_bytes": int(st["arena_bytes"]), "device": dev_name},
         "roofline": roofline,
         "parity_rel_err_vs_oracle": parity,
+        "calibration": calibration,
         "construction_s": round(t_build, 2), "arena_build_s": round(t_arena, 2),
     }
 

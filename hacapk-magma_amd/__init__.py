@@ -58,7 +58,8 @@ class _GenStats(C.Structure):
 class _ArenaStats(C.Structure):
     _fields_ = [("nd", C.c_int), ("precision", C.c_int), ("nlf", C.c_int64), ("nlr", C.c_int64),
                 ("ndense", C.c_int64), ("entries_lr", C.c_int64), ("entries_dense", C.c_int64),
-                ("tslots", C.c_int64), ("arena_bytes", C.c_int64), ("payload_bytes", C.c_int64),
+                ("tslots", C.c_int64), ("tparts", C.c_int64), ("phase1_bytes", C.c_int64),
+                ("phase2_bytes", C.c_int64), ("arena_bytes", C.c_int64), ("payload_bytes", C.c_int64),
                 ("alg_bytes", C.c_int64), ("col_groups", C.c_int64), ("row_groups", C.c_int64),
                 ("atoms", C.c_int64), ("nwork_phase1", C.c_int64), ("nwork_phase2", C.c_int64),
                 ("ktmax", C.c_int), ("pad", C.c_int), ("build_seconds", C.c_double)]
@@ -76,18 +77,18 @@ class SolveInfo(C.Structure):
 
 
 class _PlanTables(C.Structure):
-    _fields_ = [("arena", C.c_void_p), ("arena_bytes", C.c_int64), ("seg1", C.c_void_p),
-                ("nseg1", C.c_int64), ("work1", C.c_void_p), ("nwork1", C.c_int64),
-                ("seg2", C.c_void_p), ("nseg2", C.c_int64), ("work2", C.c_void_p),
-                ("nwork2", C.c_int64), ("slotmap", C.c_void_p), ("nslot", C.c_int64),
-                ("zidx", C.c_void_p), ("nzidx", C.c_int64), ("tsize", C.c_int64),
-                ("seg_bytes", C.c_int), ("work_bytes", C.c_int)]
+    _fields_ = [("arena", C.c_void_p), ("arena_bytes", C.c_int64), ("work1", C.c_void_p),
+                ("nwork1", C.c_int64), ("work2", C.c_void_p), ("nwork2", C.c_int64),
+                ("zseg", C.c_void_p), ("nzseg", C.c_int64), ("outidx", C.c_void_p),
+                ("noutidx", C.c_int64), ("zidx", C.c_void_p), ("nzidx", C.c_int64),
+                ("tdesc", C.c_void_p), ("tsize", C.c_int64), ("tpart_size", C.c_int64),
+                ("work_bytes", C.c_int), ("zseg_bytes", C.c_int)]
 
 
-SEG_DTYPE = np.dtype([("data_off", "<i8"), ("ld", "<i4"), ("ncols", "<i4"), ("nrows", "<i4"),
-                      ("acc_row", "<i4"), ("nx", "<i4"), ("xsrc", "<i4"), ("tsrc", "<i4"),
-                      ("dtype", "<i2"), ("lpr", "<i2"), ("pad0", "<i4"), ("pad1", "<i4")])
-WORK_DTYPE = np.dtype([("seg_begin", "<i4"), ("seg_end", "<i4"), ("out_base", "<i4"), ("nout", "<i4")])
+WORK_DTYPE = np.dtype([("off0", "<i8"), ("off1", "<i8"), ("rows", "<i4"), ("rows_p", "<i4"),
+                       ("ncol0", "<i4"), ("ncol1", "<i4"), ("dt0", "<i2"), ("dt1", "<i2"),
+                       ("zseg_begin", "<i4"), ("zseg_end", "<i4"), ("out_base", "<i4")])
+ZSEG_DTYPE = np.dtype([("kind", "<i4"), ("ncols", "<i4"), ("src", "<i4"), ("pad", "<i4")])
 
 
 class RefLeafmtxp(C.Structure):
@@ -160,7 +161,7 @@ def hmv_lib():
         L.hmv_init.argtypes = [C.c_int]
         L.hmv_arena_build.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.POINTER(vp)]
         L.hmv_arena_build_tuned.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int64,
-                                            C.c_int64, C.c_int, C.POINTER(vp)]
+                                            C.c_int, C.c_int, C.POINTER(vp)]
         L.hmv_arena_stats_get.argtypes = [vp, C.POINTER(_ArenaStats)]
         L.hmv_arena_free.argtypes = [vp]
         L.hmv_arena_tbytes.argtypes = [vp]
@@ -178,6 +179,7 @@ def hmv_lib():
         L.hmv_memcpy_dtoh.argtypes = [vp, vp, C.c_size_t]
         L.hmv_device_info.argtypes = [C.c_char_p, C.c_int, _I]
         L.hmv_apply_timed.argtypes = [vp, vp, vp, C.c_int, _D, _D, _D]
+        L.hmv_stream_test.argtypes = [C.c_int, vp, vp, C.c_size_t, C.c_int, _D]
         L.hmv_plan_create.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.POINTER(vp)]
         L.hmv_plan_view.argtypes = [vp, C.POINTER(_PlanTables)]
         L.hmv_plan_free.argtypes = [vp]
@@ -389,6 +391,19 @@ class DeviceBuffer:
             pass
 
 
+def stream_test(kind, nbytes, reps=20):
+    """Average ms of a plain read (kind 0) or copy (kind 1) kernel over nbytes."""
+    src = DeviceBuffer(nbytes)
+    dst = DeviceBuffer(nbytes) if kind == 1 else DeviceBuffer(64)
+    _check(hmv_lib().hmv_memcpy_htod(src.ptr, np.zeros(1).ctypes.data, 8), "init")
+    ms = C.c_double()
+    _check(hmv_lib().hmv_stream_test(kind, src.ptr, dst.ptr, int(nbytes), int(reps), C.byref(ms)),
+           "hmv_stream_test")
+    src.free()
+    dst.free()
+    return ms.value
+
+
 def device_info():
     L = hmv_lib()
     buf = C.create_string_buffer(256)
@@ -414,8 +429,8 @@ class Arena:
         if tuning:
             _check(L.hmv_arena_build_tuned(nd, nlf, leaves_ptr, stride, PREC[precision],
                                            int(tuning.get("p1_target_bytes", 0)),
-                                           int(tuning.get("p2_target_bytes", 0)),
-                                           int(tuning.get("lpr_bias", 0)), C.byref(a)),
+                                           int(tuning.get("p1_row_tile", 0)),
+                                           int(tuning.get("p2_max_rows", 0)), C.byref(a)),
                    "hmv_arena_build_tuned")
         else:
             _check(L.hmv_arena_build(nd, nlf, leaves_ptr, stride, PREC[precision], C.byref(a)),
@@ -484,7 +499,7 @@ def plan_tables(hm, precision="f64"):
     try:
         t = _PlanTables()
         _check(L.hmv_plan_view(p, C.byref(t)), "hmv_plan_view")
-        assert t.seg_bytes == SEG_DTYPE.itemsize and t.work_bytes == WORK_DTYPE.itemsize
+        assert t.work_bytes == WORK_DTYPE.itemsize and t.zseg_bytes == ZSEG_DTYPE.itemsize
 
         def arr(ptr, n, dt):
             if n == 0:
@@ -493,10 +508,11 @@ def plan_tables(hm, precision="f64"):
 
         return {
             "arena": arr(t.arena, t.arena_bytes, np.uint8),
-            "seg1": arr(t.seg1, t.nseg1, SEG_DTYPE), "work1": arr(t.work1, t.nwork1, WORK_DTYPE),
-            "seg2": arr(t.seg2, t.nseg2, SEG_DTYPE), "work2": arr(t.work2, t.nwork2, WORK_DTYPE),
-            "slotmap": arr(t.slotmap, t.nslot, np.int32), "zidx": arr(t.zidx, t.nzidx, np.int32),
-            "tsize": t.tsize,
+            "work1": arr(t.work1, t.nwork1, WORK_DTYPE), "work2": arr(t.work2, t.nwork2, WORK_DTYPE),
+            "zseg": arr(t.zseg, t.nzseg, ZSEG_DTYPE), "outidx": arr(t.outidx, t.noutidx, np.int32),
+            "zidx": arr(t.zidx, t.nzidx, np.int32),
+            "tdesc": arr(t.tdesc, 2 * t.tsize, np.int32).reshape(-1, 2),
+            "tsize": t.tsize, "tpart_size": t.tpart_size,
         }
     finally:
         L.hmv_plan_free(p)

@@ -27,9 +27,8 @@ int hip_fail(hipError_t e, const char *what) {
 
 hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta, const int *skip,
                          hipStream_t s) {
-  return hmv_launch_apply(a->acc64, a->d_arena, a->d_seg1, a->d_work1, a->nwork1, a->d_seg2,
-                          a->d_work2, a->nwork2, a->d_slotmap, a->d_zidx, a->d_t, x, y, beta, skip,
-                          s);
+  return hmv_launch_apply(a->mode, a->d_arena, a->d_work1, a->nwork1, a->d_work2, a->nwork2,
+                          a->d_zseg, a->d_outidx, a->d_zidx, a->d_tdesc, a->d_t, x, y, beta, skip, s);
 }
 
 }  // namespace hmv
@@ -62,12 +61,12 @@ void release(hmv_arena_s *a) {
   if (!a) return;
   hmv::free_solve_work(a);
   (void)hipFree(a->d_arena);
-  (void)hipFree(a->d_seg1);
-  (void)hipFree(a->d_seg2);
   (void)hipFree(a->d_work1);
   (void)hipFree(a->d_work2);
-  (void)hipFree(a->d_slotmap);
+  (void)hipFree(a->d_zseg);
+  (void)hipFree(a->d_outidx);
   (void)hipFree(a->d_zidx);
+  (void)hipFree(a->d_tdesc);
   (void)hipFree(a->d_t);
   (void)hipFree(a->d_hx);
   (void)hipFree(a->d_hy);
@@ -95,22 +94,24 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
   a->nd = nd;
   a->prec = prec;
   a->acc64 = prec == HMV_PREC_F64 ? 1 : 0;
+  a->mode = prec == HMV_PREC_F64 ? 0 : prec == HMV_PREC_F32 ? 1 : 2;
   a->tsize = P.tsize;
+  a->tpart_size = P.tpart_size;
   a->nwork1 = (int)P.work1.size();
   a->nwork2 = (int)P.work2.size();
   hipError_t e;
   if ((e = hipMalloc(&a->d_arena, P.arena_size)) != hipSuccess ||
       (e = hipMemcpy(a->d_arena, P.arena.get(), P.arena_size, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = upload(&a->d_seg1, P.seg1)) != hipSuccess ||
-      (e = upload(&a->d_seg2, P.seg2)) != hipSuccess ||
       (e = upload(&a->d_work1, P.work1)) != hipSuccess ||
       (e = upload(&a->d_work2, P.work2)) != hipSuccess ||
-      (e = upload(&a->d_slotmap, P.slotmap)) != hipSuccess ||
-      (e = upload(&a->d_zidx, P.zidx)) != hipSuccess) {
+      (e = upload(&a->d_zseg, P.zseg)) != hipSuccess ||
+      (e = upload(&a->d_outidx, P.outidx)) != hipSuccess ||
+      (e = upload(&a->d_zidx, P.zidx)) != hipSuccess ||
+      (e = upload(&a->d_tdesc, P.tdesc)) != hipSuccess) {
     release(a);
     return hip_fail(e, "hmv_arena_build: upload");
   }
-  size_t tb = (size_t)std::max<int64_t>(1, P.tsize) * (a->acc64 ? 8 : 4);
+  size_t tb = (size_t)std::max<int64_t>(1, P.tpart_size) * (a->acc64 ? 8 : 4);
   if ((e = hipMalloc(&a->d_t, tb)) != hipSuccess) {
     release(a);
     return hip_fail(e, "hmv_arena_build: t vector");
@@ -129,6 +130,9 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
   s.entries_lr = P.st.entries_lr;
   s.entries_dense = P.st.entries_dense;
   s.tslots = P.st.tslots;
+  s.tparts = P.st.tparts;
+  s.phase1_bytes = P.st.phase1_bytes;
+  s.phase2_bytes = P.st.phase2_bytes;
   s.arena_bytes = P.st.arena_bytes;
   s.payload_bytes = P.st.payload_bytes;
   s.alg_bytes = P.st.alg_bytes;
@@ -163,16 +167,12 @@ int hmv_arena_build(int nd, int nlf, const void *leaves, int leaf_stride, int pr
 }
 
 int hmv_arena_build_tuned(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
-                          int64_t p1_target_bytes, int64_t p2_target_bytes, int lpr_bias,
+                          int64_t p1_target_bytes, int p1_row_tile, int p2_max_rows,
                           hmv_arena *out) {
   hmv::PlanTuning t;
   if (p1_target_bytes > 0) t.p1_target_bytes = p1_target_bytes;
-  if (p2_target_bytes > 0) t.p2_target_bytes = p2_target_bytes;
-  if (lpr_bias < 0 || lpr_bias > 4) {
-    set_error("hmv_arena_build_tuned: lpr_bias must be in [0,4]");
-    return HMV_ERR_ARG;
-  }
-  t.lpr_bias = lpr_bias;
+  if (p1_row_tile > 0) t.p1_row_tile = p1_row_tile;
+  if (p2_max_rows > 0) t.p2_max_rows = p2_max_rows;
   return build(nd, nlf, leaves, leaf_stride, precision, t, out);
 }
 
@@ -193,7 +193,7 @@ int hmv_arena_free(hmv_arena a) {
 
 int64_t hmv_arena_tbytes(hmv_arena a) {
   if (!a) return -1;
-  return std::max<int64_t>(1, a->tsize) * (a->acc64 ? 8 : 4);
+  return std::max<int64_t>(1, a->tpart_size) * (a->acc64 ? 8 : 4);
 }
 
 int hmv_apply(hmv_arena a, const double *x, double *y, double beta, void *stream) {
@@ -302,13 +302,12 @@ int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *m
       hipEvent_t e0 = ev[3 * r], e1 = ev[3 * r + 1], e2 = ev[3 * r + 2];
       hipEventRecord(e0, s);
       if (a->nwork1 > 0)
-        e = hmv_launch_apply(a->acc64, a->d_arena, a->d_seg1, a->d_work1, a->nwork1, a->d_seg2,
-                             a->d_work2, 0, a->d_slotmap, a->d_zidx, a->d_t, x, y, 0.0, nullptr, s);
+        e = hmv_launch_apply(a->mode, a->d_arena, a->d_work1, a->nwork1, a->d_work2, 0, a->d_zseg,
+                             a->d_outidx, a->d_zidx, a->d_tdesc, a->d_t, x, y, 0.0, nullptr, s);
       hipEventRecord(e1, s);
       if (e == hipSuccess)
-        e = hmv_launch_apply(a->acc64, a->d_arena, a->d_seg1, a->d_work1, 0, a->d_seg2,
-                             a->d_work2, a->nwork2, a->d_slotmap, a->d_zidx, a->d_t, x, y, 0.0,
-                             nullptr, s);
+        e = hmv_launch_apply(a->mode, a->d_arena, a->d_work1, 0, a->d_work2, a->nwork2, a->d_zseg,
+                             a->d_outidx, a->d_zidx, a->d_tdesc, a->d_t, x, y, 0.0, nullptr, s);
       hipEventRecord(e2, s);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -328,6 +327,32 @@ int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *m
   if (ms_phase1) *ms_phase1 = a->nwork1 > 0 ? t1 / reps : 0.0;
   if (ms_phase2) *ms_phase2 = t2 / reps;
   if (ms_apply) *ms_apply = ta / reps;
+  return HMV_OK;
+}
+
+int hmv_stream_test(int kind, const void *src, void *dst, size_t bytes, int reps, double *ms) {
+  if (!src || (kind != 0 && !dst) || reps < 1 || bytes < 16) {
+    set_error("hmv_stream_test: invalid argument");
+    return HMV_ERR_ARG;
+  }
+  hipStream_t s = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "hmv_stream_test: stream");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  e = hmv_launch_stream_test(kind, src, dst, bytes, s);  // warm
+  hipEventRecord(e0, s);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = hmv_launch_stream_test(kind, src, dst, bytes, s);
+  hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  float t = 0.f;
+  if (e == hipSuccess) hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipStreamDestroy(s);
+  if (e != hipSuccess) return hip_fail(e, "hmv_stream_test");
+  if (ms) *ms = (double)t / reps;
   return HMV_OK;
 }
 
@@ -361,21 +386,21 @@ int hmv_plan_view(hmv_plan p, hmv_plan_tables *t) {
   const hmv::Plan &P = p->P;
   t->arena = P.arena.get();
   t->arena_bytes = (int64_t)P.arena_size;
-  t->seg1 = P.seg1.data();
-  t->nseg1 = (int64_t)P.seg1.size();
   t->work1 = P.work1.data();
   t->nwork1 = (int64_t)P.work1.size();
-  t->seg2 = P.seg2.data();
-  t->nseg2 = (int64_t)P.seg2.size();
   t->work2 = P.work2.data();
   t->nwork2 = (int64_t)P.work2.size();
-  t->slotmap = P.slotmap.data();
-  t->nslot = (int64_t)P.slotmap.size();
+  t->zseg = P.zseg.data();
+  t->nzseg = (int64_t)P.zseg.size();
+  t->outidx = P.outidx.data();
+  t->noutidx = (int64_t)P.outidx.size();
   t->zidx = P.zidx.data();
   t->nzidx = (int64_t)P.zidx.size();
+  t->tdesc = P.tdesc.data();
   t->tsize = P.tsize;
-  t->seg_bytes = (int)sizeof(HmvSeg);
+  t->tpart_size = P.tpart_size;
   t->work_bytes = (int)sizeof(HmvWork);
+  t->zseg_bytes = (int)sizeof(HmvZSeg);
   return HMV_OK;
 }
 

@@ -16,13 +16,15 @@ struct hmv_arena_s {
   int nd = 0;
   int prec = 0;
   int acc64 = 1;
-  int64_t tsize = 0;
+  int mode = 0;  // kernel specialisation: 0 FP64, 1 FP32, 2 BF16 low rank + FP32 dense
+  int64_t tsize = 0;       // t slots
+  int64_t tpart_size = 0;  // t partial slots
   char *d_arena = nullptr;
-  HmvSeg *d_seg1 = nullptr, *d_seg2 = nullptr;
   HmvWork *d_work1 = nullptr, *d_work2 = nullptr;
   int nwork1 = 0, nwork2 = 0;
-  int *d_slotmap = nullptr, *d_zidx = nullptr;
-  void *d_t = nullptr;
+  HmvZSeg *d_zseg = nullptr;
+  int *d_outidx = nullptr, *d_zidx = nullptr, *d_tdesc = nullptr;
+  void *d_t = nullptr;     // t partials (accumulation type)
   double *d_hx = nullptr, *d_hy = nullptr;  // scratch for hmv_apply_host
   HmvSolveWork *solve = nullptr;             // solver workspace (lazy)
   hmv_arena_stats stats{};
@@ -38,10 +40,12 @@ hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta
 void free_solve_work(hmv_arena_s *a);
 }  // namespace hmv
 
-extern "C" hipError_t hmv_launch_apply(int acc64, const char *arena, const HmvSeg *seg1,
-                                       const HmvWork *work1, int nwork1, const HmvSeg *seg2,
-                                       const HmvWork *work2, int nwork2, const int *slotmap,
-                                       const int *zidx, void *t, const double *x, double *y,
+extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *dst, size_t bytes,
+                                             hipStream_t stream);
+extern "C" hipError_t hmv_launch_apply(int mode, const char *arena, const HmvWork *work1,
+                                       int nwork1, const HmvWork *work2, int nwork2,
+                                       const HmvZSeg *zseg, const int *outidx, const int *zidx,
+                                       const int *tdesc, void *tpart, const double *x, double *y,
                                        double beta, const int *skip, hipStream_t stream);
 
 #endif

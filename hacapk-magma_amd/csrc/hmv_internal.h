@@ -1,23 +1,27 @@
 // hmv_internal.h -- shared definitions of the H-matvec engine (host + device).
 //
-// Arena layout (one per precision):
+// Arena layout (one arena per storage precision).  Every workgroup of either
+// phase owns ONE contiguous, column-major block of `rows_p` rows (rows padded
+// to the row-vector width) by `ncol0 + ncol1` columns: column c holds the
+// rows' coefficients for input z[c].  Columns [0,ncol0) are stored with
+// dtype dt0, columns [ncol0, ncol0+ncol1) with dt1 (the second part exists
+// only for BF16 arenas, where dense blocks stay FP32).
 //
-//   Phase 1 (t = V^T x):  low-rank leaves are grouped by column cluster
-//     (nstrtt, ndt).  The V factors of one group are concatenated column by
-//     column, i.e. the group's stacked V^T is one row-major block whose rows
-//     all multiply the same x segment.  Each V^T row produces one t entry.
+//   Phase 1 (t = V^T x): low-rank leaves are grouped by column cluster
+//     (nstrtt, ndt); a group's rank vectors are the rows, its x segment the
+//     inputs.  Long groups are cut into column chunks whose partial sums
+//     land in separate t-partial slots.
 //
-//   Phase 2 (y = [A | U] [x; t]):  leaves are grouped by row cluster
-//     (nstrtl, ndl).  A row group's panel is row-major with the dense blocks'
-//     rows first (inputs gathered from x) followed by the U factors' rows
-//     (inputs: the group's contiguous t segment).  Every y row is produced by
-//     exactly one workgroup, which sums the panel rows of all row clusters
-//     that contain it, so y needs no atomics and the summation order is fixed.
+//   Phase 2 (y = [A | U] [x; t]): rows are cut into atoms, maximal row
+//     ranges covered by the same set of row clusters; a workgroup owns one
+//     atom (at most 256 rows) and all panel columns of every row cluster
+//     covering it: dense columns (inputs gathered from x) and U columns
+//     (inputs: sums of the t partials).  Each y row has exactly one writer.
 //
-// Both phases run the same streaming kernel: a workgroup walks a list of
-// row-major segments, stages the segment's input vector in LDS, computes the
-// row dot products with 16-byte loads and team (lanes-per-row) shuffles,
-// accumulates per-row sums in LDS and writes its outputs once.
+// A workgroup stages its inputs z in LDS (zsegs describe where each column's
+// input comes from), then each lane accumulates `VR` fixed rows over a
+// strided set of columns with 8- or 4-byte loads that are contiguous across
+// the workgroup, and one fixed-order LDS reduction produces the outputs.
 #ifndef HMV_INTERNAL_H
 #define HMV_INTERNAL_H
 
@@ -28,30 +32,35 @@
 #define HMV_DT_BF16 2
 
 #define HMV_WG_THREADS 256
-#define HMV_MAX_ROWS 256      // rows accumulated by one workgroup
+#define HMV_MAX_ROWS 256      // rows owned by one workgroup
 #define HMV_ZCHUNK 2048       // inputs staged in LDS per pass
 #define HMV_ARENA_ALIGN 256   // byte alignment of every block in the arena
 
-// One row-major block processed by a workgroup.
-struct HmvSeg {
-  long long data_off;  // byte offset of row 0, column 0 in the arena
-  int ld;              // row stride in elements (multiple of the 16-byte vector)
-  int ncols;           // columns actually used
-  int nrows;           // rows in this segment
-  int acc_row;         // first row of the workgroup's accumulator it adds to
-  int nx;              // columns [0,nx) read x; columns [nx,ncols) read t
-  int xsrc;            // phase 1: x offset of column 0; phase 2: offset into zidx
-  int tsrc;            // t index of column nx
-  short dtype;         // HMV_DT_*
-  short lpr;           // lanes per row: 4,8,16,32,64 or 256
-  int pad0, pad1;
+// Sources of staged inputs.
+#define HMV_Z_X 0        // z[c] = x[src + c]
+#define HMV_Z_XGATHER 1  // z[c] = x[zidx[src + c]]
+#define HMV_Z_T 2        // z[c] = sum_p tpart[tdesc[src + c].base + p], p < np
+
+// One workgroup's block.
+struct HmvWork {
+  long long off0;   // byte offset of the dt0 part (column-major, ld = rows_p)
+  long long off1;   // byte offset of the dt1 part (or -1)
+  int rows;         // rows produced
+  int rows_p;       // rows stored per column (multiple of the row-vector width)
+  int ncol0;        // columns stored with dt0
+  int ncol1;        // columns stored with dt1
+  short dt0, dt1;   // HMV_DT_*
+  int zseg_begin;   // input sources [zseg_begin, zseg_end), in column order
+  int zseg_end;
+  int out_base;     // phase 1: offset into outidx; phase 2: first y row
 };
 
-// One workgroup's work.
-struct HmvWork {
-  int seg_begin, seg_end;  // range in the segment table
-  int out_base;            // phase 1: offset into slotmap; phase 2: first y row
-  int nout;                // outputs written (<= HMV_MAX_ROWS)
+// A run of consecutive input columns with one source.
+struct HmvZSeg {
+  int kind;   // HMV_Z_*
+  int ncols;
+  int src;
+  int pad;
 };
 
 #endif

@@ -1,15 +1,19 @@
-// hmv_kernels.hip -- H-matvec streaming kernels for gfx950.
+// hmv_kernels.hip -- H-matvec streaming kernel for gfx950.
 //
 // Replaces the reference's GPU leaf apply, c_hacapk_adot_body_lfmtx_batch_queue
 // (HACApK_BATCH.c:47-255) with its vbatched atomic DGEMV launches
 // (HACApK_BATCH.c:328-334), by two launches of one streaming kernel:
-//   phase 1: t = V^T x   over the column-grouped V^T stacks
-//   phase 2: y = [A|U][x;t] over the row-grouped panels (one writer per y row)
+//   phase 1: t partials = V^T x  over the column-grouped rank vectors
+//   phase 2: y = [A|U][x;t]      over row atoms (one writer per y row)
 //
-// Workgroup = 256 threads (4 wave64).  A segment's rows are handed to teams
-// of LPR lanes (LPR = 4..64 within a wave, or 256 = the whole workgroup for
-// long rows); each lane streams 16-byte vectors of its rows, multiplies by
-// the segment's inputs staged in LDS, and the team reduces with lane shuffles.
+// Workgroup = 256 threads (4 wave64) owning one column-major block of
+// rows_p x ncols.  Lane l handles rows [vr*(l%q), vr*(l%q)+vr) of columns
+// l/q, l/q + m, ... (q = rows_p/vr lanes per column, m = 256/q columns per
+// step), so one step of the workgroup reads m whole columns, i.e. one
+// contiguous run of the arena, and every lane keeps its rows' sums in
+// registers across all columns.  Inputs are staged in LDS once per 2048
+// columns; the m column-group partials of each row are summed in a fixed
+// order at the end, so results are deterministic.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -18,221 +22,190 @@
 
 namespace {
 
-template <typename T>
-struct VecLoad;
+__device__ inline float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
+__device__ inline float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
 
-template <>
-struct VecLoad<double> {
-  static constexpr int N = 2;
-  __device__ static inline double fma_dot(const double *p, const double *z, double acc) {
-    const double2 a = *reinterpret_cast<const double2 *>(p);
-    const double2 b = *reinterpret_cast<const double2 *>(z);
-    acc = __builtin_fma(a.x, b.x, acc);
-    acc = __builtin_fma(a.y, b.y, acc);
-    return acc;
-  }
-};
-
-template <>
-struct VecLoad<float> {
-  static constexpr int N = 4;
-  __device__ static inline float fma_dot(const float *p, const float *z, float acc) {
-    const float4 a = *reinterpret_cast<const float4 *>(p);
-    const float4 b = *reinterpret_cast<const float4 *>(z);
-    acc = __builtin_fmaf(a.x, b.x, acc);
-    acc = __builtin_fmaf(a.y, b.y, acc);
-    acc = __builtin_fmaf(a.z, b.z, acc);
-    acc = __builtin_fmaf(a.w, b.w, acc);
-    return acc;
-  }
-};
-
-// 16-bit brain float storage; the payload member gives the type its 2-byte
-// size, so pointer arithmetic on bf16x* steps over whole entries.
-struct bf16x {
-  uint16_t bits;
-};
-static_assert(sizeof(bf16x) == 2, "bf16x must be 2 bytes");
-
-template <>
-struct VecLoad<bf16x> {
-  static constexpr int N = 8;
-  __device__ static inline float lo(uint32_t v) { return __uint_as_float(v << 16); }
-  __device__ static inline float hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
-  __device__ static inline float fma_dot(const bf16x *p, const float *z, float acc) {
-    const uint4 a = *reinterpret_cast<const uint4 *>(p);
-    const float4 b0 = *reinterpret_cast<const float4 *>(z);
-    const float4 b1 = *reinterpret_cast<const float4 *>(z + 4);
-    acc = __builtin_fmaf(lo(a.x), b0.x, acc);
-    acc = __builtin_fmaf(hi(a.x), b0.y, acc);
-    acc = __builtin_fmaf(lo(a.y), b0.z, acc);
-    acc = __builtin_fmaf(hi(a.y), b0.w, acc);
-    acc = __builtin_fmaf(lo(a.z), b1.x, acc);
-    acc = __builtin_fmaf(hi(a.z), b1.y, acc);
-    acc = __builtin_fmaf(lo(a.w), b1.z, acc);
-    acc = __builtin_fmaf(hi(a.w), b1.w, acc);
-    return acc;
-  }
-};
-
-template <typename T>
-struct StoreSize {
-  static constexpr int value = sizeof(T);
-};
-
-// Rows [0,nrows) of a row-major block (row stride ld elements) times the
-// staged inputs z[0, nvec*N); row sums added to accrows[r].
-template <typename T, typename Acc, int LPR>
-__device__ inline void team_rows(const char *__restrict__ base, int ld, int nvec, int nrows,
-                                 const Acc *__restrict__ z, Acc *__restrict__ accrows,
-                                 Acc *__restrict__ red) {
-  constexpr int N = VecLoad<T>::N;
-  constexpr int ES = StoreSize<T>::value;
-  const int tid = threadIdx.x;
-  if constexpr (LPR <= 64) {
-    constexpr int TEAMS = HMV_WG_THREADS / LPR;
-    const int team = tid / LPR, lane = tid % LPR;
-    for (int r = team; r < nrows; r += TEAMS) {
-      const T *row = reinterpret_cast<const T *>(base + (size_t)r * ld * ES);
-      Acc s0 = 0, s1 = 0;
-      int v = lane;
-      for (; v + LPR < nvec; v += 2 * LPR) {
-        s0 = VecLoad<T>::fma_dot(row + (size_t)v * N, z + v * N, s0);
-        s1 = VecLoad<T>::fma_dot(row + (size_t)(v + LPR) * N, z + (v + LPR) * N, s1);
-      }
-      if (v < nvec) s0 = VecLoad<T>::fma_dot(row + (size_t)v * N, z + v * N, s0);
-      Acc s = s0 + s1;
-#pragma unroll
-      for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
-      if (lane == 0) accrows[r] += s;
+// Accumulates columns [ca, cb) of one part (column-major, ld = rows_p) into
+// the lane's row sums; zc points at the staged input of column ca.  DT is
+// the part's storage type (HMV_DT_*).
+template <typename Acc, int DT>
+__device__ inline void accumulate(const char *__restrict__ part, int rows_p, int rq, int cg, int m,
+                                  int ca, int cb, const Acc *__restrict__ zc, Acc &a0, Acc &a1) {
+  if constexpr (DT == HMV_DT_F64) {
+    const double *B = reinterpret_cast<const double *>(part) + rq;
+#pragma unroll 8
+    for (int c = ca + cg; c < cb; c += m)
+      a0 = __builtin_fma((Acc)B[(size_t)c * rows_p], zc[c - ca], a0);
+  } else if constexpr (DT == HMV_DT_F32) {
+    const float *B = reinterpret_cast<const float *>(part) + 2 * rq;
+#pragma unroll 8
+    for (int c = ca + cg; c < cb; c += m) {
+      const float2 v = *reinterpret_cast<const float2 *>(B + (size_t)c * rows_p);
+      const Acc zz = zc[c - ca];
+      a0 = __builtin_fma((Acc)v.x, zz, a0);
+      a1 = __builtin_fma((Acc)v.y, zz, a1);
     }
   } else {
-    // whole workgroup per row
-    const int wave = tid >> 6, lane = tid & 63;
-    for (int r = 0; r < nrows; ++r) {
-      const T *row = reinterpret_cast<const T *>(base + (size_t)r * ld * ES);
-      Acc s0 = 0, s1 = 0;
-      int v = tid;
-      for (; v + HMV_WG_THREADS < nvec; v += 2 * HMV_WG_THREADS) {
-        s0 = VecLoad<T>::fma_dot(row + (size_t)v * N, z + v * N, s0);
-        s1 = VecLoad<T>::fma_dot(row + (size_t)(v + HMV_WG_THREADS) * N, z + (v + HMV_WG_THREADS) * N, s1);
-      }
-      if (v < nvec) s0 = VecLoad<T>::fma_dot(row + (size_t)v * N, z + v * N, s0);
-      Acc s = s0 + s1;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-      if (lane == 0) red[wave] = s;
-      __syncthreads();
-      if (tid == 0) accrows[r] += (red[0] + red[1]) + (red[2] + red[3]);
-      __syncthreads();
+    const uint16_t *B = reinterpret_cast<const uint16_t *>(part) + 2 * rq;
+#pragma unroll 8
+    for (int c = ca + cg; c < cb; c += m) {
+      const uint32_t v = *reinterpret_cast<const uint32_t *>(B + (size_t)c * rows_p);
+      const Acc zz = zc[c - ca];
+      a0 = __builtin_fma((Acc)bf16_lo(v), zz, a0);
+      a1 = __builtin_fma((Acc)bf16_hi(v), zz, a1);
     }
   }
 }
 
-template <typename T, typename Acc>
-__device__ inline void dispatch_lpr(int lpr, const char *base, int ld, int nvec, int nrows,
-                                    const Acc *z, Acc *accrows, Acc *red) {
-  switch (lpr) {
-    case 4: team_rows<T, Acc, 4>(base, ld, nvec, nrows, z, accrows, red); break;
-    case 8: team_rows<T, Acc, 8>(base, ld, nvec, nrows, z, accrows, red); break;
-    case 16: team_rows<T, Acc, 16>(base, ld, nvec, nrows, z, accrows, red); break;
-    case 32: team_rows<T, Acc, 32>(base, ld, nvec, nrows, z, accrows, red); break;
-    case 64: team_rows<T, Acc, 64>(base, ld, nvec, nrows, z, accrows, red); break;
-    default: team_rows<T, Acc, 256>(base, ld, nvec, nrows, z, accrows, red); break;
+// Stages inputs of columns [c0, c0+cw) into z[0, cw).
+template <typename Acc>
+__device__ inline void stage(Acc *__restrict__ z, const HmvZSeg *__restrict__ zsegs,
+                             const HmvWork &w, int c0, int cw, const double *__restrict__ x,
+                             const int *__restrict__ zidx, const int *__restrict__ tdesc,
+                             const Acc *__restrict__ tpart) {
+  const int tid = threadIdx.x;
+  int col = 0;
+  for (int s = w.zseg_begin; s < w.zseg_end && col < c0 + cw; ++s) {
+    const HmvZSeg zs = zsegs[s];
+    const int lo = max(col, c0), hi = min(col + zs.ncols, c0 + cw);
+    for (int c = lo + tid; c < hi; c += HMV_WG_THREADS) {
+      const int j = c - col;
+      Acc v;
+      if (zs.kind == HMV_Z_X) {
+        v = (Acc)x[zs.src + j];
+      } else if (zs.kind == HMV_Z_XGATHER) {
+        v = (Acc)x[zidx[zs.src + j]];
+      } else {
+        const int slot = zs.src + j;
+        const int b = tdesc[2 * slot], np = tdesc[2 * slot + 1];
+        v = tpart[b];
+        for (int p = 1; p < np; ++p) v += tpart[b + p];
+      }
+      z[c - c0] = v;
+    }
+    col += zs.ncols;
   }
 }
 
-__device__ inline int dt_bytes(int dt) { return dt == HMV_DT_F64 ? 8 : dt == HMV_DT_F32 ? 4 : 2; }
-
-// PHASE 1: out[slotmap[out_base+i]] = rows of V^T times x.
-// PHASE 2: y[out_base+i] = beta*y + sum over segments.
-template <typename Acc, int PHASE>
+// PHASE 1: tpart[outidx[out_base + r]] = row sums.
+// PHASE 2: y[out_base + r] = row sums + beta * y.
+// DT0/DT1: storage types of the two column parts (fixed per arena precision).
+template <typename Acc, int PHASE, int DT0, int DT1>
 __global__ __launch_bounds__(HMV_WG_THREADS) void hmv_stream_kernel(
-    const char *__restrict__ arena, const HmvSeg *__restrict__ segs,
-    const HmvWork *__restrict__ works, const double *__restrict__ x,
-    const Acc *__restrict__ tin, const int *__restrict__ zidx,
-    const int *__restrict__ slotmap, Acc *__restrict__ tout, double *__restrict__ y,
-    double beta, const int *__restrict__ skip) {
+    const char *__restrict__ arena, const HmvWork *__restrict__ works,
+    const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ zidx,
+    const int *__restrict__ tdesc, const Acc *__restrict__ tpart_in, const int *__restrict__ outidx,
+    Acc *__restrict__ tpart_out, double *__restrict__ y, double beta, const int *__restrict__ skip) {
   if (skip && *skip) return;
-  __shared__ Acc acc[HMV_MAX_ROWS];
-  __shared__ __attribute__((aligned(16))) Acc z[HMV_ZCHUNK + 16];
-  __shared__ Acc red[4];
+  __shared__ __attribute__((aligned(16))) Acc z[HMV_ZCHUNK];
+  __shared__ Acc red[2 * HMV_WG_THREADS];
   const HmvWork w = works[blockIdx.x];
   const int tid = threadIdx.x;
-  for (int i = tid; i < w.nout; i += HMV_WG_THREADS) acc[i] = Acc(0);
-  for (int si = w.seg_begin; si < w.seg_end; ++si) {
-    const HmvSeg s = segs[si];
-    const int es = dt_bytes(s.dtype);
-    const int vecn = 16 / es;
-    for (int c0 = 0; c0 < s.ncols; c0 += HMV_ZCHUNK) {
-      const int cw = min(HMV_ZCHUNK, s.ncols - c0);
-      const int nvec = (cw + vecn - 1) / vecn;
-      const int cwp = nvec * vecn;
-      __syncthreads();
-      for (int c = tid; c < cwp; c += HMV_WG_THREADS) {
-        const int gc = c0 + c;
-        Acc v = Acc(0);
-        if (c < cw) {
-          if (gc < s.nx) {
-            v = (Acc)(PHASE == 1 ? x[s.xsrc + gc] : x[zidx[s.xsrc + gc]]);
-          } else {
-            v = tin[s.tsrc + (gc - s.nx)];
-          }
-        }
-        z[c] = v;
-      }
-      __syncthreads();
-      const char *base = arena + s.data_off + (long long)c0 * es;
-      if constexpr (sizeof(Acc) == 8) {
-        dispatch_lpr<double, Acc>(s.lpr, base, s.ld, nvec, s.nrows, z, acc + s.acc_row, red);
-      } else {
-        if (s.dtype == HMV_DT_F32)
-          dispatch_lpr<float, Acc>(s.lpr, base, s.ld, nvec, s.nrows, z, acc + s.acc_row, red);
-        else
-          dispatch_lpr<bf16x, Acc>(s.lpr, base, s.ld, nvec, s.nrows, z, acc + s.acc_row, red);
-      }
+  constexpr int vr = (DT0 == HMV_DT_F64) ? 1 : 2;
+  const int q = w.rows_p / vr;
+  const int m = HMV_WG_THREADS / q;
+  const bool active = tid < q * m;
+  const int rq = tid % q, cg = tid / q;
+  Acc a0 = Acc(0), a1 = Acc(0);
+  const int ncols = w.ncol0 + w.ncol1;
+  for (int c0 = 0; c0 < ncols; c0 += HMV_ZCHUNK) {
+    const int cw = min(HMV_ZCHUNK, ncols - c0);
+    if (c0 > 0) __syncthreads();
+    stage<Acc>(z, zsegs, w, c0, cw, x, zidx, tdesc, tpart_in);
+    __syncthreads();
+    if (active) {
+      const int e0 = min(c0 + cw, w.ncol0);
+      if (c0 < e0) accumulate<Acc, DT0>(arena + w.off0, w.rows_p, rq, cg, m, c0, e0, z, a0, a1);
+      const int s1 = max(c0, w.ncol0);
+      if (s1 < c0 + cw)
+        accumulate<Acc, DT1>(arena + w.off1, w.rows_p, rq, cg, m, s1 - w.ncol0, c0 + cw - w.ncol0,
+                             z + (s1 - c0), a0, a1);
     }
+  }
+  if (active) {
+    red[cg * w.rows_p + rq * vr] = a0;
+    if constexpr (vr == 2) red[cg * w.rows_p + rq * vr + 1] = a1;
   }
   __syncthreads();
-  for (int i = tid; i < w.nout; i += HMV_WG_THREADS) {
+  for (int r = tid; r < w.rows; r += HMV_WG_THREADS) {
+    Acc s = red[r];
+    for (int g = 1; g < m; ++g) s += red[g * w.rows_p + r];
     if constexpr (PHASE == 1) {
-      tout[slotmap[w.out_base + i]] = acc[i];
+      tpart_out[outidx[w.out_base + r]] = s;
     } else {
-      const int r = w.out_base + i;
-      const double v = (double)acc[i];
-      y[r] = beta == 0.0 ? v : __builtin_fma(beta, y[r], v);
+      const int row = w.out_base + r;
+      const double v = (double)s;
+      y[row] = beta == 0.0 ? v : __builtin_fma(beta, y[row], v);
     }
   }
+}
+
+template <typename Acc, int DT0, int DT1>
+void launch_mode(const char *arena, const HmvWork *work1, int nwork1, const HmvWork *work2,
+                 int nwork2, const HmvZSeg *zseg, const int *outidx, const int *zidx,
+                 const int *tdesc, void *tpart, const double *x, double *y, double beta,
+                 const int *skip, hipStream_t stream) {
+  const dim3 block(HMV_WG_THREADS);
+  // phase 1 reads only low-rank factors: both parts use DT1's type
+  if (nwork1 > 0)
+    hipLaunchKernelGGL((hmv_stream_kernel<Acc, 1, DT1, DT1>), dim3(nwork1), block, 0, stream, arena,
+                       work1, zseg, x, zidx, tdesc, (const Acc *)nullptr, outidx, (Acc *)tpart,
+                       (double *)nullptr, 0.0, skip);
+  if (nwork2 > 0)
+    hipLaunchKernelGGL((hmv_stream_kernel<Acc, 2, DT0, DT1>), dim3(nwork2), block, 0, stream, arena,
+                       work2, zseg, x, zidx, tdesc, (const Acc *)tpart, outidx, (Acc *)nullptr, y,
+                       beta, skip);
+}
+
+// Plain streaming kernels of known traffic, for bandwidth calibration.
+__global__ __launch_bounds__(HMV_WG_THREADS) void hmv_read_kernel(const double2 *__restrict__ src,
+                                                                   size_t n, double *__restrict__ out) {
+  double s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * HMV_WG_THREADS + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * HMV_WG_THREADS) {
+    const double2 v = src[i];
+    s += v.x + v.y;
+  }
+  if (s == 1.2345e-300) out[0] = s;  // keeps the loads alive; never true for the data used
+}
+
+__global__ __launch_bounds__(HMV_WG_THREADS) void hmv_copy_kernel(const double2 *__restrict__ src,
+                                                                   double2 *__restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * HMV_WG_THREADS + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * HMV_WG_THREADS)
+    dst[i] = src[i];
 }
 
 }  // namespace
 
-// Launches one apply y = H x + beta*y on `stream`.  t has length >= tsize
-// elements of the accumulation type (double if acc64, float otherwise).
-extern "C" hipError_t hmv_launch_apply(int acc64, const char *arena, const HmvSeg *seg1,
-                                       const HmvWork *work1, int nwork1, const HmvSeg *seg2,
-                                       const HmvWork *work2, int nwork2, const int *slotmap,
-                                       const int *zidx, void *t, const double *x, double *y,
+extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *dst, size_t bytes,
+                                             hipStream_t stream) {
+  const size_t n = bytes / 16;
+  const int grid = 256 * 32;
+  if (kind == 0)
+    hipLaunchKernelGGL(hmv_read_kernel, dim3(grid), dim3(HMV_WG_THREADS), 0, stream,
+                       (const double2 *)src, n, (double *)dst);
+  else
+    hipLaunchKernelGGL(hmv_copy_kernel, dim3(grid), dim3(HMV_WG_THREADS), 0, stream,
+                       (const double2 *)src, (double2 *)dst, n);
+  return hipGetLastError();
+}
+
+// Launches phase 1 (nwork1 workgroups) and/or phase 2 (nwork2) of y = H x + beta*y.
+// mode: 0 = FP64 arena, 1 = FP32 arena, 2 = BF16 low-rank + FP32 dense arena.
+extern "C" hipError_t hmv_launch_apply(int mode, const char *arena, const HmvWork *work1,
+                                       int nwork1, const HmvWork *work2, int nwork2,
+                                       const HmvZSeg *zseg, const int *outidx, const int *zidx,
+                                       const int *tdesc, void *tpart, const double *x, double *y,
                                        double beta, const int *skip, hipStream_t stream) {
-  const dim3 block(HMV_WG_THREADS);
-  if (acc64) {
-    if (nwork1 > 0)
-      hipLaunchKernelGGL((hmv_stream_kernel<double, 1>), dim3(nwork1), block, 0, stream, arena,
-                         seg1, work1, x, (const double *)nullptr, zidx, slotmap, (double *)t,
-                         (double *)nullptr, 0.0, skip);
-    if (nwork2 > 0)
-      hipLaunchKernelGGL((hmv_stream_kernel<double, 2>), dim3(nwork2), block, 0, stream, arena,
-                         seg2, work2, x, (const double *)t, zidx, slotmap, (double *)nullptr, y,
-                         beta, skip);
-  } else {
-    if (nwork1 > 0)
-      hipLaunchKernelGGL((hmv_stream_kernel<float, 1>), dim3(nwork1), block, 0, stream, arena,
-                         seg1, work1, x, (const float *)nullptr, zidx, slotmap, (float *)t,
-                         (double *)nullptr, 0.0, skip);
-    if (nwork2 > 0)
-      hipLaunchKernelGGL((hmv_stream_kernel<float, 2>), dim3(nwork2), block, 0, stream, arena,
-                         seg2, work2, x, (const float *)t, zidx, slotmap, (float *)nullptr, y, beta,
-                         skip);
-  }
+  if (mode == 0)
+    launch_mode<double, HMV_DT_F64, HMV_DT_F64>(arena, work1, nwork1, work2, nwork2, zseg, outidx,
+                                                zidx, tdesc, tpart, x, y, beta, skip, stream);
+  else if (mode == 1)
+    launch_mode<float, HMV_DT_F32, HMV_DT_F32>(arena, work1, nwork1, work2, nwork2, zseg, outidx,
+                                               zidx, tdesc, tpart, x, y, beta, skip, stream);
+  else
+    launch_mode<float, HMV_DT_F32, HMV_DT_BF16>(arena, work1, nwork1, work2, nwork2, zseg, outidx,
+                                                zidx, tdesc, tpart, x, y, beta, skip, stream);
   return hipGetLastError();
 }

@@ -2,9 +2,10 @@
 //
 // Replaces the reference's arena build c_hacapk_adot_body_lfcpy_batch_sorted_
 // (HACApK_BATCH_SORT.c:86-858): instead of thousands of per-leaf uploads into
-// padded per-leaf GEMV operands sorted into batches of 5000, every leaf is
-// copied once into a host image of the arena, grouped for the two phases of
-// the apply, and the image goes to the device in one copy.
+// padded per-leaf GEMV operands sorted into batches of 5000, every leaf entry
+// is copied once into a host image of the arena, laid out as one contiguous
+// column-major block per workgroup of the two apply phases, and the image goes
+// to the device in one copy.
 #include "hmv_plan.h"
 
 #include <algorithm>
@@ -21,13 +22,14 @@
 namespace hmv {
 
 int dtype_size(int dt) { return dt == HMV_DT_F64 ? 8 : dt == HMV_DT_F32 ? 4 : 2; }
+int row_vector(int dt) { return dt == HMV_DT_F64 ? 1 : 2; }
 
 namespace {
 
 int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
 uint16_t to_bf16(double v) {
-  // round-to-nearest-even of the float value
+  // round to nearest even of the FP32 value
   float f = (float)v;
   uint32_t u;
   std::memcpy(&u, &f, 4);
@@ -37,7 +39,7 @@ uint16_t to_bf16(double v) {
   return (uint16_t)(u >> 16);
 }
 
-void store(char *dst, int dt, int64_t i, double v) {
+inline void store(char *dst, int dt, int64_t i, double v) {
   if (dt == HMV_DT_F64) {
     reinterpret_cast<double *>(dst)[i] = v;
   } else if (dt == HMV_DT_F32) {
@@ -45,17 +47,6 @@ void store(char *dst, int dt, int64_t i, double v) {
   } else {
     reinterpret_cast<uint16_t *>(dst)[i] = to_bf16(v);
   }
-}
-
-int choose_lpr(int ncols, int vecw, int nrows, int bias) {
-  int cw = std::min(ncols, HMV_ZCHUNK);
-  int nvec = (cw + vecw - 1) / vecw;
-  int lpr = 4;
-  while (lpr < 64 && lpr < nvec) lpr <<= 1;
-  for (int b = 0; b < bias && lpr < 64; ++b) lpr <<= 1;
-  // long rows and too few rows to keep the four waves busy: whole workgroup per row
-  if (nvec > 64 && nrows < HMV_WG_THREADS / 64) lpr = 256;
-  return lpr;
 }
 
 template <typename F>
@@ -76,6 +67,10 @@ void parallel_for(int64_t n, F f) {
       }
     });
   for (auto &t : th) t.join();
+}
+
+const double *u_factor(const RefLeaf *l) {
+  return reinterpret_cast<const double *>(reinterpret_cast<const char *>(l->a1) + l->a1size);
 }
 
 }  // namespace
@@ -109,11 +104,16 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     err = "unknown precision";
     return false;
   }
+  if (tune.p1_target_bytes < 1024 || tune.p1_row_tile < 1 || tune.p1_row_tile > HMV_MAX_ROWS ||
+      tune.p2_max_rows < 1 || tune.p2_max_rows > HMV_MAX_ROWS) {
+    err = "invalid tuning parameters";
+    return false;
+  }
   const int dt_dense = prec == kF64 ? HMV_DT_F64 : HMV_DT_F32;
   const int dt_lr = prec == kF64 ? HMV_DT_F64 : prec == kF32 ? HMV_DT_F32 : HMV_DT_BF16;
   const bool merged = dt_dense == dt_lr;
   const int es_d = dtype_size(dt_dense), es_l = dtype_size(dt_lr);
-  const int vec_d = 16 / es_d, vec_l = 16 / es_l;
+  const int vr = row_vector(dt_lr);  // same row-vector width for both dtypes of an arena
 
   std::vector<const RefLeaf *> L(nlf);
   PlanStats &st = P.st;
@@ -155,31 +155,26 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     }
     st.max_ndt = std::max(st.max_ndt, l->ndt);
   }
-  if (st.tslots > INT32_MAX / 2) {
+  if (st.tslots > INT32_MAX / 4) {
     err = "too many low-rank slots for 32-bit indexing";
     return false;
   }
 
   // ---- groups -------------------------------------------------------------
-  std::map<std::pair<int, int>, int> rkey, ckey;
   struct RowGroup {
     int s, n;  // 0-based first row, rows
     std::vector<int> dense, lr;
     int wd = 0, k = 0;
     int64_t tbase = 0;
     int64_t zoff = 0;
-    int64_t off_d = -1, off_u = -1;  // byte offsets (dense/merged panel, U panel)
-    int64_t beg = -1, end = -1;      // byte extent of both panels incl. padding
-    int ld_d = 0, ld_u = 0;
   };
   struct ColGroup {
     int s, n;
     std::vector<int> lr;
     int r = 0;
-    int64_t off = -1, end = -1;  // byte extent incl. alignment padding
-    int ld = 0;
-    int64_t row0 = 0;  // first global V^T row
+    int np = 1;  // column chunks (t partials per rank vector)
   };
+  std::map<std::pair<int, int>, int> rkey, ckey;
   std::vector<RowGroup> rg;
   std::vector<ColGroup> cg;
   for (int i = 0; i < nlf; ++i) {
@@ -214,9 +209,9 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
       cg[c].r += l->kt;
     }
   }
-  lap("groups");
   st.row_groups = (int64_t)rg.size();
   st.col_groups = (int64_t)cg.size();
+  lap("groups");
 
   // t slots ordered by row group (ascending first row), leaves in leaf order
   std::vector<int> rg_order(rg.size());
@@ -225,55 +220,153 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     return rg[a].s != rg[b].s ? rg[a].s < rg[b].s : rg[a].n < rg[b].n;
   });
   std::vector<int64_t> leaf_tbase(nlf, -1);
-  int64_t tcur = 0;
+  std::vector<int> leaf_cg(nlf, -1);
+  int64_t tcur = 0, zcur = 0;
   for (int r : rg_order) {
     rg[r].tbase = tcur;
+    rg[r].zoff = zcur;
+    zcur += rg[r].wd;
     for (int i : rg[r].lr) {
       leaf_tbase[i] = tcur;
       tcur += L[i]->kt;
     }
   }
   P.tsize = tcur;
+  for (int c = 0; c < (int)cg.size(); ++c)
+    for (int i : cg[c].lr) leaf_cg[i] = c;
+
+  // ---- phase-1 pieces -----------------------------------------------------
+  struct Piece {
+    int c;       // column group
+    int r0, nr;  // rank-vector rows
+    int k;       // column chunk index
+    int c0, cw;  // columns
+    int64_t off, bytes;
+  };
+  std::vector<Piece> pieces;
+  for (int c = 0; c < (int)cg.size(); ++c) {
+    ColGroup &g = cg[c];
+    const int64_t n = g.n;
+    int rs = std::min(g.r, tune.p1_row_tile);
+    if (round_up(rs, vr) * n * es_l > tune.p1_target_bytes)
+      rs = (int)std::max<int64_t>(1, tune.p1_target_bytes / (n * es_l));
+    int cw = g.n;
+    if (rs < std::min(g.r, 8)) {
+      // long rank vectors: chunk the columns; chunk sums are t partials
+      rs = std::min(g.r, 16);
+      cw = (int)std::max<int64_t>(256, tune.p1_target_bytes / (round_up(rs, vr) * es_l));
+      if (cw >= g.n) cw = g.n;
+    }
+    g.np = (int)((n + cw - 1) / cw);
+    for (int r0 = 0; r0 < g.r; r0 += rs) {
+      const int nr = std::min(rs, g.r - r0);
+      for (int k = 0; k < g.np; ++k) {
+        Piece p{c, r0, nr, k, k * cw, (int)std::min<int64_t>(cw, n - (int64_t)k * cw), 0, 0};
+        pieces.push_back(p);
+      }
+    }
+  }
+  // t partial slots: per t slot, np consecutive partials
+  P.tdesc.assign(2 * (size_t)P.tsize, 0);
+  int64_t pcur = 0;
+  for (int r : rg_order)
+    for (int i : rg[r].lr)
+      for (int k = 0; k < L[i]->kt; ++k) {
+        const int64_t s = leaf_tbase[i] + k;
+        P.tdesc[2 * s] = (int)pcur;
+        P.tdesc[2 * s + 1] = cg[leaf_cg[i]].np;
+        pcur += cg[leaf_cg[i]].np;
+      }
+  P.tpart_size = pcur;
+  st.tparts = pcur;
+  if (pcur > INT32_MAX) {
+    err = "too many t partials for 32-bit indexing";
+    return false;
+  }
+  // rank vectors of each column group, in group order: (leaf, l)
+  std::vector<std::vector<std::pair<int, int>>> cg_rows(cg.size());
+  for (int c = 0; c < (int)cg.size(); ++c)
+    for (int i : cg[c].lr)
+      for (int k = 0; k < L[i]->kt; ++k) cg_rows[c].push_back({i, k});
+
+  // ---- phase-2 atoms ------------------------------------------------------
+  std::vector<int> bnd;
+  bnd.reserve(2 * rg.size() + 2);
+  bnd.push_back(0);
+  bnd.push_back(nd);
+  for (auto &g : rg) {
+    bnd.push_back(g.s);
+    bnd.push_back(g.s + g.n);
+  }
+  std::sort(bnd.begin(), bnd.end());
+  bnd.erase(std::unique(bnd.begin(), bnd.end()), bnd.end());
+  st.atoms = (int64_t)bnd.size() - 1;
+  std::vector<std::pair<int, int>> ranges;  // row slices of at most p2_max_rows rows
+  for (size_t a = 0; a + 1 < bnd.size(); ++a)
+    for (int r = bnd[a]; r < bnd[a + 1]; r += tune.p2_max_rows)
+      ranges.push_back({r, std::min(bnd[a + 1], r + tune.p2_max_rows)});
+  std::vector<int> by_start(rg.size());
+  std::iota(by_start.begin(), by_start.end(), 0);
+  std::sort(by_start.begin(), by_start.end(), [&](int a, int b) { return rg[a].s < rg[b].s; });
+  struct Slice {
+    int a, b;
+    std::vector<int> cov;  // covering row groups, by first row then size (outer first)
+    int ncol0 = 0, ncol1 = 0;
+    int64_t off0 = -1, off1 = -1, bytes = 0;
+  };
+  std::vector<Slice> slices(ranges.size());
+  {
+    std::vector<int> active;
+    size_t next = 0;
+    for (size_t q = 0; q < ranges.size(); ++q) {
+      const int a = ranges[q].first, b = ranges[q].second;
+      while (next < by_start.size() && rg[by_start[next]].s < b) active.push_back(by_start[next++]);
+      active.erase(std::remove_if(active.begin(), active.end(),
+                                  [&](int gi) { return rg[gi].s + rg[gi].n <= a; }),
+                   active.end());
+      Slice &S = slices[q];
+      S.a = a;
+      S.b = b;
+      for (int gi : active)
+        if (rg[gi].s <= a && rg[gi].s + rg[gi].n >= b && (rg[gi].wd + rg[gi].k) > 0) S.cov.push_back(gi);
+      std::sort(S.cov.begin(), S.cov.end(), [&](int x, int y) {
+        return rg[x].s != rg[y].s ? rg[x].s < rg[y].s : rg[x].n > rg[y].n;
+      });
+      for (int gi : S.cov) {
+        if (merged) {
+          S.ncol0 += rg[gi].wd + rg[gi].k;
+        } else {
+          S.ncol0 += rg[gi].wd;
+          S.ncol1 += rg[gi].k;
+        }
+      }
+    }
+  }
+  lap("atoms");
 
   // ---- arena layout -------------------------------------------------------
   int64_t off = 0;
-  int64_t vrows = 0;
-  for (auto &c : cg) {
-    c.ld = (int)round_up(c.n, vec_l);
-    c.off = off;
-    c.row0 = vrows;
-    vrows += c.r;
-    off = round_up(off + (int64_t)c.r * c.ld * es_l, HMV_ARENA_ALIGN);
-    c.end = off;
+  for (auto &p : pieces) {
+    p.off = off;
+    p.bytes = round_up(p.nr, vr) * (int64_t)p.cw * es_l;
+    off = round_up(off + p.bytes, HMV_ARENA_ALIGN);
+    st.phase1_bytes += p.bytes;
   }
-  int64_t zcur = 0;
-  for (int r : rg_order) {
-    RowGroup &g = rg[r];
-    g.zoff = zcur;
-    zcur += g.wd;
-    g.beg = off;
-    if (merged) {
-      int nc = g.wd + g.k;
-      if (nc > 0) {
-        g.ld_d = (int)round_up(nc, vec_d);
-        g.off_d = off;
-        off = round_up(off + (int64_t)g.n * g.ld_d * es_d, HMV_ARENA_ALIGN);
-      }
-    } else {
-      if (g.wd > 0) {
-        g.ld_d = (int)round_up(g.wd, vec_d);
-        g.off_d = off;
-        off = round_up(off + (int64_t)g.n * g.ld_d * es_d, HMV_ARENA_ALIGN);
-      }
-      if (g.k > 0) {
-        g.ld_u = (int)round_up(g.k, vec_l);
-        g.off_u = off;
-        off = round_up(off + (int64_t)g.n * g.ld_u * es_l, HMV_ARENA_ALIGN);
-      }
+  for (auto &S : slices) {
+    const int64_t rows_p = round_up(S.b - S.a, vr);
+    if (S.ncol0 > 0) {
+      S.off0 = off;
+      S.bytes += rows_p * S.ncol0 * es_d;
+      off = round_up(off + rows_p * S.ncol0 * es_d, HMV_ARENA_ALIGN);
     }
-    g.end = off;
+    if (S.ncol1 > 0) {
+      S.off1 = off;
+      S.bytes += rows_p * S.ncol1 * es_l;
+      off = round_up(off + rows_p * S.ncol1 * es_l, HMV_ARENA_ALIGN);
+    }
+    st.phase2_bytes += S.bytes;
   }
-  if (zcur > INT32_MAX || vrows > INT32_MAX) {
+  if (zcur > INT32_MAX) {
     err = "problem too large for 32-bit gather indices";
     return false;
   }
@@ -285,7 +378,6 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     // thread that fills it, so the image is first touched in parallel
     P.arena.reset(new char[(size_t)st.arena_bytes]);
     P.arena_size = (size_t)st.arena_bytes;
-    P.slotmap.assign((size_t)vrows, 0);
     P.zidx.assign((size_t)zcur, 0);
   } catch (...) {
     err = "out of host memory for the arena image";
@@ -293,232 +385,135 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   }
   char *A = P.arena.get();
   if (off < st.arena_bytes) std::memset(A + off, 0, (size_t)(st.arena_bytes - off));
-  lap("alloc");
+  lap("layout");
 
-  // ---- fill: V^T stacks ---------------------------------------------------
-  parallel_for((int64_t)cg.size(), [&](int64_t ci) {
-    ColGroup &c = cg[ci];
-    std::memset(A + c.off, 0, (size_t)(c.end - c.off));
-    int64_t row = 0;
-    for (int i : c.lr) {
-      const RefLeaf *l = L[i];
-      for (int k = 0; k < l->kt; ++k, ++row) {
-        char *dst = A + c.off + (row * c.ld) * es_l;
-        const double *src = l->a1 + (size_t)k * l->ndt;
-        for (int j = 0; j < l->ndt; ++j) store(dst, dt_lr, j, src[j]);
-        P.slotmap[c.row0 + row] = (int)(leaf_tbase[i] + k);
-      }
-    }
-  });
-
-  lap("fill V");
-  // ---- fill: row panels ---------------------------------------------------
-  parallel_for((int64_t)rg.size(), [&](int64_t ri) {
-    RowGroup &g = rg[ri];
-    if (g.end > g.beg) std::memset(A + g.beg, 0, (size_t)(g.end - g.beg));
-    int64_t zc = g.zoff;
-    for (int i : g.dense) {
-      const RefLeaf *l = L[i];
-      for (int j = 0; j < l->ndt; ++j) P.zidx[zc++] = l->nstrtt - 1 + j;
-    }
-    if (g.off_d >= 0) {
-      for (int row = 0; row < g.n; ++row) {
-        char *dst = A + g.off_d + ((int64_t)row * g.ld_d) * es_d;
-        int64_t col = 0;
-        for (int i : g.dense) {
-          const RefLeaf *l = L[i];
-          const double *src = l->a1 + (size_t)row * l->ndt;  // row `row` of A
-          for (int j = 0; j < l->ndt; ++j) store(dst, dt_dense, col + j, src[j]);
-          col += l->ndt;
-        }
-        if (merged) {
-          for (int i : g.lr) {
-            const RefLeaf *l = L[i];
-            const double *u = reinterpret_cast<const double *>(
-                reinterpret_cast<const char *>(l->a1) + l->a1size);
-            for (int k = 0; k < l->kt; ++k) store(dst, dt_dense, col + k, u[row + (size_t)k * l->ndl]);
-            col += l->kt;
-          }
-        }
-      }
-    }
-    if (g.off_u >= 0) {
-      for (int row = 0; row < g.n; ++row) {
-        char *dst = A + g.off_u + ((int64_t)row * g.ld_u) * es_l;
-        int64_t col = 0;
-        for (int i : g.lr) {
-          const RefLeaf *l = L[i];
-          const double *u = reinterpret_cast<const double *>(
-              reinterpret_cast<const char *>(l->a1) + l->a1size);
-          for (int k = 0; k < l->kt; ++k) store(dst, dt_lr, col + k, u[row + (size_t)k * l->ndl]);
-          col += l->kt;
-        }
-      }
-    }
-  });
-
-  lap("fill panels");
-  // ---- phase-1 work: pieces of the V^T stacks -----------------------------
+  // ---- tables -------------------------------------------------------------
+  for (int r : rg_order) {
+    int64_t zc = rg[r].zoff;
+    for (int i : rg[r].dense)
+      for (int j = 0; j < L[i]->ndt; ++j) P.zidx[zc++] = L[i]->nstrtt - 1 + j;
+  }
   struct Item {
     int64_t bytes;
     HmvWork w;
   };
-  std::vector<Item> it1;
-  for (auto &c : cg) {
-    if (c.r == 0) continue;
-    int64_t rowbytes = (int64_t)c.ld * es_l;
-    int rpp = (int)std::max<int64_t>(1, std::min<int64_t>(HMV_MAX_ROWS, tune.p1_target_bytes / rowbytes));
-    for (int r0 = 0; r0 < c.r; r0 += rpp) {
-      int nr = std::min(rpp, c.r - r0);
-      HmvSeg s{};
-      s.data_off = c.off + (int64_t)r0 * rowbytes;
-      s.ld = c.ld;
-      s.ncols = c.n;
-      s.nrows = nr;
-      s.acc_row = 0;
-      s.nx = c.n;
-      s.xsrc = c.s;
-      s.tsrc = 0;
-      s.dtype = (short)dt_lr;
-      s.lpr = (short)choose_lpr(c.n, vec_l, nr, tune.lpr_bias);
-      HmvWork w{};
-      w.seg_begin = (int)P.seg1.size();
-      w.seg_end = w.seg_begin + 1;
-      w.out_base = (int)(c.row0 + r0);
-      w.nout = nr;
-      P.seg1.push_back(s);
-      it1.push_back({(int64_t)nr * rowbytes, w});
+  std::vector<Item> it1, it2;
+  it1.reserve(pieces.size());
+  for (auto &p : pieces) {
+    const ColGroup &g = cg[p.c];
+    HmvWork w{};
+    w.off0 = p.off;
+    w.off1 = -1;
+    w.rows = p.nr;
+    w.rows_p = (int)round_up(p.nr, vr);
+    w.ncol0 = p.cw;
+    w.ncol1 = 0;
+    w.dt0 = (short)dt_lr;
+    w.dt1 = (short)dt_lr;
+    w.zseg_begin = (int)P.zseg.size();
+    P.zseg.push_back(HmvZSeg{HMV_Z_X, p.cw, g.s + p.c0, 0});
+    w.zseg_end = (int)P.zseg.size();
+    w.out_base = (int)P.outidx.size();
+    for (int r = 0; r < p.nr; ++r) {
+      const auto &rv = cg_rows[p.c][p.r0 + r];
+      const int64_t s = leaf_tbase[rv.first] + rv.second;
+      P.outidx.push_back(P.tdesc[2 * s] + p.k);
     }
+    it1.push_back({p.bytes, w});
   }
-
-  // ---- phase-2 work: atoms of the row partition ---------------------------
-  std::vector<int> bnd;
-  bnd.reserve(2 * rg.size() + 2);
-  bnd.push_back(0);
-  bnd.push_back(nd);
-  for (auto &g : rg) {
-    bnd.push_back(g.s);
-    bnd.push_back(g.s + g.n);
-  }
-  std::sort(bnd.begin(), bnd.end());
-  bnd.erase(std::unique(bnd.begin(), bnd.end()), bnd.end());
-  const int natoms = (int)bnd.size() - 1;
-  st.atoms = natoms;
-  // per-atom work estimate (payload bytes of the panel rows that cover it)
-  std::vector<int64_t> atom_bytes(natoms, 0);
-  for (auto &g : rg) {
-    int a0 = (int)(std::lower_bound(bnd.begin(), bnd.end(), g.s) - bnd.begin());
-    int64_t per_row = (merged ? (int64_t)(g.wd + g.k) * es_d : (int64_t)g.wd * es_d + (int64_t)g.k * es_l);
-    for (int a = a0; a < natoms && bnd[a] < g.s + g.n; ++a)
-      atom_bytes[a] += per_row * (bnd[a + 1] - bnd[a]);
-  }
-  // merge consecutive atoms into workgroups
-  std::vector<std::pair<int, int>> ranges;  // [row begin, row end)
-  {
-    int a = 0;
-    while (a < natoms) {
-      int b = a;
-      int64_t bytes = 0;
-      while (b < natoms) {
-        int rows_if = bnd[b + 1] - bnd[a];
-        if (b > a && (rows_if > tune.p2_max_rows || bytes + atom_bytes[b] > tune.p2_target_bytes)) break;
-        if (b == a && bnd[b + 1] - bnd[b] > tune.p2_max_rows) break;
-        bytes += atom_bytes[b];
-        ++b;
+  it2.reserve(slices.size());
+  for (auto &S : slices) {
+    HmvWork w{};
+    w.off0 = S.off0;
+    w.off1 = S.off1;
+    w.rows = S.b - S.a;
+    w.rows_p = (int)round_up(S.b - S.a, vr);
+    w.ncol0 = S.ncol0;
+    w.ncol1 = S.ncol1;
+    w.dt0 = (short)dt_dense;
+    w.dt1 = (short)dt_lr;
+    w.zseg_begin = (int)P.zseg.size();
+    if (merged) {
+      for (int gi : S.cov) {
+        if (rg[gi].wd > 0) P.zseg.push_back(HmvZSeg{HMV_Z_XGATHER, rg[gi].wd, (int)rg[gi].zoff, 0});
+        if (rg[gi].k > 0) P.zseg.push_back(HmvZSeg{HMV_Z_T, rg[gi].k, (int)rg[gi].tbase, 0});
       }
-      if (b == a) {
-        // a single atom taller than the row limit: split it into row slices
-        int r = bnd[a];
-        while (r < bnd[a + 1]) {
-          int e = std::min(bnd[a + 1], r + tune.p2_max_rows);
-          ranges.push_back({r, e});
-          r = e;
+    } else {
+      for (int gi : S.cov)
+        if (rg[gi].wd > 0) P.zseg.push_back(HmvZSeg{HMV_Z_XGATHER, rg[gi].wd, (int)rg[gi].zoff, 0});
+      for (int gi : S.cov)
+        if (rg[gi].k > 0) P.zseg.push_back(HmvZSeg{HMV_Z_T, rg[gi].k, (int)rg[gi].tbase, 0});
+    }
+    w.zseg_end = (int)P.zseg.size();
+    w.out_base = S.a;
+    it2.push_back({S.bytes, w});
+  }
+  lap("tables");
+
+  // ---- fill ---------------------------------------------------------------
+  parallel_for((int64_t)pieces.size(), [&](int64_t pi) {
+    const Piece &p = pieces[pi];
+    char *dst = A + p.off;
+    const int rows_p = (int)round_up(p.nr, vr);
+    std::memset(dst, 0, (size_t)(round_up(p.off + p.bytes, HMV_ARENA_ALIGN) - p.off));
+    for (int r = 0; r < p.nr; ++r) {
+      const auto &rv = cg_rows[p.c][p.r0 + r];
+      const RefLeaf *l = L[rv.first];
+      const double *v = l->a1 + (size_t)rv.second * l->ndt + p.c0;  // V(:, l) from column c0
+      for (int j = 0; j < p.cw; ++j) store(dst, dt_lr, r + (int64_t)rows_p * j, v[j]);
+    }
+  });
+  parallel_for((int64_t)slices.size(), [&](int64_t si) {
+    const Slice &S = slices[si];
+    const int rows = S.b - S.a;
+    const int64_t rows_p = round_up(rows, vr);
+    if (S.off0 >= 0)
+      std::memset(A + S.off0, 0, (size_t)(round_up(S.off0 + rows_p * S.ncol0 * es_d, HMV_ARENA_ALIGN) - S.off0));
+    if (S.off1 >= 0)
+      std::memset(A + S.off1, 0, (size_t)(round_up(S.off1 + rows_p * S.ncol1 * es_l, HMV_ARENA_ALIGN) - S.off1));
+    char *d0 = S.off0 >= 0 ? A + S.off0 : nullptr;
+    char *d1 = S.off1 >= 0 ? A + S.off1 : nullptr;
+    int64_t c0 = 0, c1 = 0;
+    auto put_dense = [&](const RowGroup &g, char *dst, int dt, int64_t &col) {
+      for (int i : g.dense) {
+        const RefLeaf *l = L[i];
+        for (int r = 0; r < rows; ++r) {
+          const double *arow = l->a1 + (size_t)(S.a + r - g.s) * l->ndt;  // row of A
+          for (int j = 0; j < l->ndt; ++j) store(dst, dt, r + rows_p * (col + j), arow[j]);
         }
-        ++a;
+        col += l->ndt;
+      }
+    };
+    auto put_u = [&](const RowGroup &g, char *dst, int dt, int64_t &col) {
+      for (int i : g.lr) {
+        const RefLeaf *l = L[i];
+        const double *u = u_factor(l);
+        for (int k = 0; k < l->kt; ++k)
+          for (int r = 0; r < rows; ++r)
+            store(dst, dt, r + rows_p * (col + k), u[(S.a + r - g.s) + (size_t)k * l->ndl]);
+        col += l->kt;
+      }
+    };
+    for (int gi : S.cov) {
+      const RowGroup &g = rg[gi];
+      if (merged) {
+        put_dense(g, d0, dt_dense, c0);
+        put_u(g, d0, dt_lr, c0);
       } else {
-        ranges.push_back({bnd[a], bnd[b]});
-        a = b;
+        put_dense(g, d0, dt_dense, c0);
+        put_u(g, d1, dt_lr, c1);
       }
     }
-  }
-  // row groups sorted by first row for the overlap search; groups that
-  // contain a row form a chain, found by scanning groups that start at or
-  // before the range and end after it.
-  std::vector<int> by_start(rg.size());
-  std::iota(by_start.begin(), by_start.end(), 0);
-  std::sort(by_start.begin(), by_start.end(), [&](int a, int b) { return rg[a].s < rg[b].s; });
-  // For each range collect overlapping groups.  Use an interval stabbing
-  // structure: process ranges in order with an active set.
-  std::vector<Item> it2;
-  it2.reserve(ranges.size());
-  {
-    std::vector<int> active;
-    size_t next = 0;
-    for (auto &rr : ranges) {
-      const int a = rr.first, b = rr.second;
-      while (next < by_start.size() && rg[by_start[next]].s < b) active.push_back(by_start[next++]);
-      // drop groups that ended at or before a
-      active.erase(std::remove_if(active.begin(), active.end(),
-                                  [&](int gi) { return rg[gi].s + rg[gi].n <= a; }),
-                   active.end());
-      HmvWork w{};
-      w.seg_begin = (int)P.seg2.size();
-      w.out_base = a;
-      w.nout = b - a;
-      int64_t bytes = 0;
-      // deterministic order: groups by first row, then size
-      std::vector<int> cov;
-      for (int gi : active)
-        if (rg[gi].s < b && rg[gi].s + rg[gi].n > a) cov.push_back(gi);
-      std::sort(cov.begin(), cov.end(), [&](int x, int y) {
-        return rg[x].s != rg[y].s ? rg[x].s < rg[y].s : rg[x].n > rg[y].n;
-      });
-      for (int gi : cov) {
-        const RowGroup &g = rg[gi];
-        int lo = std::max(a, g.s), hi = std::min(b, g.s + g.n);
-        if (g.off_d >= 0) {
-          HmvSeg s{};
-          s.data_off = g.off_d + (int64_t)(lo - g.s) * g.ld_d * es_d;
-          s.ld = g.ld_d;
-          s.ncols = merged ? g.wd + g.k : g.wd;
-          s.nrows = hi - lo;
-          s.acc_row = lo - a;
-          s.nx = g.wd;
-          s.xsrc = (int)g.zoff;
-          s.tsrc = (int)g.tbase;
-          s.dtype = (short)dt_dense;
-          s.lpr = (short)choose_lpr(s.ncols, vec_d, s.nrows, tune.lpr_bias);
-          P.seg2.push_back(s);
-          bytes += (int64_t)s.nrows * s.ld * es_d;
-        }
-        if (g.off_u >= 0) {
-          HmvSeg s{};
-          s.data_off = g.off_u + (int64_t)(lo - g.s) * g.ld_u * es_l;
-          s.ld = g.ld_u;
-          s.ncols = g.k;
-          s.nrows = hi - lo;
-          s.acc_row = lo - a;
-          s.nx = 0;
-          s.xsrc = 0;
-          s.tsrc = (int)g.tbase;
-          s.dtype = (short)dt_lr;
-          s.lpr = (short)choose_lpr(s.ncols, vec_l, s.nrows, tune.lpr_bias);
-          P.seg2.push_back(s);
-          bytes += (int64_t)s.nrows * s.ld * es_l;
-        }
-      }
-      w.seg_end = (int)P.seg2.size();
-      it2.push_back({bytes, w});
-    }
-  }
+  });
+  lap("fill");
 
-  lap("work lists");
   // largest workgroups first so the dispatcher's in-order launch ends balanced
   auto by_size = [](const Item &x, const Item &y) { return x.bytes > y.bytes; };
   std::stable_sort(it1.begin(), it1.end(), by_size);
   std::stable_sort(it2.begin(), it2.end(), by_size);
   for (auto &i : it1) P.work1.push_back(i.w);
   for (auto &i : it2) P.work2.push_back(i.w);
+  lap("order");
   return true;
 }
 

@@ -32,10 +32,13 @@ struct PlanStats {
   int64_t entries_lr = 0;     // sum_LR kt*(ndl+ndt)
   int64_t entries_dense = 0;  // sum_D ndl*ndt
   int64_t tslots = 0;         // sum_LR kt
+  int64_t tparts = 0;         // t partial slots (>= tslots)
   int64_t col_groups = 0, row_groups = 0, atoms = 0;
   int64_t arena_bytes = 0;    // bytes of the arena including padding
   int64_t payload_bytes = 0;  // bytes of factor/dense entries at storage precision
   int64_t alg_bytes = 0;      // payload + x read once + y written once (8 B each)
+  int64_t phase1_bytes = 0;   // arena bytes read by phase 1 (incl. row padding)
+  int64_t phase2_bytes = 0;   // arena bytes read by phase 2 (incl. row padding)
   int ktmax = 0;
   int max_ndt = 0;
 };
@@ -43,22 +46,23 @@ struct PlanStats {
 struct Plan {
   int nd = 0;
   int prec = kF64;
-  int64_t tsize = 0;             // length of the t vector
-  std::unique_ptr<char[]> arena;  // device image of the arena (arena_size bytes)
+  int64_t tsize = 0;               // t slots (one per rank vector)
+  int64_t tpart_size = 0;          // partial slots written by phase 1
+  std::unique_ptr<char[]> arena;   // device image of the arena (arena_size bytes)
   size_t arena_size = 0;
-  std::vector<HmvSeg> seg1, seg2;
   std::vector<HmvWork> work1, work2;
-  std::vector<int> slotmap;      // phase-1 output row -> t index
-  std::vector<int> zidx;         // phase-2 gather indices into x
+  std::vector<HmvZSeg> zseg;       // input sources of both phases
+  std::vector<int> outidx;         // phase-1 output row -> t partial slot
+  std::vector<int> zidx;           // phase-2 gather indices into x
+  std::vector<int> tdesc;          // per t slot: {first partial, number of partials}
   PlanStats st;
 };
 
 // Tunables of the repack (chosen by the autotuner or left at defaults).
 struct PlanTuning {
-  int64_t p1_target_bytes = 128 * 1024;  // payload per phase-1 workgroup
-  int64_t p2_target_bytes = 192 * 1024;  // payload per phase-2 workgroup
-  int p2_max_rows = HMV_MAX_ROWS;
-  int lpr_bias = 0;                      // +1 doubles the lanes per row chosen
+  int64_t p1_target_bytes = 96 * 1024;  // arena bytes per phase-1 workgroup
+  int p1_row_tile = 256;                // max rank vectors per phase-1 workgroup
+  int p2_max_rows = HMV_MAX_ROWS;       // max rows per phase-2 workgroup
 };
 
 // Builds the plan.  `leaves` points at nlf records spaced `stride` bytes.
@@ -67,6 +71,8 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
                 const PlanTuning &tune, Plan &out, std::string &err);
 
 int dtype_size(int dt);
+// Rows handled per lane (each lane loads `row_vector(dt)` consecutive rows).
+int row_vector(int dt);
 
 }  // namespace hmv
 

@@ -62,7 +62,10 @@ typedef struct {
   int64_t nlf, nlr, ndense;
   int64_t entries_lr;     /* sum_LR kt*(ndl+ndt)                               */
   int64_t entries_dense;  /* sum_D ndl*ndt                                     */
-  int64_t tslots;         /* sum_LR kt (length of the t vector)                */
+  int64_t tslots;         /* sum_LR kt (rank vectors)                          */
+  int64_t tparts;         /* t partial slots written by phase 1 (>= tslots)    */
+  int64_t phase1_bytes;   /* arena bytes read by the V^T x phase                */
+  int64_t phase2_bytes;   /* arena bytes read by the [A|U] phase                */
   int64_t arena_bytes;    /* device bytes of factors incl. alignment padding   */
   int64_t payload_bytes;  /* bytes of the stored entries                        */
   int64_t alg_bytes;      /* payload + 16*nd (x read once, y written once)      */
@@ -81,11 +84,12 @@ int hmv_init(int device);
 int hmv_arena_build(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
                     hmv_arena *out);
 
-/* Same, with explicit repack tunables (used by the autotuner):
- * p1_target_bytes / p2_target_bytes: payload per workgroup of each phase
- * (<=0: default), lpr_bias: 0 default, k doubles lanes-per-row k times. */
+/* Same, with explicit repack tunables (used by the autotuner); <= 0 keeps
+ * the default: p1_target_bytes = arena bytes per V^T x workgroup (96 KiB),
+ * p1_row_tile = max rank vectors per V^T x workgroup (256), p2_max_rows =
+ * max rows per [A|U] workgroup (256). */
 int hmv_arena_build_tuned(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
-                          int64_t p1_target_bytes, int64_t p2_target_bytes, int lpr_bias,
+                          int64_t p1_target_bytes, int p1_row_tile, int p2_max_rows,
                           hmv_arena *out);
 
 int hmv_arena_stats_get(hmv_arena a, hmv_arena_stats *s);
@@ -99,7 +103,7 @@ int hmv_apply(hmv_arena a, const double *x, double *y, double beta, void *stream
  * c_hacapk_adot_body_lfmtx_batch_, HACApK_BATCH.c:257-271); synchronous. */
 int hmv_apply_host(hmv_arena a, const double *zu, double *zau);
 
-/* Device work buffers needed by hmv_apply (t vector).  Returns bytes. */
+/* Bytes of the device t-partial buffer owned by the arena. */
 int64_t hmv_arena_tbytes(hmv_arena a);
 
 /* ---- BiCGSTAB ---------------------------------------------------------- */
@@ -152,23 +156,29 @@ int hmv_device_info(char *name, int name_len, int *compute_units);
 int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *ms_phase1,
                     double *ms_phase2, double *ms_apply);
 
+/* Bandwidth calibration: `reps` launches of a plain streaming kernel over
+ * `bytes` bytes (kind 0: read-only sum of src; kind 1: copy src -> dst);
+ * *ms receives the average launch time (HIP events). */
+int hmv_stream_test(int kind, const void *src, void *dst, size_t bytes, int reps, double *ms);
+
 /* ---- host-side plan inspection (no device needed) ----------------------
  * The repack that hmv_arena_build uploads, exposed for inspection and for
  * CPU-side validation of the arena's indexing.  Tables use the engine's
- * internal record layouts: segment = {int64 data_off (bytes); int ld, ncols,
- * nrows, acc_row, nx, xsrc, tsrc; int16 dtype, lpr; int pad[2]} (48 bytes),
- * work = {int seg_begin, seg_end, out_base, nout} (16 bytes). */
+ * internal record layouts: work = {int64 off0, off1; int rows, rows_p,
+ * ncol0, ncol1; int16 dt0, dt1; int zseg_begin, zseg_end, out_base}
+ * (48 bytes), zseg = {int kind, ncols, src, pad} (16 bytes), tdesc = 2 ints
+ * per t slot {first partial, partials}. */
 typedef struct hmv_plan_s *hmv_plan;
 typedef struct {
   const void *arena;   int64_t arena_bytes;
-  const void *seg1;    int64_t nseg1;
   const void *work1;   int64_t nwork1;
-  const void *seg2;    int64_t nseg2;
   const void *work2;   int64_t nwork2;
-  const int *slotmap;  int64_t nslot;
+  const void *zseg;    int64_t nzseg;
+  const int *outidx;   int64_t noutidx;
   const int *zidx;     int64_t nzidx;
-  int64_t tsize;
-  int seg_bytes, work_bytes;
+  const int *tdesc;
+  int64_t tsize, tpart_size;
+  int work_bytes, zseg_bytes;
 } hmv_plan_tables;
 int hmv_plan_create(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
                     hmv_plan *out);

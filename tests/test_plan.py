@@ -25,15 +25,30 @@ def test_plan_emulation_matches_oracle(H, O, request, case, prec):
 
 
 def test_plan_alignment_and_order(H, sphere4):
-    T = H.plan_tables(sphere4, "f64")
-    for segs in (T["seg1"], T["seg2"]):
-        assert np.all(segs["data_off"] % 16 == 0)
-        assert np.all((segs["ld"] * 8) % 16 == 0)
-    # workgroups are dispatched largest first
-    for w, segs in ((T["work1"], T["seg1"]), (T["work2"], T["seg2"])):
-        sizes = [int(sum(segs[i]["nrows"] * segs[i]["ld"] for i in range(a, b)))
-                 for a, b in zip(w["seg_begin"], w["seg_end"])]
-        assert sizes == sorted(sizes, reverse=True)
+    for prec in ("f64", "bf16"):
+        T = H.plan_tables(sphere4, prec)
+        for w in (T["work1"], T["work2"]):
+            assert np.all(w["off0"][w["ncol0"] > 0] % 256 == 0)
+            assert np.all(w["off1"][w["ncol1"] > 0] % 256 == 0)
+        # workgroups are dispatched largest first
+        for w in (T["work1"], T["work2"]):
+            es0 = np.where(w["dt0"] == 0, 8, np.where(w["dt0"] == 1, 4, 2))
+            es1 = np.where(w["dt1"] == 0, 8, np.where(w["dt1"] == 1, 4, 2))
+            size = w["rows_p"] * (w["ncol0"] * es0 + w["ncol1"] * es1)
+            assert np.all(np.diff(size) <= 0)
+
+
+def test_long_column_groups_are_chunked(H, O):
+    """A wide low-rank block is cut into column chunks whose sums are t
+    partials; the emulated apply still equals the oracle."""
+    m = H.merge(H.icosphere(4), H.icosphere(4, 1.0, (3.0, 0.0, 0.0)))
+    hm = H.HMatrix(*m, H.baseline_params(aca_eps=1e-5))
+    T = H.plan_tables(hm, "f64")
+    assert T["tpart_size"] >= T["tsize"]
+    x = H.uniform(2, hm.nd)
+    _, y = PE.run(T, x, hm.nd)
+    y_ref = O.adot_body_lfmtx(hm, x)
+    assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < 1e-14
 
 
 def test_plan_rejects_bad_input(H, sphere3):
