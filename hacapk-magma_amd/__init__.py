@@ -80,15 +80,17 @@ class _PlanTables(C.Structure):
     _fields_ = [("arena", C.c_void_p), ("arena_bytes", C.c_int64), ("work1", C.c_void_p),
                 ("nwork1", C.c_int64), ("work2", C.c_void_p), ("nwork2", C.c_int64),
                 ("zseg", C.c_void_p), ("nzseg", C.c_int64), ("outidx", C.c_void_p),
-                ("noutidx", C.c_int64), ("zidx", C.c_void_p), ("nzidx", C.c_int64),
-                ("tdesc", C.c_void_p), ("tsize", C.c_int64), ("tpart_size", C.c_int64),
-                ("work_bytes", C.c_int), ("zseg_bytes", C.c_int)]
+                ("noutidx", C.c_int64), ("codes", C.c_void_p), ("ncodes", C.c_int64),
+                ("tfin", C.c_void_p), ("ntfin", C.c_int64),
+                ("tsize", C.c_int64), ("tpart_size", C.c_int64),
+                ("work_bytes", C.c_int), ("zseg_bytes", C.c_int), ("tfin_bytes", C.c_int)]
 
 
 WORK_DTYPE = np.dtype([("off0", "<i8"), ("off1", "<i8"), ("rows", "<i4"), ("rows_p", "<i4"),
                        ("ncol0", "<i4"), ("ncol1", "<i4"), ("dt0", "<i2"), ("dt1", "<i2"),
                        ("zseg_begin", "<i4"), ("zseg_end", "<i4"), ("out_base", "<i4")])
 ZSEG_DTYPE = np.dtype([("kind", "<i4"), ("ncols", "<i4"), ("src", "<i4"), ("pad", "<i4")])
+TFIN_DTYPE = np.dtype([("slot", "<i4"), ("base", "<i4"), ("np", "<i4"), ("pad", "<i4")])
 
 
 class RefLeafmtxp(C.Structure):
@@ -500,6 +502,7 @@ def plan_tables(hm, precision="f64"):
         t = _PlanTables()
         _check(L.hmv_plan_view(p, C.byref(t)), "hmv_plan_view")
         assert t.work_bytes == WORK_DTYPE.itemsize and t.zseg_bytes == ZSEG_DTYPE.itemsize
+        assert t.tfin_bytes == TFIN_DTYPE.itemsize
 
         def arr(ptr, n, dt):
             if n == 0:
@@ -510,8 +513,7 @@ def plan_tables(hm, precision="f64"):
             "arena": arr(t.arena, t.arena_bytes, np.uint8),
             "work1": arr(t.work1, t.nwork1, WORK_DTYPE), "work2": arr(t.work2, t.nwork2, WORK_DTYPE),
             "zseg": arr(t.zseg, t.nzseg, ZSEG_DTYPE), "outidx": arr(t.outidx, t.noutidx, np.int32),
-            "zidx": arr(t.zidx, t.nzidx, np.int32),
-            "tdesc": arr(t.tdesc, 2 * t.tsize, np.int32).reshape(-1, 2),
+            "codes": arr(t.codes, t.ncodes, np.int32), "tfin": arr(t.tfin, t.ntfin, TFIN_DTYPE),
             "tsize": t.tsize, "tpart_size": t.tpart_size,
         }
     finally:

@@ -27,8 +27,9 @@ int hip_fail(hipError_t e, const char *what) {
 
 hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta, const int *skip,
                          hipStream_t s) {
-  return hmv_launch_apply(a->mode, a->d_arena, a->d_work1, a->nwork1, a->d_work2, a->nwork2,
-                          a->d_zseg, a->d_outidx, a->d_zidx, a->d_tdesc, a->d_t, x, y, beta, skip, s);
+  return hmv_launch_apply(a->mode, a->d_arena, a->d_work1, a->nwork1, a->d_tfin, a->ntfin,
+                          a->d_work2, a->nwork2, a->d_zseg, a->d_outidx, a->d_codes, a->d_t, x, y,
+                          beta, skip, s);
 }
 
 }  // namespace hmv
@@ -63,10 +64,10 @@ void release(hmv_arena_s *a) {
   (void)hipFree(a->d_arena);
   (void)hipFree(a->d_work1);
   (void)hipFree(a->d_work2);
+  (void)hipFree(a->d_tfin);
   (void)hipFree(a->d_zseg);
   (void)hipFree(a->d_outidx);
-  (void)hipFree(a->d_zidx);
-  (void)hipFree(a->d_tdesc);
+  (void)hipFree(a->d_codes);
   (void)hipFree(a->d_t);
   (void)hipFree(a->d_hx);
   (void)hipFree(a->d_hy);
@@ -99,20 +100,21 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
   a->tpart_size = P.tpart_size;
   a->nwork1 = (int)P.work1.size();
   a->nwork2 = (int)P.work2.size();
+  a->ntfin = (int)P.tfin.size();
   hipError_t e;
   if ((e = hipMalloc(&a->d_arena, P.arena_size)) != hipSuccess ||
       (e = hipMemcpy(a->d_arena, P.arena.get(), P.arena_size, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = upload(&a->d_work1, P.work1)) != hipSuccess ||
       (e = upload(&a->d_work2, P.work2)) != hipSuccess ||
+      (e = upload(&a->d_tfin, P.tfin)) != hipSuccess ||
       (e = upload(&a->d_zseg, P.zseg)) != hipSuccess ||
       (e = upload(&a->d_outidx, P.outidx)) != hipSuccess ||
-      (e = upload(&a->d_zidx, P.zidx)) != hipSuccess ||
-      (e = upload(&a->d_tdesc, P.tdesc)) != hipSuccess) {
+      (e = upload(&a->d_codes, P.codes)) != hipSuccess) {
     release(a);
     return hip_fail(e, "hmv_arena_build: upload");
   }
-  size_t tb = (size_t)std::max<int64_t>(1, P.tpart_size) * (a->acc64 ? 8 : 4);
-  if ((e = hipMalloc(&a->d_t, tb)) != hipSuccess) {
+  size_t tb = (size_t)std::max<int64_t>(1, P.tpart_size) * sizeof(double);
+  if ((e = hipMalloc((void **)&a->d_t, tb)) != hipSuccess) {
     release(a);
     return hip_fail(e, "hmv_arena_build: t vector");
   }
@@ -193,7 +195,7 @@ int hmv_arena_free(hmv_arena a) {
 
 int64_t hmv_arena_tbytes(hmv_arena a) {
   if (!a) return -1;
-  return std::max<int64_t>(1, a->tpart_size) * (a->acc64 ? 8 : 4);
+  return std::max<int64_t>(1, a->tpart_size) * (int64_t)sizeof(double);
 }
 
 int hmv_apply(hmv_arena a, const double *x, double *y, double beta, void *stream) {
@@ -301,13 +303,16 @@ int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *m
     for (int r = 0; r < reps && e == hipSuccess; ++r) {
       hipEvent_t e0 = ev[3 * r], e1 = ev[3 * r + 1], e2 = ev[3 * r + 2];
       hipEventRecord(e0, s);
+      // phase 1 = t (rank-vector products and the chunk-partial sums)
       if (a->nwork1 > 0)
-        e = hmv_launch_apply(a->mode, a->d_arena, a->d_work1, a->nwork1, a->d_work2, 0, a->d_zseg,
-                             a->d_outidx, a->d_zidx, a->d_tdesc, a->d_t, x, y, 0.0, nullptr, s);
+        e = hmv_launch_apply(a->mode, a->d_arena, a->d_work1, a->nwork1, a->d_tfin, a->ntfin,
+                             a->d_work2, 0, a->d_zseg, a->d_outidx, a->d_codes, a->d_t, x, y, 0.0,
+                             nullptr, s);
       hipEventRecord(e1, s);
       if (e == hipSuccess)
-        e = hmv_launch_apply(a->mode, a->d_arena, a->d_work1, 0, a->d_work2, a->nwork2, a->d_zseg,
-                             a->d_outidx, a->d_zidx, a->d_tdesc, a->d_t, x, y, 0.0, nullptr, s);
+        e = hmv_launch_apply(a->mode, a->d_arena, a->d_work1, 0, a->d_tfin, 0, a->d_work2,
+                             a->nwork2, a->d_zseg, a->d_outidx, a->d_codes, a->d_t, x, y, 0.0,
+                             nullptr, s);
       hipEventRecord(e2, s);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -394,13 +399,15 @@ int hmv_plan_view(hmv_plan p, hmv_plan_tables *t) {
   t->nzseg = (int64_t)P.zseg.size();
   t->outidx = P.outidx.data();
   t->noutidx = (int64_t)P.outidx.size();
-  t->zidx = P.zidx.data();
-  t->nzidx = (int64_t)P.zidx.size();
-  t->tdesc = P.tdesc.data();
+  t->codes = P.codes.data();
+  t->ncodes = (int64_t)P.codes.size();
+  t->tfin = P.tfin.data();
+  t->ntfin = (int64_t)P.tfin.size();
   t->tsize = P.tsize;
   t->tpart_size = P.tpart_size;
   t->work_bytes = (int)sizeof(HmvWork);
   t->zseg_bytes = (int)sizeof(HmvZSeg);
+  t->tfin_bytes = (int)sizeof(HmvTFin);
   return HMV_OK;
 }
 

@@ -18,13 +18,15 @@ struct hmv_arena_s {
   int acc64 = 1;
   int mode = 0;  // kernel specialisation: 0 FP64, 1 FP32, 2 BF16 low rank + FP32 dense
   int64_t tsize = 0;       // t slots
-  int64_t tpart_size = 0;  // t partial slots
+  int64_t tpart_size = 0;  // t buffer entries (slots + chunk partials)
   char *d_arena = nullptr;
   HmvWork *d_work1 = nullptr, *d_work2 = nullptr;
   int nwork1 = 0, nwork2 = 0;
+  HmvTFin *d_tfin = nullptr;  // chunked rank vectors
+  int ntfin = 0;
   HmvZSeg *d_zseg = nullptr;
-  int *d_outidx = nullptr, *d_zidx = nullptr, *d_tdesc = nullptr;
-  void *d_t = nullptr;     // t partials (accumulation type)
+  int *d_outidx = nullptr, *d_codes = nullptr;
+  double *d_t = nullptr;   // t slots + chunk partials (FP64 in every mode)
   double *d_hx = nullptr, *d_hy = nullptr;  // scratch for hmv_apply_host
   HmvSolveWork *solve = nullptr;             // solver workspace (lazy)
   hmv_arena_stats stats{};
@@ -43,9 +45,10 @@ void free_solve_work(hmv_arena_s *a);
 extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *dst, size_t bytes,
                                              hipStream_t stream);
 extern "C" hipError_t hmv_launch_apply(int mode, const char *arena, const HmvWork *work1,
-                                       int nwork1, const HmvWork *work2, int nwork2,
-                                       const HmvZSeg *zseg, const int *outidx, const int *zidx,
-                                       const int *tdesc, void *tpart, const double *x, double *y,
-                                       double beta, const int *skip, hipStream_t stream);
+                                       int nwork1, const HmvTFin *tfin, int ntfin,
+                                       const HmvWork *work2, int nwork2, const HmvZSeg *zseg,
+                                       const int *outidx, const int *codes, double *t,
+                                       const double *x, double *y, double beta, const int *skip,
+                                       hipStream_t stream);
 
 #endif

@@ -17,11 +17,21 @@
 //     atom (at most 256 rows) and all panel columns of every row cluster
 //     covering it: dense columns (inputs gathered from x) and U columns
 //     (inputs: sums of the t partials).  Each y row has exactly one writer.
+//     The input of every column is given by a code (x index or t slot), so
+//     staging is one flat gather.
+//
+//   Between the phases, rank vectors whose column group was cut into chunks
+//   get their t entry as the sum of the chunk partials in chunk order (one
+//   small launch over the HmvTFin list); every other rank vector's phase-1
+//   workgroup writes its t entry directly.
 //
 // A workgroup stages its inputs z in LDS (zsegs describe where each column's
 // input comes from), then each lane accumulates `VR` fixed rows over a
 // strided set of columns with 8- or 4-byte loads that are contiguous across
 // the workgroup, and one fixed-order LDS reduction produces the outputs.
+// Work lists are ordered so that the workgroups the dispatcher sends to one
+// XCD (every 8th) cover one contiguous eighth of the rows (or columns), which
+// keeps each XCD's gathers in its own L2.
 #ifndef HMV_INTERNAL_H
 #define HMV_INTERNAL_H
 
@@ -37,9 +47,10 @@
 #define HMV_ARENA_ALIGN 256   // byte alignment of every block in the arena
 
 // Sources of staged inputs.
-#define HMV_Z_X 0        // z[c] = x[src + c]
-#define HMV_Z_XGATHER 1  // z[c] = x[zidx[src + c]]
-#define HMV_Z_T 2        // z[c] = sum_p tpart[tdesc[src + c].base + p], p < np
+#define HMV_Z_X 0      // z[c] = x[src + c]
+#define HMV_Z_CODES 1  // z[c] from code = codes[src + c]:
+                       //   code >= 0: x[code];  code < 0: t[-code - 1]
+#define HMV_MAX_TPARTS 32  // t partials per rank vector (column chunks of a group)
 
 // One workgroup's block.
 struct HmvWork {
@@ -53,6 +64,15 @@ struct HmvWork {
   int zseg_begin;   // input sources [zseg_begin, zseg_end), in column order
   int zseg_end;
   int out_base;     // phase 1: offset into outidx; phase 2: first y row
+};
+
+// t[slot] = t[base] + t[base + 1] + ... + t[base + np - 1]  (np >= 2; the
+// partials live past the t slots, base >= number of slots).
+struct HmvTFin {
+  int slot;
+  int base;
+  int np;
+  int pad;
 };
 
 // A run of consecutive input columns with one source.

@@ -22,83 +22,176 @@
 
 namespace {
 
+typedef float float2_nt __attribute__((ext_vector_type(2)));
+
 __device__ inline float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
 __device__ inline float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
 
-// Accumulates columns [ca, cb) of one part (column-major, ld = rows_p) into
-// the lane's row sums; zc points at the staged input of column ca.  DT is
-// the part's storage type (HMV_DT_*).
-template <typename Acc, int DT>
-__device__ inline void accumulate(const char *__restrict__ part, int rows_p, int rq, int cg, int m,
-                                  int ca, int cb, const Acc *__restrict__ zc, Acc &a0, Acc &a1) {
-  if constexpr (DT == HMV_DT_F64) {
-    const double *B = reinterpret_cast<const double *>(part) + rq;
-#pragma unroll 8
-    for (int c = ca + cg; c < cb; c += m)
-      a0 = __builtin_fma((Acc)B[(size_t)c * rows_p], zc[c - ca], a0);
-  } else if constexpr (DT == HMV_DT_F32) {
-    const float *B = reinterpret_cast<const float *>(part) + 2 * rq;
-#pragma unroll 8
-    for (int c = ca + cg; c < cb; c += m) {
-      const float2 v = *reinterpret_cast<const float2 *>(B + (size_t)c * rows_p);
-      const Acc zz = zc[c - ca];
-      a0 = __builtin_fma((Acc)v.x, zz, a0);
-      a1 = __builtin_fma((Acc)v.y, zz, a1);
-    }
-  } else {
-    const uint16_t *B = reinterpret_cast<const uint16_t *>(part) + 2 * rq;
-#pragma unroll 8
-    for (int c = ca + cg; c < cb; c += m) {
-      const uint32_t v = *reinterpret_cast<const uint32_t *>(B + (size_t)c * rows_p);
-      const Acc zz = zc[c - ca];
-      a0 = __builtin_fma((Acc)bf16_lo(v), zz, a0);
-      a1 = __builtin_fma((Acc)bf16_hi(v), zz, a1);
-    }
-  }
+// Buffer resource over one column part (raw, byte-addressed).  A load whose
+// vector offset is at or past num_records returns zero without touching
+// memory; lanes with nothing to load use kOffNone.
+constexpr uint32_t kOffNone = 0x80000000u;
+constexpr int kBufNT = 2;  // cache policy bits: nontemporal
+__device__ inline __amdgpu_buffer_rsrc_t part_rsrc(const char *base, uint32_t nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)nbytes, 0x00020000);
 }
 
-// Stages inputs of columns [c0, c0+cw) into z[0, cw).
+// Per-lane element of one column part: VR consecutive rows of one column.
+template <int DT>
+struct Lane;
+template <>
+struct Lane<HMV_DT_F64> {
+  using raw = double;
+  static constexpr int bytes = 8;
+  __device__ static raw load(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kBufNT));
+  }
+  template <typename Acc>
+  __device__ static void fma(raw v, Acc z, Acc &a0, Acc &) {
+    a0 = __builtin_fma((Acc)v, z, a0);
+  }
+};
+template <>
+struct Lane<HMV_DT_F32> {
+  using raw = float2_nt;
+  static constexpr int bytes = 8;
+  __device__ static raw load(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
+    return __builtin_bit_cast(float2_nt, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kBufNT));
+  }
+  template <typename Acc>
+  __device__ static void fma(raw v, Acc z, Acc &a0, Acc &a1) {
+    a0 = __builtin_fma((Acc)v.x, z, a0);
+    a1 = __builtin_fma((Acc)v.y, z, a1);
+  }
+};
+template <>
+struct Lane<HMV_DT_BF16> {
+  using raw = uint32_t;
+  static constexpr int bytes = 4;
+  __device__ static raw load(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, kBufNT);
+  }
+  template <typename Acc>
+  __device__ static void fma(raw v, Acc z, Acc &a0, Acc &a1) {
+    a0 = __builtin_fma((Acc)bf16_lo(v), z, a0);
+    a1 = __builtin_fma((Acc)bf16_hi(v), z, a1);
+  }
+};
+
+// Stages inputs of columns [c0, c0+cw) into z[0, cw).  Threads take
+// columns c0+tid, c0+tid+256, ... in batches of K: all K index loads, then
+// all K value loads are issued before any is waited for, so a batch costs
+// two memory round trips.
 template <typename Acc>
 __device__ inline void stage(Acc *__restrict__ z, const HmvZSeg *__restrict__ zsegs,
                              const HmvWork &w, int c0, int cw, const double *__restrict__ x,
-                             const int *__restrict__ zidx, const int *__restrict__ tdesc,
-                             const Acc *__restrict__ tpart) {
+                             const int *__restrict__ codes, const double *__restrict__ t) {
+  constexpr int K = 4;
   const int tid = threadIdx.x;
   int col = 0;
   for (int s = w.zseg_begin; s < w.zseg_end && col < c0 + cw; ++s) {
     const HmvZSeg zs = zsegs[s];
     const int lo = max(col, c0), hi = min(col + zs.ncols, c0 + cw);
-    for (int c = lo + tid; c < hi; c += HMV_WG_THREADS) {
-      const int j = c - col;
-      Acc v;
+    for (int cb = lo + tid; cb < hi; cb += K * HMV_WG_THREADS) {
+      const double *src[K];
       if (zs.kind == HMV_Z_X) {
-        v = (Acc)x[zs.src + j];
-      } else if (zs.kind == HMV_Z_XGATHER) {
-        v = (Acc)x[zidx[zs.src + j]];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          src[k] = x + zs.src + (min(cb + k * HMV_WG_THREADS, hi - 1) - col);
       } else {
-        const int slot = zs.src + j;
-        const int b = tdesc[2 * slot], np = tdesc[2 * slot + 1];
-        v = tpart[b];
-        for (int p = 1; p < np; ++p) v += tpart[b + p];
+        int code[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int c = cb + k * HMV_WG_THREADS;
+          code[k] = c < hi ? codes[zs.src + (c - col)] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) src[k] = code[k] >= 0 ? x + code[k] : t + (-code[k] - 1);
       }
-      z[c - c0] = v;
+      double val[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) val[k] = *src[k];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int c = cb + k * HMV_WG_THREADS;
+        if (c < hi) z[c - c0] = (Acc)val[k];
+      }
     }
     col += zs.ncols;
   }
 }
 
-// PHASE 1: tpart[outidx[out_base + r]] = row sums.
+// Accumulates the lane's n columns of one part, starting at byte offset vb
+// (column stride `step` bytes); the input of the lane's j-th column is
+// z[zb + j*m].  Loads go in batches of U issued together; the last, partial
+// batch gives its missing columns the out-of-range offset (they load zero)
+// and the zero slot z[zmax] as input.
+template <typename Acc, int DT, int U>
+__device__ inline void accumulate(__amdgpu_buffer_rsrc_t rs, uint32_t vb, int step, int n,
+                                  const Acc *__restrict__ z, int zb, int m, int zmax, Acc &a0,
+                                  Acc &a1) {
+  using LN = Lane<DT>;
+  int j = 0;
+  for (; j + U <= n; j += U) {
+    typename LN::raw v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) v[k] = LN::load(rs, vb, (j + k) * step);
+#pragma unroll
+    for (int k = 0; k < U; ++k) LN::fma(v[k], z[zb + (j + k) * m], a0, a1);
+  }
+  if (j < n) {
+    typename LN::raw v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) v[k] = LN::load(rs, j + k < n ? vb : kOffNone, (j + k) * step);
+#pragma unroll
+    for (int k = 0; k < U; ++k) LN::fma(v[k], z[min(zb + (j + k) * m, zmax)], a0, a1);
+  }
+}
+
+// One column part of a block: buffer resource, the lane's byte offset of its
+// first column (column cg), and the byte stride of one step of m columns.
+template <int DT>
+struct Part {
+  static constexpr int es = Lane<DT>::bytes / ((DT == HMV_DT_F64) ? 1 : 2);  // bytes per entry
+  __amdgpu_buffer_rsrc_t rs;
+  uint32_t voff;
+  int step;
+  __device__ Part(const char *base, int rows_p, int ncols, int rq, int cg, int m) {
+    constexpr int vr = (DT == HMV_DT_F64) ? 1 : 2;
+    rs = part_rsrc(base, (uint32_t)((size_t)rows_p * ncols * es));
+    voff = (uint32_t)(rq * vr + cg * rows_p) * (uint32_t)es;
+    step = m * rows_p * es;
+  }
+};
+
+// Lane's columns of local range [lo, hi) of a part: local column cg + j*m,
+// j in [ceil((lo-cg)/m), ceil((hi-cg)/m)); zshift maps a local column to its
+// staged slot (slot = local column + zshift).
+template <typename Acc, int DT, int U>
+__device__ inline void run_cols(const Part<DT> &P, int lo, int hi, int cg, int m, int zshift,
+                                int zmax, const Acc *__restrict__ z, Acc &a0, Acc &a1) {
+  const int j0 = lo > cg ? (lo - cg + m - 1) / m : 0;
+  const int j1 = hi > cg ? (hi - cg + m - 1) / m : 0;
+  if (j1 > j0)
+    accumulate<Acc, DT, U>(P.rs, P.voff + (uint32_t)(j0 * P.step), P.step, j1 - j0, z,
+                           cg + j0 * m + zshift, m, zmax, a0, a1);
+}
+
+// PHASE 1: t[outidx[out_base + r]] = row sums.
 // PHASE 2: y[out_base + r] = row sums + beta * y.
 // DT0/DT1: storage types of the two column parts (fixed per arena precision).
 template <typename Acc, int PHASE, int DT0, int DT1>
-__global__ __launch_bounds__(HMV_WG_THREADS) void hmv_stream_kernel(
+__global__ __launch_bounds__(HMV_WG_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+hmv_stream_kernel(
     const char *__restrict__ arena, const HmvWork *__restrict__ works,
-    const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ zidx,
-    const int *__restrict__ tdesc, const Acc *__restrict__ tpart_in, const int *__restrict__ outidx,
-    Acc *__restrict__ tpart_out, double *__restrict__ y, double beta, const int *__restrict__ skip) {
+    const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ codes,
+    const double *__restrict__ t_in, const int *__restrict__ outidx, double *__restrict__ t_out,
+    double *__restrict__ y, double beta, const int *__restrict__ skip) {
   if (skip && *skip) return;
-  __shared__ __attribute__((aligned(16))) Acc z[HMV_ZCHUNK];
-  __shared__ Acc red[2 * HMV_WG_THREADS];
+  constexpr int U = 4;
+  // inputs of one chunk plus the zero slot; reused for the row reduction
+  __shared__ __attribute__((aligned(16))) Acc z[HMV_ZCHUNK + 4];
+  static_assert(2 * HMV_WG_THREADS <= HMV_ZCHUNK, "reduction buffer fits in z");
   const HmvWork w = works[blockIdx.x];
   const int tid = threadIdx.x;
   constexpr int vr = (DT0 == HMV_DT_F64) ? 1 : 2;
@@ -108,20 +201,41 @@ __global__ __launch_bounds__(HMV_WG_THREADS) void hmv_stream_kernel(
   const int rq = tid % q, cg = tid / q;
   Acc a0 = Acc(0), a1 = Acc(0);
   const int ncols = w.ncol0 + w.ncol1;
+  const Part<DT0> P0(arena + w.off0, w.rows_p, w.ncol0, rq, cg, m);
+  // The lane's first PF columns of part 0 do not depend on the staged
+  // inputs: their loads are issued before staging, so the first HBM round
+  // trip overlaps the gathers and the barrier.
+  constexpr int PF = 4;
+  const int lim0 = min(w.ncol0, HMV_ZCHUNK);
+  const int npf = active ? max(0, min(PF, (lim0 - cg + m - 1) / m)) : 0;
+  typename Lane<DT0>::raw pre[PF];
+#pragma unroll
+  for (int k = 0; k < PF; ++k) pre[k] = Lane<DT0>::load(P0.rs, k < npf ? P0.voff : kOffNone, k * P0.step);
   for (int c0 = 0; c0 < ncols; c0 += HMV_ZCHUNK) {
     const int cw = min(HMV_ZCHUNK, ncols - c0);
     if (c0 > 0) __syncthreads();
-    stage<Acc>(z, zsegs, w, c0, cw, x, zidx, tdesc, tpart_in);
+    stage<Acc>(z, zsegs, w, c0, cw, x, codes, t_in);
+    if (tid == 0) z[cw] = Acc(0);
     __syncthreads();
     if (active) {
-      const int e0 = min(c0 + cw, w.ncol0);
-      if (c0 < e0) accumulate<Acc, DT0>(arena + w.off0, w.rows_p, rq, cg, m, c0, e0, z, a0, a1);
-      const int s1 = max(c0, w.ncol0);
-      if (s1 < c0 + cw)
-        accumulate<Acc, DT1>(arena + w.off1, w.rows_p, rq, cg, m, s1 - w.ncol0, c0 + cw - w.ncol0,
-                             z + (s1 - c0), a0, a1);
+      int lo0 = c0;
+      if (c0 == 0) {
+#pragma unroll
+        for (int k = 0; k < PF; ++k) Lane<DT0>::fma(pre[k], z[min(cg + k * m, cw)], a0, a1);
+        lo0 = min(npf * m, w.ncol0);  // columns < npf*m are done (for this lane: j < npf)
+        lo0 = npf == PF ? lo0 : cw;   // a lane with fewer than PF columns here has none left
+      }
+      const int hi0 = min(c0 + cw, w.ncol0);
+      if (lo0 < hi0) run_cols<Acc, DT0, U>(P0, lo0, hi0, cg, m, -c0, cw, z, a0, a1);
+      if (w.ncol1 > 0 && c0 + cw > w.ncol0) {
+        const Part<DT1> P1(arena + w.off1, w.rows_p, w.ncol1, rq, cg, m);
+        run_cols<Acc, DT1, U>(P1, max(c0, w.ncol0) - w.ncol0, c0 + cw - w.ncol0, cg, m,
+                              w.ncol0 - c0, cw, z, a0, a1);
+      }
     }
   }
+  Acc *red = z;
+  __syncthreads();
   if (active) {
     red[cg * w.rows_p + rq * vr] = a0;
     if constexpr (vr == 2) red[cg * w.rows_p + rq * vr + 1] = a1;
@@ -131,7 +245,7 @@ __global__ __launch_bounds__(HMV_WG_THREADS) void hmv_stream_kernel(
     Acc s = red[r];
     for (int g = 1; g < m; ++g) s += red[g * w.rows_p + r];
     if constexpr (PHASE == 1) {
-      tpart_out[outidx[w.out_base + r]] = s;
+      t_out[outidx[w.out_base + r]] = (double)s;
     } else {
       const int row = w.out_base + r;
       const double v = (double)s;
@@ -140,20 +254,42 @@ __global__ __launch_bounds__(HMV_WG_THREADS) void hmv_stream_kernel(
   }
 }
 
+// t[slot] = sum of the chunk partials t[base + p], p = 0 .. np-1, in order.
+__global__ __launch_bounds__(HMV_WG_THREADS) void hmv_tfinal_kernel(
+    const HmvTFin *__restrict__ fin, int nfin, double *__restrict__ t, const int *__restrict__ skip) {
+  if (skip && *skip) return;
+  const int i = blockIdx.x * HMV_WG_THREADS + threadIdx.x;
+  if (i >= nfin) return;
+  const HmvTFin f = fin[i];
+  constexpr int K = 8;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = k < f.np ? t[f.base + k] : 0.0;
+  double s = v[0];
+#pragma unroll
+  for (int k = 1; k < K; ++k)
+    if (k < f.np) s += v[k];
+  for (int p = K; p < f.np; ++p) s += t[f.base + p];
+  t[f.slot] = s;
+}
+
 template <typename Acc, int DT0, int DT1>
-void launch_mode(const char *arena, const HmvWork *work1, int nwork1, const HmvWork *work2,
-                 int nwork2, const HmvZSeg *zseg, const int *outidx, const int *zidx,
-                 const int *tdesc, void *tpart, const double *x, double *y, double beta,
-                 const int *skip, hipStream_t stream) {
+void launch_mode(const char *arena, const HmvWork *work1, int nwork1, const HmvTFin *tfin,
+                 int ntfin, const HmvWork *work2, int nwork2, const HmvZSeg *zseg,
+                 const int *outidx, const int *codes, double *t, const double *x, double *y,
+                 double beta, const int *skip, hipStream_t stream) {
   const dim3 block(HMV_WG_THREADS);
   // phase 1 reads only low-rank factors: both parts use DT1's type
   if (nwork1 > 0)
     hipLaunchKernelGGL((hmv_stream_kernel<Acc, 1, DT1, DT1>), dim3(nwork1), block, 0, stream, arena,
-                       work1, zseg, x, zidx, tdesc, (const Acc *)nullptr, outidx, (Acc *)tpart,
+                       work1, zseg, x, codes, (const double *)nullptr, outidx, t,
                        (double *)nullptr, 0.0, skip);
+  if (ntfin > 0)
+    hipLaunchKernelGGL(hmv_tfinal_kernel, dim3((ntfin + HMV_WG_THREADS - 1) / HMV_WG_THREADS), block,
+                       0, stream, tfin, ntfin, t, skip);
   if (nwork2 > 0)
     hipLaunchKernelGGL((hmv_stream_kernel<Acc, 2, DT0, DT1>), dim3(nwork2), block, 0, stream, arena,
-                       work2, zseg, x, zidx, tdesc, (const Acc *)tpart, outidx, (Acc *)nullptr, y,
+                       work2, zseg, x, codes, (const double *)t, outidx, (double *)nullptr, y,
                        beta, skip);
 }
 
@@ -191,21 +327,23 @@ extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *ds
   return hipGetLastError();
 }
 
-// Launches phase 1 (nwork1 workgroups) and/or phase 2 (nwork2) of y = H x + beta*y.
+// Launches phase 1 (nwork1 workgroups; then the ntfin chunk-partial sums)
+// and/or phase 2 (nwork2) of y = H x + beta*y.
 // mode: 0 = FP64 arena, 1 = FP32 arena, 2 = BF16 low-rank + FP32 dense arena.
 extern "C" hipError_t hmv_launch_apply(int mode, const char *arena, const HmvWork *work1,
-                                       int nwork1, const HmvWork *work2, int nwork2,
-                                       const HmvZSeg *zseg, const int *outidx, const int *zidx,
-                                       const int *tdesc, void *tpart, const double *x, double *y,
-                                       double beta, const int *skip, hipStream_t stream) {
+                                       int nwork1, const HmvTFin *tfin, int ntfin,
+                                       const HmvWork *work2, int nwork2, const HmvZSeg *zseg,
+                                       const int *outidx, const int *codes, double *t,
+                                       const double *x, double *y, double beta, const int *skip,
+                                       hipStream_t stream) {
   if (mode == 0)
-    launch_mode<double, HMV_DT_F64, HMV_DT_F64>(arena, work1, nwork1, work2, nwork2, zseg, outidx,
-                                                zidx, tdesc, tpart, x, y, beta, skip, stream);
+    launch_mode<double, HMV_DT_F64, HMV_DT_F64>(arena, work1, nwork1, tfin, ntfin, work2, nwork2,
+                                                zseg, outidx, codes, t, x, y, beta, skip, stream);
   else if (mode == 1)
-    launch_mode<float, HMV_DT_F32, HMV_DT_F32>(arena, work1, nwork1, work2, nwork2, zseg, outidx,
-                                               zidx, tdesc, tpart, x, y, beta, skip, stream);
+    launch_mode<float, HMV_DT_F32, HMV_DT_F32>(arena, work1, nwork1, tfin, ntfin, work2, nwork2,
+                                               zseg, outidx, codes, t, x, y, beta, skip, stream);
   else
-    launch_mode<float, HMV_DT_F32, HMV_DT_BF16>(arena, work1, nwork1, work2, nwork2, zseg, outidx,
-                                                zidx, tdesc, tpart, x, y, beta, skip, stream);
+    launch_mode<float, HMV_DT_F32, HMV_DT_BF16>(arena, work1, nwork1, tfin, ntfin, work2, nwork2,
+                                                zseg, outidx, codes, t, x, y, beta, skip, stream);
   return hipGetLastError();
 }

@@ -166,7 +166,6 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     std::vector<int> dense, lr;
     int wd = 0, k = 0;
     int64_t tbase = 0;
-    int64_t zoff = 0;
   };
   struct ColGroup {
     int s, n;
@@ -221,11 +220,9 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   });
   std::vector<int64_t> leaf_tbase(nlf, -1);
   std::vector<int> leaf_cg(nlf, -1);
-  int64_t tcur = 0, zcur = 0;
+  int64_t tcur = 0;
   for (int r : rg_order) {
     rg[r].tbase = tcur;
-    rg[r].zoff = zcur;
-    zcur += rg[r].wd;
     for (int i : rg[r].lr) {
       leaf_tbase[i] = tcur;
       tcur += L[i]->kt;
@@ -255,6 +252,7 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
       // long rank vectors: chunk the columns; chunk sums are t partials
       rs = std::min(g.r, 16);
       cw = (int)std::max<int64_t>(256, tune.p1_target_bytes / (round_up(rs, vr) * es_l));
+      cw = (int)std::max<int64_t>(cw, (n + HMV_MAX_TPARTS - 1) / HMV_MAX_TPARTS);
       if (cw >= g.n) cw = g.n;
     }
     g.np = (int)((n + cw - 1) / cw);
@@ -266,21 +264,28 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
       }
     }
   }
-  // t partial slots: per t slot, np consecutive partials
-  P.tdesc.assign(2 * (size_t)P.tsize, 0);
-  int64_t pcur = 0;
+  // t buffer: the tsize slots, then the partials of rank vectors whose
+  // column group is chunked (np > 1).  tout[s] = where chunk 0 of slot s
+  // lands (chunk k at tout[s] + k).
+  std::vector<int64_t> tout((size_t)P.tsize, 0);
+  int64_t pcur = P.tsize;
   for (int r : rg_order)
     for (int i : rg[r].lr)
       for (int k = 0; k < L[i]->kt; ++k) {
         const int64_t s = leaf_tbase[i] + k;
-        P.tdesc[2 * s] = (int)pcur;
-        P.tdesc[2 * s + 1] = cg[leaf_cg[i]].np;
-        pcur += cg[leaf_cg[i]].np;
+        const int np = cg[leaf_cg[i]].np;
+        if (np == 1) {
+          tout[s] = s;
+        } else {
+          tout[s] = pcur;
+          P.tfin.push_back(HmvTFin{(int)s, (int)pcur, np, 0});
+          pcur += np;
+        }
       }
   P.tpart_size = pcur;
   st.tparts = pcur;
   if (pcur > INT32_MAX) {
-    err = "too many t partials for 32-bit indexing";
+    err = "too many t entries for 32-bit input codes";
     return false;
   }
   // rank vectors of each column group, in group order: (leaf, l)
@@ -366,10 +371,6 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     }
     st.phase2_bytes += S.bytes;
   }
-  if (zcur > INT32_MAX) {
-    err = "problem too large for 32-bit gather indices";
-    return false;
-  }
   st.arena_bytes = std::max<int64_t>(off, HMV_ARENA_ALIGN);
   st.payload_bytes = st.entries_lr * es_l + st.entries_dense * es_d;
   st.alg_bytes = st.payload_bytes + 16LL * nd;
@@ -378,7 +379,6 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     // thread that fills it, so the image is first touched in parallel
     P.arena.reset(new char[(size_t)st.arena_bytes]);
     P.arena_size = (size_t)st.arena_bytes;
-    P.zidx.assign((size_t)zcur, 0);
   } catch (...) {
     err = "out of host memory for the arena image";
     return false;
@@ -388,13 +388,10 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   lap("layout");
 
   // ---- tables -------------------------------------------------------------
-  for (int r : rg_order) {
-    int64_t zc = rg[r].zoff;
-    for (int i : rg[r].dense)
-      for (int j = 0; j < L[i]->ndt; ++j) P.zidx[zc++] = L[i]->nstrtt - 1 + j;
-  }
+  auto tcode = [&](int64_t slot) { return (int)(-1 - slot); };
   struct Item {
     int64_t bytes;
+    int64_t key;  // locality: first x column (phase 1) or first row (phase 2)
     HmvWork w;
   };
   std::vector<Item> it1, it2;
@@ -417,9 +414,9 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     for (int r = 0; r < p.nr; ++r) {
       const auto &rv = cg_rows[p.c][p.r0 + r];
       const int64_t s = leaf_tbase[rv.first] + rv.second;
-      P.outidx.push_back(P.tdesc[2 * s] + p.k);
+      P.outidx.push_back((int)(tout[s] + p.k));
     }
-    it1.push_back({p.bytes, w});
+    it1.push_back({p.bytes, (int64_t)g.s + p.c0, w});
   }
   it2.reserve(slices.size());
   for (auto &S : slices) {
@@ -433,20 +430,33 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     w.dt0 = (short)dt_dense;
     w.dt1 = (short)dt_lr;
     w.zseg_begin = (int)P.zseg.size();
+    const int64_t cbase = (int64_t)P.codes.size();
+    auto dense_codes = [&](const RowGroup &g) {
+      for (int i : g.dense)
+        for (int j = 0; j < L[i]->ndt; ++j) P.codes.push_back(L[i]->nstrtt - 1 + j);
+    };
+    auto t_codes = [&](const RowGroup &g) {
+      for (int64_t k = 0; k < g.k; ++k) P.codes.push_back(tcode(g.tbase + k));
+    };
     if (merged) {
       for (int gi : S.cov) {
-        if (rg[gi].wd > 0) P.zseg.push_back(HmvZSeg{HMV_Z_XGATHER, rg[gi].wd, (int)rg[gi].zoff, 0});
-        if (rg[gi].k > 0) P.zseg.push_back(HmvZSeg{HMV_Z_T, rg[gi].k, (int)rg[gi].tbase, 0});
+        dense_codes(rg[gi]);
+        t_codes(rg[gi]);
       }
     } else {
-      for (int gi : S.cov)
-        if (rg[gi].wd > 0) P.zseg.push_back(HmvZSeg{HMV_Z_XGATHER, rg[gi].wd, (int)rg[gi].zoff, 0});
-      for (int gi : S.cov)
-        if (rg[gi].k > 0) P.zseg.push_back(HmvZSeg{HMV_Z_T, rg[gi].k, (int)rg[gi].tbase, 0});
+      for (int gi : S.cov) dense_codes(rg[gi]);
+      for (int gi : S.cov) t_codes(rg[gi]);
+    }
+    if (S.ncol0 + S.ncol1 > 0) {
+      if ((int64_t)P.codes.size() - cbase != S.ncol0 + S.ncol1 || cbase > INT32_MAX) {
+        err = "internal: code table does not match the block columns";
+        return false;
+      }
+      P.zseg.push_back(HmvZSeg{HMV_Z_CODES, S.ncol0 + S.ncol1, (int)cbase, 0});
     }
     w.zseg_end = (int)P.zseg.size();
     w.out_base = S.a;
-    it2.push_back({S.bytes, w});
+    it2.push_back({S.bytes, S.a, w});
   }
   lap("tables");
 
@@ -507,12 +517,35 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   });
   lap("fill");
 
-  // largest workgroups first so the dispatcher's in-order launch ends balanced
-  auto by_size = [](const Item &x, const Item &y) { return x.bytes > y.bytes; };
-  std::stable_sort(it1.begin(), it1.end(), by_size);
-  std::stable_sort(it2.begin(), it2.end(), by_size);
-  for (auto &i : it1) P.work1.push_back(i.w);
-  for (auto &i : it2) P.work2.push_back(i.w);
+  // Dispatch order.  The hardware sends workgroup b to XCD b % G; give each
+  // XCD one contiguous G-th of the locality-sorted list (its gathers then
+  // stay in its own L2), largest workgroups first within each XCD so its
+  // in-order stream ends balanced.
+  auto order = [&](std::vector<Item> &items, std::vector<HmvWork> &out) {
+    const int G = std::max(1, tune.xcd_groups);
+    std::stable_sort(items.begin(), items.end(),
+                     [](const Item &x, const Item &y) { return x.key < y.key; });
+    const size_t n = items.size();
+    std::vector<std::vector<Item>> chunk(G);
+    // the first n % G chunks take one extra item, so that workgroup b of the
+    // interleaved list always belongs to chunk b % G
+    const size_t q = n / G, r = n % G;
+    size_t b = 0;
+    for (int k = 0; k < G; ++k) {
+      const size_t e = b + q + ((size_t)k < r ? 1 : 0);
+      chunk[k].assign(items.begin() + b, items.begin() + e);
+      b = e;
+      std::stable_sort(chunk[k].begin(), chunk[k].end(),
+                       [](const Item &x, const Item &y) { return x.bytes > y.bytes; });
+    }
+    out.clear();
+    out.reserve(n);
+    for (size_t pos = 0; out.size() < n; ++pos)
+      for (int k = 0; k < G; ++k)
+        if (pos < chunk[k].size()) out.push_back(chunk[k][pos].w);
+  };
+  order(it1, P.work1);
+  order(it2, P.work2);
   lap("order");
   return true;
 }

@@ -32,7 +32,7 @@ struct PlanStats {
   int64_t entries_lr = 0;     // sum_LR kt*(ndl+ndt)
   int64_t entries_dense = 0;  // sum_D ndl*ndt
   int64_t tslots = 0;         // sum_LR kt
-  int64_t tparts = 0;         // t partial slots (>= tslots)
+  int64_t tparts = 0;         // t buffer entries (slots + chunk partials)
   int64_t col_groups = 0, row_groups = 0, atoms = 0;
   int64_t arena_bytes = 0;    // bytes of the arena including padding
   int64_t payload_bytes = 0;  // bytes of factor/dense entries at storage precision
@@ -47,14 +47,14 @@ struct Plan {
   int nd = 0;
   int prec = kF64;
   int64_t tsize = 0;               // t slots (one per rank vector)
-  int64_t tpart_size = 0;          // partial slots written by phase 1
+  int64_t tpart_size = 0;          // t buffer entries: tsize slots, then chunk partials
   std::unique_ptr<char[]> arena;   // device image of the arena (arena_size bytes)
   size_t arena_size = 0;
   std::vector<HmvWork> work1, work2;
   std::vector<HmvZSeg> zseg;       // input sources of both phases
   std::vector<int> outidx;         // phase-1 output row -> t partial slot
-  std::vector<int> zidx;           // phase-2 gather indices into x
-  std::vector<int> tdesc;          // per t slot: {first partial, number of partials}
+  std::vector<int> codes;          // phase-2 column input codes (HMV_Z_CODES)
+  std::vector<HmvTFin> tfin;       // chunked rank vectors: t slot = sum of partials
   PlanStats st;
 };
 
@@ -63,6 +63,7 @@ struct PlanTuning {
   int64_t p1_target_bytes = 96 * 1024;  // arena bytes per phase-1 workgroup
   int p1_row_tile = 256;                // max rank vectors per phase-1 workgroup
   int p2_max_rows = HMV_MAX_ROWS;       // max rows per phase-2 workgroup
+  int xcd_groups = 8;                   // dispatcher round-robin width (XCDs)
 };
 
 // Builds the plan.  `leaves` points at nlf records spaced `stride` bytes.

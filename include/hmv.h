@@ -63,7 +63,7 @@ typedef struct {
   int64_t entries_lr;     /* sum_LR kt*(ndl+ndt)                               */
   int64_t entries_dense;  /* sum_D ndl*ndt                                     */
   int64_t tslots;         /* sum_LR kt (rank vectors)                          */
-  int64_t tparts;         /* t partial slots written by phase 1 (>= tslots)    */
+  int64_t tparts;         /* t buffer entries: slots + column-chunk partials   */
   int64_t phase1_bytes;   /* arena bytes read by the V^T x phase                */
   int64_t phase2_bytes;   /* arena bytes read by the [A|U] phase                */
   int64_t arena_bytes;    /* device bytes of factors incl. alignment padding   */
@@ -166,8 +166,12 @@ int hmv_stream_test(int kind, const void *src, void *dst, size_t bytes, int reps
  * CPU-side validation of the arena's indexing.  Tables use the engine's
  * internal record layouts: work = {int64 off0, off1; int rows, rows_p,
  * ncol0, ncol1; int16 dt0, dt1; int zseg_begin, zseg_end, out_base}
- * (48 bytes), zseg = {int kind, ncols, src, pad} (16 bytes), tdesc = 2 ints
- * per t slot {first partial, partials}. */
+ * (48 bytes), zseg = {int kind, ncols, src, pad} (16 bytes), codes = one
+ * int per phase-2 column (>= 0: x index; < 0: t slot -1-code), tfin =
+ * {int slot, base, np, pad} (16 bytes): t[slot] = t[base] + ... +
+ * t[base+np-1] for rank vectors whose column group is cut into np chunks.
+ * The t buffer holds tsize slots followed by those partials (tpart_size
+ * entries in all). */
 typedef struct hmv_plan_s *hmv_plan;
 typedef struct {
   const void *arena;   int64_t arena_bytes;
@@ -175,10 +179,10 @@ typedef struct {
   const void *work2;   int64_t nwork2;
   const void *zseg;    int64_t nzseg;
   const int *outidx;   int64_t noutidx;
-  const int *zidx;     int64_t nzidx;
-  const int *tdesc;
+  const int *codes;    int64_t ncodes;
+  const void *tfin;    int64_t ntfin;
   int64_t tsize, tpart_size;
-  int work_bytes, zseg_bytes;
+  int work_bytes, zseg_bytes, tfin_bytes;
 } hmv_plan_tables;
 int hmv_plan_create(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
                     hmv_plan *out);

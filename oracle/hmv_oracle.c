@@ -20,8 +20,9 @@
  *     same leaves (closed definition of the leaf apply), against each other, and
  *     against golden y=Hx fixtures in tests/golden.
  *   - bicgstab: against the residual history recorded by the reference at
- *     N=10,400 (ivb/job_cpu.sh.o78632:29-37), against scipy's BiCGSTAB-free
- *     direct solve on small dense systems, and the true-residual invariant.
+ *     N=10,400 (ivb/job_cpu.sh.o78632:29-37), against the iterates of scipy's
+ *     own BiCGSTAB (same method and shadow residual) on a small H-matrix, and
+ *     the true-residual invariant on small dense systems.
  *   - setcutthread: leaf ranges tile 1..nlf (invariant test).
  */
 #include <math.h>

@@ -12,7 +12,8 @@ ZCHUNK = 2048
 MAX_ROWS = 256
 THREADS = 256
 DT_BYTES = {0: 8, 1: 4, 2: 2}
-Z_X, Z_XGATHER, Z_T = 0, 1, 2
+Z_X, Z_CODES = 0, 1
+MAX_TPARTS = 32
 
 
 def _decode(raw, dt):
@@ -45,8 +46,8 @@ def _lane_layout(w):
     return vr, q, m
 
 
-def _stage(tables, w, x, tpart, nd):
-    zsegs, zidx, tdesc = tables["zseg"], tables["zidx"], tables["tdesc"]
+def _stage(tables, w, x, tbuf, tsize, nd):
+    zsegs, codes = tables["zseg"], tables["codes"]
     zb, ze = int(w["zseg_begin"]), int(w["zseg_end"])
     assert 0 <= zb <= ze <= len(zsegs)
     parts = []
@@ -56,16 +57,19 @@ def _stage(tables, w, x, tpart, nd):
         if kind == Z_X:
             assert 0 <= src and src + n <= nd
             parts.append(x[src:src + n])
-        elif kind == Z_XGATHER:
-            assert 0 <= src and src + n <= zidx.size
-            xi = zidx[src:src + n]
-            assert xi.min() >= 0 and xi.max() < nd
-            parts.append(x[xi])
         else:
-            assert kind == Z_T and 0 <= src and src + n <= len(tdesc)
-            d = tdesc[src:src + n]
-            assert d[:, 0].min() >= 0 and (d[:, 0] + d[:, 1]).max() <= tpart.size and d[:, 1].min() >= 1
-            parts.append(np.array([tpart[b:b + k].sum() for b, k in d]))
+            assert kind == Z_CODES and 0 <= src and src + n <= codes.size
+            out = np.empty(n)
+            for j, code in enumerate(codes[src:src + n]):
+                code = int(code)
+                if code >= 0:
+                    assert code < nd
+                    out[j] = x[code]
+                else:
+                    slot = -code - 1
+                    assert 0 <= slot < tsize, "codes address only final t slots"
+                    out[j] = tbuf[slot]
+            parts.append(out)
     z = np.concatenate(parts) if parts else np.zeros(0)
     assert z.size == int(w["ncol0"]) + int(w["ncol1"]), "zsegs must cover every column"
     return z
@@ -83,31 +87,45 @@ def _block_product(arena, w, z):
 
 
 def run(tables, x, nd):
-    """Returns (tpart, y) computed as the two kernel phases would, with bounds checks."""
+    """Returns (t, y) computed as the kernel launches would (phase 1, the
+    chunk-partial sums, phase 2), with bounds checks.  t is the tsize final
+    slots."""
     arena = tables["arena"]
+    tsize = int(tables["tsize"])
     tpart_size = int(tables["tpart_size"])
+    assert tpart_size >= tsize
     outidx = tables["outidx"]
-    tpart = np.full(max(tpart_size, 1), np.nan)
+    tbuf = np.full(max(tpart_size, 1), np.nan)
     written = np.zeros(max(tpart_size, 1), bool)
     for w in tables["work1"]:
         _lane_layout(w)
         assert int(w["ncol1"]) == 0
-        z = _stage(tables, w, x, tpart, nd)
+        z = _stage(tables, w, x, tbuf, tsize, nd)
         out = _block_product(arena, w, z)
         ob, rows = int(w["out_base"]), int(w["rows"])
         assert 0 <= ob and ob + rows <= outidx.size
         idx = outidx[ob:ob + rows]
         assert idx.min() >= 0 and idx.max() < tpart_size
-        assert not written[idx].any(), "a t partial is written twice"
+        assert not written[idx].any(), "a t entry is written twice"
         written[idx] = True
-        tpart[idx] = out
-    if tpart_size:
-        assert written[:tpart_size].all(), "some t partials are never written"
+        tbuf[idx] = out
+    fin = tables["tfin"]
+    for f in fin:
+        slot, base, k = int(f["slot"]), int(f["base"]), int(f["np"])
+        assert 0 <= slot < tsize and tsize <= base and base + k <= tpart_size and 2 <= k <= MAX_TPARTS
+        assert written[base:base + k].all() and not written[slot]
+        s = tbuf[base]
+        for p in range(1, k):
+            s += tbuf[base + p]
+        tbuf[slot] = s
+        written[slot] = True
+    if tsize:
+        assert written[:tsize].all(), "some t slots are never produced"
     y = np.full(nd, np.nan)
     wy = np.zeros(nd, bool)
     for w in tables["work2"]:
         _lane_layout(w)
-        z = _stage(tables, w, x, tpart, nd)
+        z = _stage(tables, w, x, tbuf, tsize, nd)
         out = _block_product(arena, w, z) if z.size else np.zeros(int(w["rows"]))
         ob, rows = int(w["out_base"]), int(w["rows"])
         assert 0 <= ob and ob + rows <= nd
@@ -115,4 +133,4 @@ def run(tables, x, nd):
         wy[ob:ob + rows] = True
         y[ob:ob + rows] = out
     assert wy.all(), "some y rows are never written"
-    return tpart[:tpart_size], y
+    return tbuf[:tsize], y

@@ -30,21 +30,37 @@ def test_plan_alignment_and_order(H, sphere4):
         for w in (T["work1"], T["work2"]):
             assert np.all(w["off0"][w["ncol0"] > 0] % 256 == 0)
             assert np.all(w["off1"][w["ncol1"] > 0] % 256 == 0)
-        # workgroups are dispatched largest first
+        # the dispatcher sends workgroup b to XCD b % 8: within each XCD's
+        # stream workgroups come largest first, and the XCDs' phase-2 row
+        # ranges are disjoint (each XCD owns a contiguous eighth of the rows)
         for w in (T["work1"], T["work2"]):
             es0 = np.where(w["dt0"] == 0, 8, np.where(w["dt0"] == 1, 4, 2))
             es1 = np.where(w["dt1"] == 0, 8, np.where(w["dt1"] == 1, 4, 2))
             size = w["rows_p"] * (w["ncol0"] * es0 + w["ncol1"] * es1)
-            assert np.all(np.diff(size) <= 0)
+            for k in range(8):
+                assert np.all(np.diff(size[k::8]) <= 0)
+        rows = T["work2"]["out_base"]
+        spans = sorted((rows[k::8].min(), rows[k::8].max()) for k in range(8))
+        assert all(a[1] < b[0] for a, b in zip(spans, spans[1:]))
 
 
 def test_long_column_groups_are_chunked(H, O):
-    """A wide low-rank block is cut into column chunks whose sums are t
-    partials; the emulated apply still equals the oracle."""
+    """A wide low-rank block is cut into column chunks whose partial sums sit
+    past the t slots and are summed by the finalize step; the emulated apply
+    still equals the oracle."""
     m = H.merge(H.icosphere(4), H.icosphere(4, 1.0, (3.0, 0.0, 0.0)))
     hm = H.HMatrix(*m, H.baseline_params(aca_eps=1e-5))
     T = H.plan_tables(hm, "f64")
-    assert T["tpart_size"] >= T["tsize"]
+    fin = T["tfin"]
+    assert len(fin) > 0 and np.all(fin["np"] >= 2)
+    assert T["tpart_size"] == T["tsize"] + fin["np"].sum()
+    # partial ranges tile [tsize, tpart_size) and each chunked slot appears once
+    order = np.argsort(fin["base"])
+    assert fin["base"][order][0] == T["tsize"]
+    assert np.all(np.diff(fin["base"][order]) == fin["np"][order][:-1])
+    assert len(np.unique(fin["slot"])) == len(fin)
+    # codes address final slots only
+    assert T["codes"].min() >= -T["tsize"]
     x = H.uniform(2, hm.nd)
     _, y = PE.run(T, x, hm.nd)
     y_ref = O.adot_body_lfmtx(hm, x)
