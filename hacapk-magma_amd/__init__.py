@@ -173,7 +173,7 @@ def hmv_lib():
         L.hmv_bicgstab.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, _D,
                                    C.POINTER(SolveInfo)]
         L.hmv_bicgstab_ir.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
-                                      _D, C.POINTER(SolveInfo)]
+                                      C.c_int, _D, C.POINTER(SolveInfo)]
         L.hmv_last_error.restype = C.c_char_p
         L.hmv_device_malloc.argtypes = [C.POINTER(vp), C.c_size_t]
         L.hmv_device_free.argtypes = [vp]
@@ -520,15 +520,18 @@ def plan_tables(hm, precision="f64"):
         L.hmv_plan_free(p)
 
 
-def bicgstab_ir(a64, alow, b, u0=None, eps=1e-8, inner_eps=1e-4, max_outer=30, max_inner=1000):
-    """Mixed-precision refinement (FP64 residual, low-precision inner BiCGSTAB)."""
+def bicgstab_ir(a64, alow, b, u0=None, eps=1e-8, inner_eps=1e-4, max_outer=30, max_inner=1000,
+                check_every=8):
+    """Mixed-precision refinement (FP64 residual, low-precision inner BiCGSTAB;
+    the inner solver reads its residual every `check_every` iterations)."""
     b = np.ascontiguousarray(b, np.float64)
     u0 = np.zeros_like(b) if u0 is None else np.ascontiguousarray(u0, np.float64)
     db, du = DeviceBuffer.from_array(b), DeviceBuffer.from_array(u0)
     hist = np.full(max_outer + 1, np.nan)
     info = SolveInfo()
     _check(hmv_lib().hmv_bicgstab_ir(a64.handle, alow.handle, db.ptr, du.ptr, eps, inner_eps,
-                                     max_outer, max_inner, _dptr(hist), C.byref(info)),
+                                     max_outer, max_inner, int(check_every), _dptr(hist),
+                                     C.byref(info)),
            "hmv_bicgstab_ir")
     u = du.download()
     return u, info.as_dict(), hist[: info.outer + 1]

@@ -167,6 +167,14 @@ struct Part {
 // Lane's columns of local range [lo, hi) of a part: local column cg + j*m,
 // j in [ceil((lo-cg)/m), ceil((hi-cg)/m)); zshift maps a local column to its
 // staged slot (slot = local column + zshift).
+// Loads per batch.  8-byte lane elements: 4 (32 bytes in flight per lane).
+// 4-byte BF16 pairs: 6 in a one-part block (phase 1); in the two-part phase-2
+// block of BF16 arenas only 4 fit 64 VGPRs without spilling.
+template <int DT, bool TWO_PARTS>
+constexpr int batch_of() {
+  return (Lane<DT>::bytes == 8 || TWO_PARTS) ? 4 : 6;
+}
+
 template <typename Acc, int DT, int U>
 __device__ inline void run_cols(const Part<DT> &P, int lo, int hi, int cg, int m, int zshift,
                                 int zmax, const Acc *__restrict__ z, Acc &a0, Acc &a1) {
@@ -188,7 +196,6 @@ hmv_stream_kernel(
     const double *__restrict__ t_in, const int *__restrict__ outidx, double *__restrict__ t_out,
     double *__restrict__ y, double beta, const int *__restrict__ skip) {
   if (skip && *skip) return;
-  constexpr int U = 4;
   // inputs of one chunk plus the zero slot; reused for the row reduction
   __shared__ __attribute__((aligned(16))) Acc z[HMV_ZCHUNK + 4];
   static_assert(2 * HMV_WG_THREADS <= HMV_ZCHUNK, "reduction buffer fits in z");
@@ -226,11 +233,13 @@ hmv_stream_kernel(
         lo0 = npf == PF ? lo0 : cw;   // a lane with fewer than PF columns here has none left
       }
       const int hi0 = min(c0 + cw, w.ncol0);
-      if (lo0 < hi0) run_cols<Acc, DT0, U>(P0, lo0, hi0, cg, m, -c0, cw, z, a0, a1);
+      constexpr int U0 = batch_of<DT0, (DT0 != DT1)>();
+      if (lo0 < hi0) run_cols<Acc, DT0, U0>(P0, lo0, hi0, cg, m, -c0, cw, z, a0, a1);
       if (w.ncol1 > 0 && c0 + cw > w.ncol0) {
         const Part<DT1> P1(arena + w.off1, w.rows_p, w.ncol1, rq, cg, m);
-        run_cols<Acc, DT1, U>(P1, max(c0, w.ncol0) - w.ncol0, c0 + cw - w.ncol0, cg, m,
-                              w.ncol0 - c0, cw, z, a0, a1);
+        constexpr int U1 = batch_of<DT1, (DT0 != DT1)>();
+        run_cols<Acc, DT1, U1>(P1, max(c0, w.ncol0) - w.ncol0, c0 + cw - w.ncol0, cg, m,
+                               w.ncol0 - c0, cw, z, a0, a1);
       }
     }
   }

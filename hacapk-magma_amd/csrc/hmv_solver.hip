@@ -486,7 +486,7 @@ int hmv_bicgstab(hmv_arena a, const double *b, double *u, double eps, int maxit,
 }
 
 int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
-                    double inner_eps, int max_outer, int max_inner, double *hist,
+                    double inner_eps, int max_outer, int max_inner, int check_every, double *hist,
                     hmv_solve_info *info) {
   if (!a64 || !alow || !b || !u || a64->nd != alow->nd) {
     hmv::set_error("hmv_bicgstab_ir: invalid arguments");
@@ -514,7 +514,7 @@ int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, d
       return hmv::hip_fail(e, "ir: sync");
     hipLaunchKernelGGL(k_fill, dim3(g), dim3(kThreads), 0, wl->stream, n, wl->x2, 0.0);
     hmv_solve_info in{};
-    rc = run_bicgstab(alow, w64->r2, wl->x2, inner_eps, max_inner, 1, nullptr, &in, true);
+    rc = run_bicgstab(alow, w64->r2, wl->x2, inner_eps, max_inner, check_every, nullptr, &in, true);
     if (rc) return rc;
     inner_total += in.nstp;
     if ((e = hipStreamSynchronize(wl->stream)) != hipSuccess) return hmv::hip_fail(e, "ir: sync");

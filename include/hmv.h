@@ -133,10 +133,11 @@ int hmv_bicgstab(hmv_arena a, const double *b, double *u, double eps, int maxit,
 
 /* Mixed-precision iterative refinement: the FP64 residual r = b - A64 u, an
  * inner BiCGSTAB with the low-precision arena `alow` to inner_eps, u += d;
- * repeats until ||b - A64 u||/||b|| < eps or max_outer steps.
+ * repeats until ||b - A64 u||/||b|| < eps or max_outer steps.  check_every:
+ * residual read-back period of the inner solver.
  * hist (host, may be NULL) receives max_outer+1 outer relative residuals. */
 int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
-                    double inner_eps, int max_outer, int max_inner, double *hist,
+                    double inner_eps, int max_outer, int max_inner, int check_every, double *hist,
                     hmv_solve_info *info);
 
 const char *hmv_last_error(void);
