@@ -83,7 +83,25 @@ class _PlanTables(C.Structure):
                 ("noutidx", C.c_int64), ("codes", C.c_void_p), ("ncodes", C.c_int64),
                 ("tfin", C.c_void_p), ("ntfin", C.c_int64),
                 ("tsize", C.c_int64), ("tpart_size", C.c_int64),
-                ("work_bytes", C.c_int), ("zseg_bytes", C.c_int), ("tfin_bytes", C.c_int)]
+                ("work_bytes", C.c_int), ("zseg_bytes", C.c_int), ("tfin_bytes", C.c_int),
+                ("nwork1_small", C.c_int), ("nt1_small", C.c_int),
+                ("nwork2_small", C.c_int), ("nt2_small", C.c_int)]
+
+
+class Tuning(C.Structure):
+    """hmv_tuning: repack tunables (0 = the precision's default)."""
+    _fields_ = [("p1_target_bytes", C.c_int64), ("p1_row_tile", C.c_int), ("p2_max_rows", C.c_int),
+                ("p1_small_threads", C.c_int), ("p2_small_threads", C.c_int)]
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for k, v in (d or {}).items():
+            setattr(t, k, int(v))
+        return t
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
 
 
 WORK_DTYPE = np.dtype([("off0", "<i8"), ("off1", "<i8"), ("rows", "<i4"), ("rows_p", "<i4"),
@@ -162,8 +180,12 @@ def hmv_lib():
         vp = C.c_void_p
         L.hmv_init.argtypes = [C.c_int]
         L.hmv_arena_build.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.POINTER(vp)]
-        L.hmv_arena_build_tuned.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int64,
-                                            C.c_int, C.c_int, C.POINTER(vp)]
+        L.hmv_arena_build_tuned.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int,
+                                            C.POINTER(Tuning), C.POINTER(vp)]
+        L.hmv_default_tuning.argtypes = [C.c_int, C.POINTER(Tuning)]
+        L.hmv_arena_tuning_get.argtypes = [vp, C.POINTER(Tuning)]
+        L.hmv_autotune.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.POINTER(Tuning),
+                                   _D, _D]
         L.hmv_arena_stats_get.argtypes = [vp, C.POINTER(_ArenaStats)]
         L.hmv_arena_free.argtypes = [vp]
         L.hmv_arena_tbytes.argtypes = [vp]
@@ -184,6 +206,8 @@ def hmv_lib():
         L.hmv_stream_test.argtypes = [C.c_int, vp, vp, C.c_size_t, C.c_int, _D]
         L.hmv_plan_create.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.POINTER(vp)]
         L.hmv_plan_view.argtypes = [vp, C.POINTER(_PlanTables)]
+        L.hmv_plan_create_tuned.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.POINTER(Tuning),
+                                            C.POINTER(vp)]
         L.hmv_plan_free.argtypes = [vp]
         L.c_hacapk_adot_body_lfcpy_batch_sorted_.argtypes = [_I, C.POINTER(RefLeafmtxp)]
         L.c_hacapk_adot_body_lfcpy_batch_sorted_.restype = None
@@ -429,11 +453,9 @@ class Arena:
         self.precision = precision
         a = C.c_void_p()
         if tuning:
-            _check(L.hmv_arena_build_tuned(nd, nlf, leaves_ptr, stride, PREC[precision],
-                                           int(tuning.get("p1_target_bytes", 0)),
-                                           int(tuning.get("p1_row_tile", 0)),
-                                           int(tuning.get("p2_max_rows", 0)), C.byref(a)),
-                   "hmv_arena_build_tuned")
+            t = Tuning.from_dict(tuning)
+            _check(L.hmv_arena_build_tuned(nd, nlf, leaves_ptr, stride, PREC[precision], C.byref(t),
+                                           C.byref(a)), "hmv_arena_build_tuned")
         else:
             _check(L.hmv_arena_build(nd, nlf, leaves_ptr, stride, PREC[precision], C.byref(a)),
                    "hmv_arena_build")
@@ -445,6 +467,13 @@ class Arena:
     @property
     def handle(self):
         return self._a
+
+    @property
+    def tuning(self):
+        """The tuning this arena was built with (defaults resolved)."""
+        t = Tuning()
+        _check(hmv_lib().hmv_arena_tuning_get(self._a, C.byref(t)), "hmv_arena_tuning_get")
+        return t.as_dict()
 
     def free(self):
         if getattr(self, "_a", None):
@@ -491,12 +520,14 @@ class Arena:
         return u, info.as_dict(), hist
 
 
-def plan_tables(hm, precision="f64"):
-    """Host-side repack of an HMatrix (hmv_plan_create): numpy copies of the
-    arena image and the engine's segment/work tables.  Needs no device."""
+def plan_tables(hm, precision="f64", tuning=None):
+    """Host-side repack of an HMatrix (hmv_plan_create_tuned): numpy copies of
+    the arena image and the engine's segment/work tables.  Needs no device."""
     L = hmv_lib()
     p = C.c_void_p()
-    _check(L.hmv_plan_create(hm.nd, hm.nlf, hm.leaves_ptr, hm.stride, PREC[precision], C.byref(p)),
+    t = Tuning.from_dict(tuning) if tuning else None
+    _check(L.hmv_plan_create_tuned(hm.nd, hm.nlf, hm.leaves_ptr, hm.stride, PREC[precision],
+                                   C.byref(t) if t is not None else None, C.byref(p)),
            "hmv_plan_create")
     try:
         t = _PlanTables()
@@ -515,9 +546,27 @@ def plan_tables(hm, precision="f64"):
             "zseg": arr(t.zseg, t.nzseg, ZSEG_DTYPE), "outidx": arr(t.outidx, t.noutidx, np.int32),
             "codes": arr(t.codes, t.ncodes, np.int32), "tfin": arr(t.tfin, t.ntfin, TFIN_DTYPE),
             "tsize": t.tsize, "tpart_size": t.tpart_size,
+            "nwork1_small": t.nwork1_small, "nt1_small": t.nt1_small,
+            "nwork2_small": t.nwork2_small, "nt2_small": t.nt2_small,
         }
     finally:
         L.hmv_plan_free(p)
+
+
+def default_tuning(precision="f64"):
+    """The repack tunables hmv_arena_build uses for a precision."""
+    t = Tuning()
+    _check(hmv_lib().hmv_default_tuning(PREC[precision], C.byref(t)), "hmv_default_tuning")
+    return t.as_dict()
+
+
+def autotune(hm, precision="f64", reps=20):
+    """hmv_autotune: returns (best tuning dict, ms per apply, ms per apply with defaults)."""
+    t = Tuning()
+    best, dflt = C.c_double(), C.c_double()
+    _check(hmv_lib().hmv_autotune(hm.nd, hm.nlf, hm.leaves_ptr, hm.stride, PREC[precision], int(reps),
+                                  C.byref(t), C.byref(best), C.byref(dflt)), "hmv_autotune")
+    return t.as_dict(), best.value, dflt.value
 
 
 def bicgstab_ir(a64, alow, b, u0=None, eps=1e-8, inner_eps=1e-4, max_outer=30, max_inner=1000,

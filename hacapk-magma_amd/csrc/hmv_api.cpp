@@ -27,9 +27,7 @@ int hip_fail(hipError_t e, const char *what) {
 
 hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta, const int *skip,
                          hipStream_t s) {
-  return hmv_launch_apply(a->mode, a->d_arena, a->d_work1, a->nwork1, a->d_tfin, a->ntfin,
-                          a->d_work2, a->nwork2, a->d_zseg, a->d_outidx, a->d_codes, a->d_t, x, y,
-                          beta, skip, s);
+  return hmv_launch_apply(&a->launch, HMV_PHASE1 | HMV_PHASE2, x, y, beta, skip, s);
 }
 
 }  // namespace hmv
@@ -72,6 +70,28 @@ void release(hmv_arena_s *a) {
   (void)hipFree(a->d_hx);
   (void)hipFree(a->d_hy);
   delete a;
+}
+
+hmv_tuning to_public(const hmv::PlanTuning &t) {
+  hmv_tuning o{};
+  o.p1_target_bytes = t.p1_target_bytes;
+  o.p1_row_tile = t.p1_row_tile;
+  o.p2_max_rows = t.p2_max_rows;
+  o.p1_small_threads = t.p1_small_threads;
+  o.p2_small_threads = t.p2_small_threads;
+  return o;
+}
+
+// Public tuning (0 = the precision's default) -> plan tuning.
+hmv::PlanTuning from_public(int prec, const hmv_tuning *t) {
+  hmv::PlanTuning p = hmv::default_tuning(prec);
+  if (!t) return p;
+  if (t->p1_target_bytes != 0) p.p1_target_bytes = t->p1_target_bytes;
+  if (t->p1_row_tile != 0) p.p1_row_tile = t->p1_row_tile;
+  if (t->p2_max_rows != 0) p.p2_max_rows = t->p2_max_rows;
+  if (t->p1_small_threads != 0) p.p1_small_threads = t->p1_small_threads;
+  if (t->p2_small_threads != 0) p.p2_small_threads = t->p2_small_threads;
+  return p;
 }
 
 int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::PlanTuning &tune,
@@ -122,6 +142,24 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
     release(a);
     return hip_fail(e, "hmv_arena_build: sync");
   }
+  HmvLaunch &L = a->launch;
+  L.mode = a->mode;
+  L.arena = a->d_arena;
+  L.work1 = a->d_work1;
+  L.nwork1 = a->nwork1;
+  L.nwork1_small = P.nwork1_small;
+  L.nt1_small = P.nt1_small;
+  L.tfin = a->d_tfin;
+  L.ntfin = a->ntfin;
+  L.work2 = a->d_work2;
+  L.nwork2 = a->nwork2;
+  L.nwork2_small = P.nwork2_small;
+  L.nt2_small = P.nt2_small;
+  L.zseg = a->d_zseg;
+  L.outidx = a->d_outidx;
+  L.codes = a->d_codes;
+  L.t = a->d_t;
+  a->tuning = to_public(tune);
   auto t1 = std::chrono::steady_clock::now();
   hmv_arena_stats &s = a->stats;
   s.nd = nd;
@@ -165,17 +203,30 @@ int hmv_init(int device) {
 
 int hmv_arena_build(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
                     hmv_arena *out) {
-  return build(nd, nlf, leaves, leaf_stride, precision, hmv::PlanTuning(), out);
+  return build(nd, nlf, leaves, leaf_stride, precision, from_public(precision, nullptr), out);
+}
+
+int hmv_default_tuning(int precision, hmv_tuning *t) {
+  if (!t) {
+    set_error("hmv_default_tuning: null argument");
+    return HMV_ERR_ARG;
+  }
+  *t = to_public(hmv::default_tuning(precision));
+  return HMV_OK;
 }
 
 int hmv_arena_build_tuned(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
-                          int64_t p1_target_bytes, int p1_row_tile, int p2_max_rows,
-                          hmv_arena *out) {
-  hmv::PlanTuning t;
-  if (p1_target_bytes > 0) t.p1_target_bytes = p1_target_bytes;
-  if (p1_row_tile > 0) t.p1_row_tile = p1_row_tile;
-  if (p2_max_rows > 0) t.p2_max_rows = p2_max_rows;
-  return build(nd, nlf, leaves, leaf_stride, precision, t, out);
+                          const hmv_tuning *tuning, hmv_arena *out) {
+  return build(nd, nlf, leaves, leaf_stride, precision, from_public(precision, tuning), out);
+}
+
+int hmv_arena_tuning_get(hmv_arena a, hmv_tuning *t) {
+  if (!a || !t) {
+    set_error("hmv_arena_tuning_get: null argument");
+    return HMV_ERR_ARG;
+  }
+  *t = a->tuning;
+  return HMV_OK;
 }
 
 int hmv_arena_stats_get(hmv_arena a, hmv_arena_stats *s) {
@@ -304,15 +355,9 @@ int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *m
       hipEvent_t e0 = ev[3 * r], e1 = ev[3 * r + 1], e2 = ev[3 * r + 2];
       hipEventRecord(e0, s);
       // phase 1 = t (rank-vector products and the chunk-partial sums)
-      if (a->nwork1 > 0)
-        e = hmv_launch_apply(a->mode, a->d_arena, a->d_work1, a->nwork1, a->d_tfin, a->ntfin,
-                             a->d_work2, 0, a->d_zseg, a->d_outidx, a->d_codes, a->d_t, x, y, 0.0,
-                             nullptr, s);
+      if (a->nwork1 > 0) e = hmv_launch_apply(&a->launch, HMV_PHASE1, x, y, 0.0, nullptr, s);
       hipEventRecord(e1, s);
-      if (e == hipSuccess)
-        e = hmv_launch_apply(a->mode, a->d_arena, a->d_work1, 0, a->d_tfin, 0, a->d_work2,
-                             a->nwork2, a->d_zseg, a->d_outidx, a->d_codes, a->d_t, x, y, 0.0,
-                             nullptr, s);
+      if (e == hipSuccess) e = hmv_launch_apply(&a->launch, HMV_PHASE2, x, y, 0.0, nullptr, s);
       hipEventRecord(e2, s);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -332,6 +377,86 @@ int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *m
   if (ms_phase1) *ms_phase1 = a->nwork1 > 0 ? t1 / reps : 0.0;
   if (ms_phase2) *ms_phase2 = t2 / reps;
   if (ms_apply) *ms_apply = ta / reps;
+  return HMV_OK;
+}
+
+int hmv_autotune(int nd, int nlf, const void *leaves, int leaf_stride, int precision, int reps,
+                 hmv_tuning *best, double *best_ms, double *default_ms) {
+  if (!leaves || !best || nd <= 0 || nlf <= 0) {
+    set_error("hmv_autotune: invalid argument");
+    return HMV_ERR_ARG;
+  }
+  int rc = check_device();
+  if (rc) return rc;
+  if (reps < 3) reps = 3;
+  const size_t nb = sizeof(double) * (size_t)nd;
+  double *dx = nullptr, *dy = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e;
+  if ((e = hipMalloc(&dx, nb)) != hipSuccess || (e = hipMalloc(&dy, nb)) != hipSuccess ||
+      (e = hipMemset(dx, 0, nb)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+    rc = hip_fail(e, "hmv_autotune: setup");
+  }
+  // time per apply of an arena built with t (the arena lives only for the measurement)
+  auto measure = [&](const hmv::PlanTuning &t, double *ms) -> int {
+    hmv_arena a = nullptr;
+    int r = build(nd, nlf, leaves, leaf_stride, precision, t, &a);
+    if (r) return r;
+    hipError_t he = hipSuccess;
+    for (int i = 0; i < 2 && he == hipSuccess; ++i) he = hmv::enqueue_apply(a, dx, dy, 0.0, nullptr, s);
+    if (he == hipSuccess) he = hipEventRecord(e0, s);
+    for (int i = 0; i < reps && he == hipSuccess; ++i)
+      he = hmv::enqueue_apply(a, dx, dy, 0.0, nullptr, s);
+    if (he == hipSuccess) he = hipEventRecord(e1, s);
+    if (he == hipSuccess) he = hipEventSynchronize(e1);
+    float m = 0.f;
+    if (he == hipSuccess) he = hipEventElapsedTime(&m, e0, e1);
+    release(a);
+    if (he != hipSuccess) return hip_fail(he, "hmv_autotune: timing");
+    *ms = (double)m / reps;
+    return HMV_OK;
+  };
+  auto same = [](const hmv::PlanTuning &a, const hmv::PlanTuning &b) {
+    return a.p1_target_bytes == b.p1_target_bytes && a.p1_row_tile == b.p1_row_tile &&
+           a.p2_max_rows == b.p2_max_rows && a.p1_small_threads == b.p1_small_threads &&
+           a.p2_small_threads == b.p2_small_threads;
+  };
+  hmv::PlanTuning cur = hmv::default_tuning(precision);
+  double cur_ms = 0.0;
+  if (!rc) rc = measure(cur, &cur_ms);
+  if (!rc && default_ms) *default_ms = cur_ms;
+  // coordinate search, one tunable at a time; a candidate must beat the
+  // current best by more than 1% (timing noise) to replace it
+  auto search = [&](auto set, std::initializer_list<int64_t> values) {
+    for (int64_t v : values) {
+      if (rc) return;
+      hmv::PlanTuning t = cur;
+      set(t, v);
+      if (same(t, cur)) continue;
+      double ms = 0.0;
+      rc = measure(t, &ms);
+      if (!rc && ms < 0.99 * cur_ms) {
+        cur = t;
+        cur_ms = ms;
+      }
+    }
+  };
+  search([](hmv::PlanTuning &t, int64_t v) { t.p1_small_threads = (int)v; }, {-1, 64, 128});
+  search([](hmv::PlanTuning &t, int64_t v) { t.p2_small_threads = (int)v; }, {-1, 64, 128});
+  search([](hmv::PlanTuning &t, int64_t v) { t.p1_target_bytes = v; }, {48 << 10, 96 << 10, 192 << 10});
+  search([](hmv::PlanTuning &t, int64_t v) { t.p2_max_rows = (int)v; }, {128, 256});
+  search([](hmv::PlanTuning &t, int64_t v) { t.p1_row_tile = (int)v; }, {128, 256});
+  if (e1) hipEventDestroy(e1);
+  if (e0) hipEventDestroy(e0);
+  if (s) hipStreamDestroy(s);
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  if (rc) return rc;
+  *best = to_public(cur);
+  if (best_ms) *best_ms = cur_ms;
   return HMV_OK;
 }
 
@@ -365,8 +490,8 @@ struct hmv_plan_s {
   hmv::Plan P;
 };
 
-int hmv_plan_create(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
-                    hmv_plan *out) {
+int hmv_plan_create_tuned(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
+                          const hmv_tuning *tuning, hmv_plan *out) {
   if (!out) {
     set_error("hmv_plan_create: null output");
     return HMV_ERR_ARG;
@@ -374,13 +499,19 @@ int hmv_plan_create(int nd, int nlf, const void *leaves, int leaf_stride, int pr
   *out = nullptr;
   hmv_plan p = new hmv_plan_s();
   std::string err;
-  if (!hmv::build_plan(nd, nlf, leaves, leaf_stride, precision, hmv::PlanTuning(), p->P, err)) {
+  if (!hmv::build_plan(nd, nlf, leaves, leaf_stride, precision, from_public(precision, tuning),
+                       p->P, err)) {
     delete p;
     set_error("hmv_plan_create: " + err);
     return HMV_ERR_ARG;
   }
   *out = p;
   return HMV_OK;
+}
+
+int hmv_plan_create(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
+                    hmv_plan *out) {
+  return hmv_plan_create_tuned(nd, nlf, leaves, leaf_stride, precision, nullptr, out);
 }
 
 int hmv_plan_view(hmv_plan p, hmv_plan_tables *t) {
@@ -408,6 +539,10 @@ int hmv_plan_view(hmv_plan p, hmv_plan_tables *t) {
   t->work_bytes = (int)sizeof(HmvWork);
   t->zseg_bytes = (int)sizeof(HmvZSeg);
   t->tfin_bytes = (int)sizeof(HmvTFin);
+  t->nwork1_small = P.nwork1_small;
+  t->nt1_small = P.nt1_small;
+  t->nwork2_small = P.nwork2_small;
+  t->nt2_small = P.nt2_small;
   return HMV_OK;
 }
 

@@ -27,6 +27,8 @@ struct hmv_arena_s {
   HmvZSeg *d_zseg = nullptr;
   int *d_outidx = nullptr, *d_codes = nullptr;
   double *d_t = nullptr;   // t slots + chunk partials (FP64 in every mode)
+  HmvLaunch launch{};      // device tables of one apply
+  hmv_tuning tuning{};     // the tuning the arena was built with
   double *d_hx = nullptr, *d_hy = nullptr;  // scratch for hmv_apply_host
   HmvSolveWork *solve = nullptr;             // solver workspace (lazy)
   hmv_arena_stats stats{};
@@ -44,11 +46,7 @@ void free_solve_work(hmv_arena_s *a);
 
 extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *dst, size_t bytes,
                                              hipStream_t stream);
-extern "C" hipError_t hmv_launch_apply(int mode, const char *arena, const HmvWork *work1,
-                                       int nwork1, const HmvTFin *tfin, int ntfin,
-                                       const HmvWork *work2, int nwork2, const HmvZSeg *zseg,
-                                       const int *outidx, const int *codes, double *t,
-                                       const double *x, double *y, double beta, const int *skip,
-                                       hipStream_t stream);
+extern "C" hipError_t hmv_launch_apply(const HmvLaunch *L, int phases, const double *x, double *y,
+                                       double beta, const int *skip, hipStream_t stream);
 
 #endif

@@ -75,6 +75,26 @@ struct HmvTFin {
   int pad;
 };
 
+// Everything one apply launches over (device pointers).  Each phase's work
+// list holds its short blocks first (nwork_small of them, run with nt_small
+// threads per workgroup), then the rest (256 threads).
+struct HmvLaunch {
+  int mode;  // 0 FP64, 1 FP32, 2 BF16 low rank + FP32 dense
+  const char *arena;
+  const struct HmvWork *work1;
+  int nwork1, nwork1_small, nt1_small;
+  const struct HmvTFin *tfin;
+  int ntfin;
+  const struct HmvWork *work2;
+  int nwork2, nwork2_small, nt2_small;
+  const struct HmvZSeg *zseg;
+  const int *outidx;
+  const int *codes;
+  double *t;
+};
+#define HMV_PHASE1 1  // phase 1 and the chunk-partial sums
+#define HMV_PHASE2 2
+
 // A run of consecutive input columns with one source.
 struct HmvZSeg {
   int kind;   // HMV_Z_*

@@ -82,7 +82,7 @@ struct Lane<HMV_DT_BF16> {
 // columns c0+tid, c0+tid+256, ... in batches of K: all K index loads, then
 // all K value loads are issued before any is waited for, so a batch costs
 // two memory round trips.
-template <typename Acc>
+template <typename Acc, int NT>
 __device__ inline void stage(Acc *__restrict__ z, const HmvZSeg *__restrict__ zsegs,
                              const HmvWork &w, int c0, int cw, const double *__restrict__ x,
                              const int *__restrict__ codes, const double *__restrict__ t) {
@@ -92,17 +92,17 @@ __device__ inline void stage(Acc *__restrict__ z, const HmvZSeg *__restrict__ zs
   for (int s = w.zseg_begin; s < w.zseg_end && col < c0 + cw; ++s) {
     const HmvZSeg zs = zsegs[s];
     const int lo = max(col, c0), hi = min(col + zs.ncols, c0 + cw);
-    for (int cb = lo + tid; cb < hi; cb += K * HMV_WG_THREADS) {
+    for (int cb = lo + tid; cb < hi; cb += K * NT) {
       const double *src[K];
       if (zs.kind == HMV_Z_X) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
-          src[k] = x + zs.src + (min(cb + k * HMV_WG_THREADS, hi - 1) - col);
+          src[k] = x + zs.src + (min(cb + k * NT, hi - 1) - col);
       } else {
         int code[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-          const int c = cb + k * HMV_WG_THREADS;
+          const int c = cb + k * NT;
           code[k] = c < hi ? codes[zs.src + (c - col)] : 0;
         }
 #pragma unroll
@@ -113,7 +113,7 @@ __device__ inline void stage(Acc *__restrict__ z, const HmvZSeg *__restrict__ zs
       for (int k = 0; k < K; ++k) val[k] = *src[k];
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const int c = cb + k * HMV_WG_THREADS;
+        const int c = cb + k * NT;
         if (c < hi) z[c - c0] = (Acc)val[k];
       }
     }
@@ -188,8 +188,9 @@ __device__ inline void run_cols(const Part<DT> &P, int lo, int hi, int cg, int m
 // PHASE 1: t[outidx[out_base + r]] = row sums.
 // PHASE 2: y[out_base + r] = row sums + beta * y.
 // DT0/DT1: storage types of the two column parts (fixed per arena precision).
-template <typename Acc, int PHASE, int DT0, int DT1>
-__global__ __launch_bounds__(HMV_WG_THREADS) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+// NT: threads per workgroup; inputs are staged NT*8 columns at a time.
+template <typename Acc, int PHASE, int DT0, int DT1, int NT = HMV_WG_THREADS>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 hmv_stream_kernel(
     const char *__restrict__ arena, const HmvWork *__restrict__ works,
     const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ codes,
@@ -197,13 +198,14 @@ hmv_stream_kernel(
     double *__restrict__ y, double beta, const int *__restrict__ skip) {
   if (skip && *skip) return;
   // inputs of one chunk plus the zero slot; reused for the row reduction
-  __shared__ __attribute__((aligned(16))) Acc z[HMV_ZCHUNK + 4];
-  static_assert(2 * HMV_WG_THREADS <= HMV_ZCHUNK, "reduction buffer fits in z");
+  constexpr int ZCH = HMV_ZCHUNK * NT / HMV_WG_THREADS;
+  __shared__ __attribute__((aligned(16))) Acc z[ZCH + 4];
+  static_assert(2 * NT <= ZCH, "reduction buffer fits in z");
   const HmvWork w = works[blockIdx.x];
   const int tid = threadIdx.x;
   constexpr int vr = (DT0 == HMV_DT_F64) ? 1 : 2;
   const int q = w.rows_p / vr;
-  const int m = HMV_WG_THREADS / q;
+  const int m = NT / q;
   const bool active = tid < q * m;
   const int rq = tid % q, cg = tid / q;
   Acc a0 = Acc(0), a1 = Acc(0);
@@ -213,15 +215,15 @@ hmv_stream_kernel(
   // inputs: their loads are issued before staging, so the first HBM round
   // trip overlaps the gathers and the barrier.
   constexpr int PF = 4;
-  const int lim0 = min(w.ncol0, HMV_ZCHUNK);
+  const int lim0 = min(w.ncol0, ZCH);
   const int npf = active ? max(0, min(PF, (lim0 - cg + m - 1) / m)) : 0;
   typename Lane<DT0>::raw pre[PF];
 #pragma unroll
   for (int k = 0; k < PF; ++k) pre[k] = Lane<DT0>::load(P0.rs, k < npf ? P0.voff : kOffNone, k * P0.step);
-  for (int c0 = 0; c0 < ncols; c0 += HMV_ZCHUNK) {
-    const int cw = min(HMV_ZCHUNK, ncols - c0);
+  for (int c0 = 0; c0 < ncols; c0 += ZCH) {
+    const int cw = min(ZCH, ncols - c0);
     if (c0 > 0) __syncthreads();
-    stage<Acc>(z, zsegs, w, c0, cw, x, codes, t_in);
+    stage<Acc, NT>(z, zsegs, w, c0, cw, x, codes, t_in);
     if (tid == 0) z[cw] = Acc(0);
     __syncthreads();
     if (active) {
@@ -250,7 +252,7 @@ hmv_stream_kernel(
     if constexpr (vr == 2) red[cg * w.rows_p + rq * vr + 1] = a1;
   }
   __syncthreads();
-  for (int r = tid; r < w.rows; r += HMV_WG_THREADS) {
+  for (int r = tid; r < w.rows; r += NT) {
     Acc s = red[r];
     for (int g = 1; g < m; ++g) s += red[g * w.rows_p + r];
     if constexpr (PHASE == 1) {
@@ -282,24 +284,41 @@ __global__ __launch_bounds__(HMV_WG_THREADS) void hmv_tfinal_kernel(
   t[f.slot] = s;
 }
 
+template <typename Acc, int PHASE, int DT0, int DT1>
+void launch_list(const HmvLaunch &L, const HmvWork *work, int n, int nsmall, int nt_small,
+                 const double *x, double *y, double beta, const int *skip, hipStream_t stream) {
+  const double *t_in = PHASE == 2 ? L.t : nullptr;
+  double *t_out = PHASE == 1 ? L.t : nullptr;
+  if (nsmall > 0) {
+    if (nt_small == 64)
+      hipLaunchKernelGGL((hmv_stream_kernel<Acc, PHASE, DT0, DT1, 64>), dim3(nsmall), dim3(64), 0,
+                         stream, L.arena, work, L.zseg, x, L.codes, t_in, L.outidx, t_out, y, beta,
+                         skip);
+    else
+      hipLaunchKernelGGL((hmv_stream_kernel<Acc, PHASE, DT0, DT1, 128>), dim3(nsmall), dim3(128), 0,
+                         stream, L.arena, work, L.zseg, x, L.codes, t_in, L.outidx, t_out, y, beta,
+                         skip);
+  }
+  if (n > nsmall)
+    hipLaunchKernelGGL((hmv_stream_kernel<Acc, PHASE, DT0, DT1>), dim3(n - nsmall),
+                       dim3(HMV_WG_THREADS), 0, stream, L.arena, work + nsmall, L.zseg, x, L.codes,
+                       t_in, L.outidx, t_out, y, beta, skip);
+}
+
 template <typename Acc, int DT0, int DT1>
-void launch_mode(const char *arena, const HmvWork *work1, int nwork1, const HmvTFin *tfin,
-                 int ntfin, const HmvWork *work2, int nwork2, const HmvZSeg *zseg,
-                 const int *outidx, const int *codes, double *t, const double *x, double *y,
-                 double beta, const int *skip, hipStream_t stream) {
-  const dim3 block(HMV_WG_THREADS);
-  // phase 1 reads only low-rank factors: both parts use DT1's type
-  if (nwork1 > 0)
-    hipLaunchKernelGGL((hmv_stream_kernel<Acc, 1, DT1, DT1>), dim3(nwork1), block, 0, stream, arena,
-                       work1, zseg, x, codes, (const double *)nullptr, outidx, t,
-                       (double *)nullptr, 0.0, skip);
-  if (ntfin > 0)
-    hipLaunchKernelGGL(hmv_tfinal_kernel, dim3((ntfin + HMV_WG_THREADS - 1) / HMV_WG_THREADS), block,
-                       0, stream, tfin, ntfin, t, skip);
-  if (nwork2 > 0)
-    hipLaunchKernelGGL((hmv_stream_kernel<Acc, 2, DT0, DT1>), dim3(nwork2), block, 0, stream, arena,
-                       work2, zseg, x, codes, (const double *)t, outidx, (double *)nullptr, y,
-                       beta, skip);
+void launch_mode(const HmvLaunch &L, int phases, const double *x, double *y, double beta,
+                 const int *skip, hipStream_t stream) {
+  if (phases & HMV_PHASE1) {
+    // phase 1 reads only low-rank factors: both parts use DT1's type
+    launch_list<Acc, 1, DT1, DT1>(L, L.work1, L.nwork1, L.nwork1_small, L.nt1_small, x, nullptr,
+                                  0.0, skip, stream);
+    if (L.ntfin > 0)
+      hipLaunchKernelGGL(hmv_tfinal_kernel, dim3((L.ntfin + HMV_WG_THREADS - 1) / HMV_WG_THREADS),
+                         dim3(HMV_WG_THREADS), 0, stream, L.tfin, L.ntfin, L.t, skip);
+  }
+  if (phases & HMV_PHASE2)
+    launch_list<Acc, 2, DT0, DT1>(L, L.work2, L.nwork2, L.nwork2_small, L.nt2_small, x, y, beta,
+                                  skip, stream);
 }
 
 // Plain streaming kernels of known traffic, for bandwidth calibration.
@@ -336,23 +355,15 @@ extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *ds
   return hipGetLastError();
 }
 
-// Launches phase 1 (nwork1 workgroups; then the ntfin chunk-partial sums)
-// and/or phase 2 (nwork2) of y = H x + beta*y.
-// mode: 0 = FP64 arena, 1 = FP32 arena, 2 = BF16 low-rank + FP32 dense arena.
-extern "C" hipError_t hmv_launch_apply(int mode, const char *arena, const HmvWork *work1,
-                                       int nwork1, const HmvTFin *tfin, int ntfin,
-                                       const HmvWork *work2, int nwork2, const HmvZSeg *zseg,
-                                       const int *outidx, const int *codes, double *t,
-                                       const double *x, double *y, double beta, const int *skip,
-                                       hipStream_t stream) {
-  if (mode == 0)
-    launch_mode<double, HMV_DT_F64, HMV_DT_F64>(arena, work1, nwork1, tfin, ntfin, work2, nwork2,
-                                                zseg, outidx, codes, t, x, y, beta, skip, stream);
-  else if (mode == 1)
-    launch_mode<float, HMV_DT_F32, HMV_DT_F32>(arena, work1, nwork1, tfin, ntfin, work2, nwork2,
-                                               zseg, outidx, codes, t, x, y, beta, skip, stream);
+// Launches the phases selected by `phases` (HMV_PHASE1: t and the
+// chunk-partial sums; HMV_PHASE2: y = [A|U][x;t] + beta*y) of one apply.
+extern "C" hipError_t hmv_launch_apply(const HmvLaunch *L, int phases, const double *x, double *y,
+                                       double beta, const int *skip, hipStream_t stream) {
+  if (L->mode == 0)
+    launch_mode<double, HMV_DT_F64, HMV_DT_F64>(*L, phases, x, y, beta, skip, stream);
+  else if (L->mode == 1)
+    launch_mode<float, HMV_DT_F32, HMV_DT_F32>(*L, phases, x, y, beta, skip, stream);
   else
-    launch_mode<float, HMV_DT_F32, HMV_DT_BF16>(arena, work1, nwork1, tfin, ntfin, work2, nwork2,
-                                                zseg, outidx, codes, t, x, y, beta, skip, stream);
+    launch_mode<float, HMV_DT_F32, HMV_DT_BF16>(*L, phases, x, y, beta, skip, stream);
   return hipGetLastError();
 }

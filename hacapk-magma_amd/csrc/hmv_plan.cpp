@@ -75,9 +75,24 @@ const double *u_factor(const RefLeaf *l) {
 
 }  // namespace
 
+PlanTuning default_tuning(int prec) {
+  PlanTuning t;
+  if (prec == kF32) t.p1_small_threads = 128;
+  if (prec == kBF16) {
+    t.p1_small_threads = 128;
+    t.p2_small_threads = 64;
+  }
+  return t;
+}
+
 bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
                 const PlanTuning &tune, Plan &P, std::string &err) {
   P = Plan();
+  auto nt_ok = [](int v) { return v == -1 || v == 64 || v == 128; };
+  if (!nt_ok(tune.p1_small_threads) || !nt_ok(tune.p2_small_threads)) {
+    err = "invalid tuning: small-workgroup threads must be -1, 64 or 128";
+    return false;
+  }
   const bool timing = std::getenv("HMV_PLAN_TIMING") != nullptr;
   auto tp = std::chrono::steady_clock::now();
   auto lap = [&](const char *what) {
@@ -544,8 +559,30 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
       for (int k = 0; k < G; ++k)
         if (pos < chunk[k].size()) out.push_back(chunk[k][pos].w);
   };
-  order(it1, P.work1);
-  order(it2, P.work2);
+  // Short blocks (at most nt lanes per column) first, in their own dispatch
+  // order, when the phase runs them in nt-thread workgroups.
+  auto order_split = [&](std::vector<Item> &items, int nt, std::vector<HmvWork> &out, int &nsmall,
+                         int &nt_small) {
+    nsmall = 0;
+    nt_small = 0;
+    if (nt == 64 || nt == 128) {
+      std::vector<Item> small, large;
+      for (auto &it : items) (it.w.rows_p / vr <= nt ? small : large).push_back(it);
+      if (!small.empty()) {
+        std::vector<HmvWork> a, b;
+        order(small, a);
+        order(large, b);
+        nsmall = (int)a.size();
+        nt_small = nt;
+        out = std::move(a);
+        out.insert(out.end(), b.begin(), b.end());
+        return;
+      }
+    }
+    order(items, out);
+  };
+  order_split(it1, tune.p1_small_threads, P.work1, P.nwork1_small, P.nt1_small);
+  order_split(it2, tune.p2_small_threads, P.work2, P.nwork2_small, P.nt2_small);
   lap("order");
   return true;
 }

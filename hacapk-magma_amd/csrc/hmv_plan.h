@@ -51,6 +51,8 @@ struct Plan {
   std::unique_ptr<char[]> arena;   // device image of the arena (arena_size bytes)
   size_t arena_size = 0;
   std::vector<HmvWork> work1, work2;
+  int nwork1_small = 0, nwork2_small = 0;  // short blocks at the front of each list
+  int nt1_small = 0, nt2_small = 0;        // their workgroup threads (0: none)
   std::vector<HmvZSeg> zseg;       // input sources of both phases
   std::vector<int> outidx;         // phase-1 output row -> t partial slot
   std::vector<int> codes;          // phase-2 column input codes (HMV_Z_CODES)
@@ -63,8 +65,13 @@ struct PlanTuning {
   int64_t p1_target_bytes = 96 * 1024;  // arena bytes per phase-1 workgroup
   int p1_row_tile = 256;                // max rank vectors per phase-1 workgroup
   int p2_max_rows = HMV_MAX_ROWS;       // max rows per phase-2 workgroup
+  int p1_small_threads = -1;            // 64/128: short phase-1 blocks in smaller workgroups
+  int p2_small_threads = -1;            // same for phase 2; -1: off
   int xcd_groups = 8;                   // dispatcher round-robin width (XCDs)
 };
+// Defaults of a storage precision (measured on MI355X: the narrow types
+// gain from more, smaller workgroups on short blocks; FP64 does not).
+PlanTuning default_tuning(int prec);
 
 // Builds the plan.  `leaves` points at nlf records spaced `stride` bytes.
 // Returns false and fills `err` on invalid input.

@@ -88,9 +88,35 @@ int hmv_arena_build(int nd, int nlf, const void *leaves, int leaf_stride, int pr
  * the default: p1_target_bytes = arena bytes per V^T x workgroup (96 KiB),
  * p1_row_tile = max rank vectors per V^T x workgroup (256), p2_max_rows =
  * max rows per [A|U] workgroup (256). */
+/* Repack tunables; 0 in a field = the precision's default.
+ *   p1_target_bytes   arena bytes per phase-1 workgroup (rank-vector rows x columns)
+ *   p1_row_tile       most rank vectors per phase-1 workgroup
+ *   p2_max_rows       most rows per phase-2 workgroup
+ *   p1/p2_small_threads  64 or 128: blocks needing at most that many lanes per
+ *                     column run in workgroups of that many threads (more
+ *                     workgroups in flight for short blocks); -1: all blocks
+ *                     run in 256-thread workgroups. */
+typedef struct {
+  int64_t p1_target_bytes;
+  int p1_row_tile;
+  int p2_max_rows;
+  int p1_small_threads;
+  int p2_small_threads;
+} hmv_tuning;
+/* Defaults of a precision (what hmv_arena_build uses). */
+int hmv_default_tuning(int precision, hmv_tuning *t);
 int hmv_arena_build_tuned(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
-                          int64_t p1_target_bytes, int p1_row_tile, int p2_max_rows,
-                          hmv_arena *out);
+                          const hmv_tuning *tuning, hmv_arena *out);
+/* The tuning an arena was built with (defaults resolved). */
+int hmv_arena_tuning_get(hmv_arena a, hmv_tuning *t);
+/* Autotuner: searches the tunables one at a time (small-workgroup threads
+ * of each phase, phase-1 bytes per workgroup, phase-2 rows), building an
+ * arena for each candidate on the current device and timing `reps` applies
+ * (HIP events).  *best receives the fastest tuning, *best_ms its time per
+ * apply, *default_ms the defaults' time.  Needs device memory for one arena
+ * at a time. */
+int hmv_autotune(int nd, int nlf, const void *leaves, int leaf_stride, int precision, int reps,
+                 hmv_tuning *best, double *best_ms, double *default_ms);
 
 int hmv_arena_stats_get(hmv_arena a, hmv_arena_stats *s);
 int hmv_arena_free(hmv_arena a);
@@ -184,9 +210,13 @@ typedef struct {
   const void *tfin;    int64_t ntfin;
   int64_t tsize, tpart_size;
   int work_bytes, zseg_bytes, tfin_bytes;
+  int nwork1_small, nt1_small;  /* short blocks first in work1, run with nt1_small threads */
+  int nwork2_small, nt2_small;
 } hmv_plan_tables;
 int hmv_plan_create(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
                     hmv_plan *out);
+int hmv_plan_create_tuned(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
+                          const hmv_tuning *tuning, hmv_plan *out);
 int hmv_plan_view(hmv_plan p, hmv_plan_tables *t);
 int hmv_plan_free(hmv_plan p);
 

@@ -33,14 +33,16 @@ def _part(arena, off, rows_p, ncols, dt):
     return _decode(arena[off:end], dt).reshape(ncols, rows_p).T
 
 
-def _lane_layout(w):
+def _lane_layout(w, nt=THREADS):
+    """Lane layout of a workgroup of nt threads (short blocks may run with 64
+    or 128); asserts the block fits it."""
     vr = 1 if int(w["dt0"]) == 0 else 2
     rows, rows_p = int(w["rows"]), int(w["rows_p"])
     assert 0 < rows <= rows_p <= MAX_ROWS * vr and rows_p % vr == 0 and rows_p - rows < vr
     q = rows_p // vr
-    assert 1 <= q <= THREADS
-    m = THREADS // q
-    assert m * rows_p <= 2 * THREADS, "reduction buffer overflow"
+    assert 1 <= q <= nt, "more lanes per column than threads"
+    m = nt // q
+    assert m * rows_p <= 2 * nt, "reduction buffer overflow"
     if int(w["ncol1"]) > 0:
         assert (1 if int(w["dt1"]) == 0 else 2) == vr, "both parts share the row-vector width"
     return vr, q, m
@@ -86,6 +88,16 @@ def _block_product(arena, w, z):
     return y[:int(w["rows"])]
 
 
+def _threads(tables):
+    """Per phase: work index -> workgroup threads (short blocks first)."""
+    out = []
+    for k in (1, 2):
+        ns, nt = int(tables.get("nwork%d_small" % k, 0)), int(tables.get("nt%d_small" % k, 0))
+        assert ns == 0 or nt in (64, 128)
+        out.append(lambda i, ns=ns, nt=nt: nt if i < ns else THREADS)
+    return out
+
+
 def run(tables, x, nd):
     """Returns (t, y) computed as the kernel launches would (phase 1, the
     chunk-partial sums, phase 2), with bounds checks.  t is the tsize final
@@ -97,8 +109,9 @@ def run(tables, x, nd):
     outidx = tables["outidx"]
     tbuf = np.full(max(tpart_size, 1), np.nan)
     written = np.zeros(max(tpart_size, 1), bool)
-    for w in tables["work1"]:
-        _lane_layout(w)
+    nts = _threads(tables)
+    for i, w in enumerate(tables["work1"]):
+        _lane_layout(w, nts[0](i))
         assert int(w["ncol1"]) == 0
         z = _stage(tables, w, x, tbuf, tsize, nd)
         out = _block_product(arena, w, z)
@@ -123,8 +136,8 @@ def run(tables, x, nd):
         assert written[:tsize].all(), "some t slots are never produced"
     y = np.full(nd, np.nan)
     wy = np.zeros(nd, bool)
-    for w in tables["work2"]:
-        _lane_layout(w)
+    for i, w in enumerate(tables["work2"]):
+        _lane_layout(w, nts[1](i))
         z = _stage(tables, w, x, tbuf, tsize, nd)
         out = _block_product(arena, w, z) if z.size else np.zeros(int(w["rows"]))
         ob, rows = int(w["out_base"]), int(w["rows"])

@@ -71,6 +71,41 @@ def test_low_precision_apply(gpu, O, sphere4, prec, tol):
     a.free()
 
 
+@pytest.mark.parametrize("prec,tuning", [
+    ("f64", {"p1_small_threads": 64, "p2_small_threads": 64}),
+    ("f64", {"p1_small_threads": 128, "p2_small_threads": 128, "p1_target_bytes": 16384,
+             "p1_row_tile": 32, "p2_max_rows": 16}),
+    ("f32", {"p1_small_threads": -1, "p2_small_threads": 128}),
+    ("bf16", {"p1_small_threads": 64, "p2_small_threads": -1}),
+])
+def test_tuned_arenas_match_oracle(gpu, O, two_spheres4, prec, tuning):
+    """Arenas built with other tunings (small workgroups for short blocks,
+    other tile sizes) give the same product to the precision's bar."""
+    hm = two_spheres4
+    a = gpu.Arena(hm, prec, tuning=tuning)
+    got = a.tuning
+    for k, v in tuning.items():
+        assert got[k] == v
+    x = gpu.uniform(8, hm.nd)
+    err = _rel(a.matvec(x), O.adot_body_lfmtx(hm, x))
+    print(prec, tuning, "rel err %.3e" % err, flush=True)
+    assert err <= {"f64": TOL64, "f32": 1e-5, "bf16": 2e-2}[prec]
+    a.free()
+
+
+def test_autotune(gpu, O, sphere4):
+    """The autotuner returns a valid tuning no slower than the defaults (it
+    keeps the defaults unless a candidate is faster), and an arena built
+    with it matches the oracle."""
+    best, best_ms, dflt_ms = gpu.autotune(sphere4, "f64", reps=5)
+    print("autotune", best, "%.4f ms (defaults %.4f ms)" % (best_ms, dflt_ms), flush=True)
+    assert 0 < best_ms <= dflt_ms
+    a = gpu.Arena(sphere4, "f64", tuning=best)
+    x = gpu.uniform(9, sphere4.nd)
+    assert _rel(a.matvec(x), O.adot_body_lfmtx(sphere4, x)) <= TOL64
+    a.free()
+
+
 def test_linearity(gpu, sphere4):
     a = gpu.Arena(sphere4, "f64")
     x1, x2 = gpu.uniform(10, sphere4.nd), gpu.uniform(11, sphere4.nd)

@@ -67,6 +67,36 @@ def test_long_column_groups_are_chunked(H, O):
     assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < 1e-14
 
 
+@pytest.mark.parametrize("prec,tuning", [
+    ("f64", {"p1_small_threads": 64, "p2_small_threads": 128}),
+    ("f64", {"p1_target_bytes": 16384, "p1_row_tile": 32, "p2_max_rows": 16}),
+    ("f32", {"p1_small_threads": 64, "p2_small_threads": 64, "p2_max_rows": 64}),
+    ("bf16", {"p1_small_threads": -1, "p2_small_threads": 128, "p1_target_bytes": 200000}),
+])
+def test_tuned_plans(H, O, two_spheres4, prec, tuning):
+    """Every tuning the autotuner can pick gives a repack whose emulated apply
+    equals the oracle, with short blocks first in each list."""
+    T = H.plan_tables(two_spheres4, prec, tuning)
+    for k in (1, 2):
+        key = "p%d_small_threads" % k
+        if tuning.get(key, 0) in (64, 128):
+            assert T["nt%d_small" % k] == tuning[key] and T["nwork%d_small" % k] > 0
+            w = T["work%d" % k]
+            vr = 1 if prec == "f64" else 2
+            assert np.all(w["rows_p"][:T["nwork%d_small" % k]] // vr <= tuning[key])
+        elif tuning.get(key, 0) == -1:
+            assert T["nwork%d_small" % k] == 0
+    x = H.uniform(4, two_spheres4.nd)
+    _, y = PE.run(T, x, two_spheres4.nd)
+    y_ref = O.adot_body_lfmtx(two_spheres4, x)
+    assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < TOL[prec]
+
+
+def test_default_tuning(H):
+    assert H.default_tuning("f64")["p1_small_threads"] == -1
+    assert H.default_tuning("bf16")["p2_small_threads"] == 64
+
+
 def test_plan_rejects_bad_input(H, sphere3):
     import ctypes as C
     p = C.c_void_p()
@@ -76,3 +106,7 @@ def test_plan_rejects_bad_input(H, sphere3):
     rc = H.hmv_lib().hmv_plan_create(sphere3.nd, sphere3.nlf, sphere3.leaves_ptr, sphere3.stride, 7,
                                      C.byref(p))
     assert rc != 0 and "precision" in H.hmv_lib().hmv_last_error().decode()
+    with pytest.raises(H.HmvError, match="small-workgroup"):
+        H.plan_tables(sphere3, "f64", {"p1_small_threads": 96})
+    with pytest.raises(H.HmvError, match="tuning"):
+        H.plan_tables(sphere3, "f64", {"p2_max_rows": 1000})
