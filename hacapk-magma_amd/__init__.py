@@ -88,6 +88,17 @@ class _PlanTables(C.Structure):
                 ("nwork2_small", C.c_int), ("nt2_small", C.c_int)]
 
 
+class ColumnInfo(C.Structure):
+    """hmv_column_info: one column of a block solve."""
+    _fields_ = [("nstp", C.c_int), ("status", C.c_int), ("relres", C.c_double),
+                ("true_relres", C.c_double)]
+
+    def as_dict(self):
+        d = {f[0]: getattr(self, f[0]) for f in self._fields_}
+        d["status_name"] = SOLVE_STATUS.get(self.status, "?")
+        return d
+
+
 class Tuning(C.Structure):
     """hmv_tuning: repack tunables (0 = the precision's default)."""
     _fields_ = [("p1_target_bytes", C.c_int64), ("p1_row_tile", C.c_int), ("p2_max_rows", C.c_int),
@@ -183,6 +194,10 @@ def hmv_lib():
         L.hmv_arena_build_tuned.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int,
                                             C.POINTER(Tuning), C.POINTER(vp)]
         L.hmv_default_tuning.argtypes = [C.c_int, C.POINTER(Tuning)]
+        L.hmv_apply_block.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp]
+        L.hmv_apply_block_timed.argtypes = [vp, C.c_int, vp, vp, C.c_int, _D]
+        L.hmv_block_bicgstab.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_int, C.c_int,
+                                         C.POINTER(ColumnInfo), C.POINTER(SolveInfo)]
         L.hmv_arena_tuning_get.argtypes = [vp, C.POINTER(Tuning)]
         L.hmv_autotune.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.POINTER(Tuning),
                                    _D, _D]
@@ -496,6 +511,40 @@ class Arena:
         y = np.zeros_like(x)
         _check(hmv_lib().hmv_apply_host(self._a, _dptr(x), _dptr(y)), "hmv_apply_host")
         return y
+
+    def apply_block(self, x_ptr, y_ptr, k, beta=0.0, stream=None):
+        """Y = H X + beta Y for k row-interleaved right-hand sides on device (asynchronous)."""
+        _check(hmv_lib().hmv_apply_block(self._a, int(k), x_ptr, y_ptr, float(beta), stream),
+               "hmv_apply_block")
+
+    def matmat(self, X):
+        """Host convenience: H X for an (N, k) array (cluster order); returns (N, k)."""
+        X = np.ascontiguousarray(X, np.float64)
+        assert X.ndim == 2 and X.shape[0] == self.nd
+        k = X.shape[1]
+        dx, dy = DeviceBuffer.from_array(X.reshape(-1)), DeviceBuffer.from_array(np.zeros(X.size))
+        self.apply_block(dx.ptr, dy.ptr, k)
+        return dy.download().reshape(self.nd, k)
+
+    def apply_block_timed(self, x_ptr, y_ptr, k, reps):
+        ms = C.c_double()
+        _check(hmv_lib().hmv_apply_block_timed(self._a, int(k), x_ptr, y_ptr, int(reps), C.byref(ms)),
+               "hmv_apply_block_timed")
+        return ms.value
+
+    def block_bicgstab(self, B, U0=None, eps=1e-8, maxit=1000, check_every=8):
+        """k independent BiCGSTAB solves sharing one H-matmat per step; B is (N, k)
+        (k a power of two <= 128).  Returns (U, per-column info list, info)."""
+        B = np.ascontiguousarray(B, np.float64)
+        k = B.shape[1]
+        U0 = np.zeros_like(B) if U0 is None else np.ascontiguousarray(U0, np.float64)
+        db, du = DeviceBuffer.from_array(B.reshape(-1)), DeviceBuffer.from_array(U0.reshape(-1))
+        cols = (ColumnInfo * k)()
+        info = SolveInfo()
+        _check(hmv_lib().hmv_block_bicgstab(self._a, k, db.ptr, du.ptr, float(eps), int(maxit),
+                                            int(check_every), cols, C.byref(info)),
+               "hmv_block_bicgstab")
+        return du.download().reshape(self.nd, k), [c.as_dict() for c in cols], info.as_dict()
 
     def apply_timed(self, x_ptr, y_ptr, reps):
         m1, m2, ma = C.c_double(), C.c_double(), C.c_double()

@@ -25,6 +25,24 @@ int hip_fail(hipError_t e, const char *what) {
   return (e == hipErrorOutOfMemory) ? HMV_ERR_MEMORY : HMV_ERR_DEVICE;
 }
 
+int ensure_tk(hmv_arena_s *a, int k) {
+  const int64_t need = std::max<int64_t>(1, a->tpart_size) * k;
+  if (need <= a->tk_cap) return HMV_OK;
+  (void)hipFree(a->d_tk);
+  a->d_tk = nullptr;
+  a->tk_cap = 0;
+  hipError_t e = hipMalloc((void **)&a->d_tk, sizeof(double) * (size_t)need);
+  if (e != hipSuccess) return hip_fail(e, "H-matmat t buffer");
+  a->tk_cap = need;
+  return HMV_OK;
+}
+
+hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
+                               const int *skip, hipStream_t s) {
+  return hmv_launch_block(&a->launch, a->d_tiles1, a->ntiles1, a->d_tiles2, a->ntiles2, k, x,
+                          a->d_tk, y, beta, skip, s);
+}
+
 hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta, const int *skip,
                          hipStream_t s) {
   return hmv_launch_apply(&a->launch, HMV_PHASE1 | HMV_PHASE2, x, y, beta, skip, s);
@@ -63,6 +81,9 @@ void release(hmv_arena_s *a) {
   (void)hipFree(a->d_work1);
   (void)hipFree(a->d_work2);
   (void)hipFree(a->d_tfin);
+  (void)hipFree(a->d_tiles1);
+  (void)hipFree(a->d_tiles2);
+  (void)hipFree(a->d_tk);
   (void)hipFree(a->d_zseg);
   (void)hipFree(a->d_outidx);
   (void)hipFree(a->d_codes);
@@ -132,6 +153,21 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
       (e = upload(&a->d_codes, P.codes)) != hipSuccess) {
     release(a);
     return hip_fail(e, "hmv_arena_build: upload");
+  }
+  // H-matmat items: every work item's 16-row tiles, in the work lists' order
+  std::vector<int2> tiles1, tiles2;
+  auto tiles_of = [](const std::vector<HmvWork> &wl, std::vector<int2> &out) {
+    for (size_t i = 0; i < wl.size(); ++i)
+      for (int m = 0; m * 16 < wl[i].rows; ++m) out.push_back(make_int2((int)i, m));
+  };
+  tiles_of(P.work1, tiles1);
+  tiles_of(P.work2, tiles2);
+  a->ntiles1 = (int)tiles1.size();
+  a->ntiles2 = (int)tiles2.size();
+  if ((e = upload(&a->d_tiles1, tiles1)) != hipSuccess ||
+      (e = upload(&a->d_tiles2, tiles2)) != hipSuccess) {
+    release(a);
+    return hip_fail(e, "hmv_arena_build: tile upload");
   }
   size_t tb = (size_t)std::max<int64_t>(1, P.tpart_size) * sizeof(double);
   if ((e = hipMalloc((void **)&a->d_t, tb)) != hipSuccess) {
@@ -260,6 +296,46 @@ int hmv_apply(hmv_arena a, const double *x, double *y, double beta, void *stream
   }
   hipError_t e = hmv::enqueue_apply(a, x, y, beta, nullptr, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "hmv_apply: launch");
+  return HMV_OK;
+}
+
+int hmv_apply_block(hmv_arena a, int k, const double *x, double *y, double beta, void *stream) {
+  if (!a || !x || !y || k < 1 || k > HMV_MAX_RHS) {
+    set_error("hmv_apply_block: invalid argument (k must be 1..128)");
+    return HMV_ERR_ARG;
+  }
+  int rc = hmv::ensure_tk(a, k);
+  if (rc) return rc;
+  hipError_t e = hmv::enqueue_apply_block(a, k, x, y, beta, nullptr, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "hmv_apply_block");
+  return HMV_OK;
+}
+
+int hmv_apply_block_timed(hmv_arena a, int k, const double *x, double *y, int reps, double *ms) {
+  if (!a || !x || !y || k < 1 || k > HMV_MAX_RHS || reps < 1) {
+    set_error("hmv_apply_block_timed: invalid argument");
+    return HMV_ERR_ARG;
+  }
+  int rc = hmv::ensure_tk(a, k);
+  if (rc) return rc;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hmv::enqueue_apply_block(a, k, x, y, 0.0, nullptr, s);
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (int r = 0; r < reps && e == hipSuccess; ++r)
+    e = hmv::enqueue_apply_block(a, k, x, y, 0.0, nullptr, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float m = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
+  if (e1) hipEventDestroy(e1);
+  if (e0) hipEventDestroy(e0);
+  if (s) hipStreamDestroy(s);
+  if (e != hipSuccess) return hip_fail(e, "hmv_apply_block_timed");
+  if (ms) *ms = (double)m / reps;
   return HMV_OK;
 }
 

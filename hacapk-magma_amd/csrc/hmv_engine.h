@@ -28,6 +28,10 @@ struct hmv_arena_s {
   int *d_outidx = nullptr, *d_codes = nullptr;
   double *d_t = nullptr;   // t slots + chunk partials (FP64 in every mode)
   HmvLaunch launch{};      // device tables of one apply
+  int2 *d_tiles1 = nullptr, *d_tiles2 = nullptr;  // (work, 16-row tile) items of the H-matmat
+  int ntiles1 = 0, ntiles2 = 0;
+  double *d_tk = nullptr;  // t buffer of the H-matmat (tpart_size * k doubles)
+  int64_t tk_cap = 0;      // its capacity in doubles
   hmv_tuning tuning{};     // the tuning the arena was built with
   double *d_hx = nullptr, *d_hy = nullptr;  // scratch for hmv_apply_host
   HmvSolveWork *solve = nullptr;             // solver workspace (lazy)
@@ -42,11 +46,20 @@ int hip_fail(hipError_t e, const char *what);
 hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta, const int *skip,
                          hipStream_t s);
 void free_solve_work(hmv_arena_s *a);
+// Enqueues Y = H X + beta*Y for k interleaved right-hand sides (1..128).
+hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
+                               const int *skip, hipStream_t s);
+// Makes the H-matmat t buffer hold k right-hand sides.
+int ensure_tk(hmv_arena_s *a, int k);
 }  // namespace hmv
 
 extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *dst, size_t bytes,
                                              hipStream_t stream);
 extern "C" hipError_t hmv_launch_apply(const HmvLaunch *L, int phases, const double *x, double *y,
                                        double beta, const int *skip, hipStream_t stream);
+extern "C" hipError_t hmv_launch_block(const HmvLaunch *L, const int2 *tiles1, int ntiles1,
+                                       const int2 *tiles2, int ntiles2, int k, const double *x,
+                                       double *t, double *y, double beta, const int *skip,
+                                       hipStream_t stream);
 
 #endif

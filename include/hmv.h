@@ -125,6 +125,16 @@ int hmv_arena_free(hmv_arena a);
  * ignored).  Asynchronous on `stream`. */
 int hmv_apply(hmv_arena a, const double *x, double *y, double beta, void *stream);
 
+/* H-matmat: Y = H X + beta*Y for k right-hand sides (1 <= k <= HMV_MAX_RHS)
+ * on device arrays stored row-interleaved (N x k row-major: row i's k values
+ * contiguous).  Runs on MFMA (FP64, or FP32 for FP32/BF16 arenas); each
+ * column equals hmv_apply of that column to rounding.  No reference
+ * counterpart (the reference applies one vector per call). */
+#define HMV_MAX_RHS 128
+int hmv_apply_block(hmv_arena a, int k, const double *x, double *y, double beta, void *stream);
+/* Average time of `reps` H-matmats (HIP events on an internal stream). */
+int hmv_apply_block_timed(hmv_arena a, int k, const double *x, double *y, int reps, double *ms);
+
 /* zau += H*zu on host vectors (the accumulate semantics of
  * c_hacapk_adot_body_lfmtx_batch_, HACApK_BATCH.c:257-271); synchronous. */
 int hmv_apply_host(hmv_arena a, const double *zu, double *zau);
@@ -165,6 +175,23 @@ int hmv_bicgstab(hmv_arena a, const double *b, double *u, double eps, int maxit,
 int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
                     double inner_eps, int max_outer, int max_inner, int check_every, double *hist,
                     hmv_solve_info *info);
+
+/* Block BiCGSTAB: k independent solves A u_j = b_j (k a power of two,
+ * 1..128) sharing one H-matmat per step; b, u device arrays N x k
+ * row-interleaved (u holds the initial guesses).  Column j stops at
+ * ||r_j||/||b_j|| < eps and is frozen while the others continue.  cols (k
+ * entries, may be NULL) receives per-column step counts, status and the
+ * recursive and true relative residuals; info the iterations run, the worst
+ * status, the largest true residual and the wall time.  No reference
+ * counterpart (the reference solves one right-hand side per call). */
+typedef struct {
+  int nstp;
+  int status;
+  double relres;
+  double true_relres;
+} hmv_column_info;
+int hmv_block_bicgstab(hmv_arena a, int k, const double *b, double *u, double eps, int maxit,
+                       int check_every, hmv_column_info *cols, hmv_solve_info *info);
 
 const char *hmv_last_error(void);
 

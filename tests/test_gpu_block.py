@@ -1,0 +1,98 @@
+"""Multi-right-hand-side H-matmat (MFMA) and block BiCGSTAB through the C ABI.
+
+Bar: every column of the H-matmat equals the single-vector apply of that
+column (FP64: <= 1e-13 relative, FP32 / BF16 arenas: <= 1e-5 relative, both
+accumulate in FP32 but in different orders) and the CPU oracle (FP64 <= 1e-12);
+block BiCGSTAB columns reach the true residual bar and agree with the
+single-vector solver's solutions.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 16, 32, 40, 128])
+def test_matmat_f64_columns(gpu, O, two_spheres4, k):
+    hm = two_spheres4
+    a = gpu.Arena(hm, "f64")
+    X = np.stack([gpu.uniform(100 + j, hm.nd) for j in range(k)], axis=1)
+    Y = a.matmat(X)
+    worst = 0.0
+    for j in sorted({0, k // 2, k - 1}):
+        y1 = a.matvec(X[:, j])
+        worst = max(worst, _rel(Y[:, j], y1))
+        assert _rel(Y[:, j], O.adot_body_lfmtx(hm, X[:, j])) <= 1e-12
+    print("k", k, "max rel diff vs single apply %.2e" % worst, flush=True)
+    assert worst <= 1e-13
+    a.free()
+
+
+@pytest.mark.parametrize("prec,k", [("f32", 8), ("f32", 32), ("bf16", 16)])
+def test_matmat_low_precision(gpu, O, two_spheres4, prec, k):
+    hm = two_spheres4
+    a = gpu.Arena(hm, prec)
+    X = np.stack([gpu.uniform(200 + j, hm.nd) for j in range(k)], axis=1)
+    Y = a.matmat(X)
+    for j in (0, k - 1):
+        assert _rel(Y[:, j], a.matvec(X[:, j])) <= 1e-5
+        assert _rel(Y[:, j], O.adot_body_lfmtx(hm, X[:, j])) <= {"f32": 1e-5, "bf16": 2e-2}[prec]
+    a.free()
+
+
+def test_matmat_beta_accumulates(gpu, sphere4):
+    a = gpu.Arena(sphere4, "f64")
+    k = 8
+    X = np.stack([gpu.uniform(300 + j, sphere4.nd) for j in range(k)], axis=1)
+    Y0 = np.stack([gpu.uniform(400 + j, sphere4.nd) for j in range(k)], axis=1)
+    dx, dy = gpu.DeviceBuffer.from_array(X.reshape(-1)), gpu.DeviceBuffer.from_array(Y0.reshape(-1))
+    a.apply_block(dx.ptr, dy.ptr, k, beta=1.0)
+    gpu.hmv_lib().hmv_device_synchronize()
+    Y = dy.download().reshape(sphere4.nd, k)
+    assert _rel(Y, Y0 + a.matmat(X)) <= 1e-13
+    a.free()
+
+
+def test_block_bicgstab_matches_single(gpu, sphere4):
+    hm = sphere4
+    a = gpu.Arena(hm, "f64")
+    k = 8
+    B = np.stack([gpu.uniform(500 + j, hm.nd) for j in range(k)], axis=1)
+    B[:, 0] = 1.0
+    U, cols, info = a.block_bicgstab(B, eps=1e-8, maxit=500, check_every=4)
+    print("block", info, [c["nstp"] for c in cols], flush=True)
+    assert info["status_name"] == "converged"
+    for j in range(k):
+        assert cols[j]["status_name"] == "converged" and cols[j]["true_relres"] < 1e-8 * 1.5
+        u1, i1, _ = a.bicgstab(B[:, j], eps=1e-8, maxit=500)
+        assert abs(cols[j]["nstp"] - i1["nstp"]) <= 3
+        assert _rel(U[:, j], u1) < 1e-6
+    a.free()
+
+
+def test_block_bicgstab_stops_columns_independently(gpu, sphere4):
+    """A zero right-hand side stops its column at setup (non-finite relative
+    residual); the other columns still converge."""
+    hm = sphere4
+    a = gpu.Arena(hm, "f64")
+    B = np.stack([gpu.uniform(600 + j, hm.nd) for j in range(4)], axis=1)
+    B[:, 2] = 0.0
+    U, cols, info = a.block_bicgstab(B, eps=1e-8, maxit=500)
+    assert cols[2]["status_name"] == "nonfinite" and cols[2]["nstp"] == 0
+    assert np.all(U[:, 2] == 0.0)
+    for j in (0, 1, 3):
+        assert cols[j]["status_name"] == "converged"
+    a.free()
+
+
+def test_block_rejects_bad_k(gpu, sphere3):
+    a = gpu.Arena(sphere3, "f64")
+    with pytest.raises(gpu.HmvError):
+        a.block_bicgstab(np.ones((sphere3.nd, 3)))
+    with pytest.raises(gpu.HmvError):
+        a.matmat(np.ones((sphere3.nd, 129)))
+    a.free()
