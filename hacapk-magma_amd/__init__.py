@@ -195,6 +195,12 @@ def hmv_lib():
                                             C.POINTER(Tuning), C.POINTER(vp)]
         L.hmv_default_tuning.argtypes = [C.c_int, C.POINTER(Tuning)]
         L.hmv_apply_block.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp]
+        L.hmv_arena_build_multi.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.POINTER(vp), C.c_int, C.c_int, C.POINTER(Tuning),
+                                            C.POINTER(vp)]
+        L.hmv_arena_segments.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
+        L.hmv_multi_bicgstab.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int,
+                                         C.POINTER(ColumnInfo), _D, C.POINTER(SolveInfo)]
         L.hmv_apply_block_timed.argtypes = [vp, C.c_int, vp, vp, C.c_int, _D]
         L.hmv_block_bicgstab.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_int, C.c_int,
                                          C.POINTER(ColumnInfo), C.POINTER(SolveInfo)]
@@ -474,9 +480,12 @@ class Arena:
         else:
             _check(L.hmv_arena_build(nd, nlf, leaves_ptr, stride, PREC[precision], C.byref(a)),
                    "hmv_arena_build")
+        self._finish(a)
+
+    def _finish(self, a):
         self._a = a
         st = _ArenaStats()
-        _check(L.hmv_arena_stats_get(a, C.byref(st)), "arena stats")
+        _check(hmv_lib().hmv_arena_stats_get(a, C.byref(st)), "arena stats")
         self.stats = {f[0]: getattr(st, f[0]) for f in _ArenaStats._fields_ if f[0] != "pad"}
 
     @property
@@ -600,6 +609,46 @@ def plan_tables(hm, precision="f64", tuning=None):
         }
     finally:
         L.hmv_plan_free(p)
+
+
+class MultiArena(Arena):
+    """Many independent H-matrices in one arena (hmv_arena_build_multi): one
+    launch per phase applies all of them; vectors are concatenated."""
+
+    def __init__(self, hms, precision="f64", tuning=None):
+        L = hmv_lib()
+        n = len(hms)
+        strides = {hm.stride for hm in hms}
+        assert len(strides) == 1, "leaf records of one layout"
+        nd = (C.c_int * n)(*[hm.nd for hm in hms])
+        nlf = (C.c_int * n)(*[hm.nlf for hm in hms])
+        lv = (C.c_void_p * n)(*[hm.leaves_ptr for hm in hms])
+        t = Tuning.from_dict(tuning) if tuning else None
+        a = C.c_void_p()
+        _check(L.hmv_arena_build_multi(n, nd, nlf, lv, strides.pop(), PREC[precision],
+                                       C.byref(t) if t is not None else None, C.byref(a)),
+               "hmv_arena_build_multi")
+        self.nd = sum(hm.nd for hm in hms)
+        self.precision = precision
+        self._finish(a)
+        off = (C.c_int * (n + 1))()
+        assert L.hmv_arena_segments(a, off, n + 1) == n
+        self.offsets = np.array(off[:], dtype=np.int64)
+
+    def bicgstab_multi(self, b, u0=None, eps=1e-8, maxit=1000, check_every=8):
+        """One BiCGSTAB per instance, all in lockstep.  Returns (u, per-instance
+        info list, per-instance seconds, info)."""
+        b = np.ascontiguousarray(b, np.float64)
+        u0 = np.zeros_like(b) if u0 is None else np.ascontiguousarray(u0, np.float64)
+        n = len(self.offsets) - 1
+        db, du = DeviceBuffer.from_array(b), DeviceBuffer.from_array(u0)
+        inst = (ColumnInfo * n)()
+        secs = np.zeros(n)
+        info = SolveInfo()
+        _check(hmv_lib().hmv_multi_bicgstab(self._a, db.ptr, du.ptr, float(eps), int(maxit),
+                                            int(check_every), inst, _dptr(secs), C.byref(info)),
+               "hmv_multi_bicgstab")
+        return du.download(), [c.as_dict() for c in inst], secs, info.as_dict()
 
 
 def default_tuning(precision="f64"):

@@ -2,6 +2,7 @@
 // the reference-named entry points a Fortran driver links against.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -44,8 +45,8 @@ hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y
 }
 
 hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta, const int *skip,
-                         hipStream_t s) {
-  return hmv_launch_apply(&a->launch, HMV_PHASE1 | HMV_PHASE2, x, y, beta, skip, s);
+                         hipStream_t s, const int *inst_done) {
+  return hmv_launch_apply(&a->launch, HMV_PHASE1 | HMV_PHASE2, x, y, beta, skip, inst_done, s);
 }
 
 }  // namespace hmv
@@ -81,6 +82,8 @@ void release(hmv_arena_s *a) {
   (void)hipFree(a->d_work1);
   (void)hipFree(a->d_work2);
   (void)hipFree(a->d_tfin);
+  (void)hipFree(a->d_winst1);
+  (void)hipFree(a->d_winst2);
   (void)hipFree(a->d_tiles1);
   (void)hipFree(a->d_tiles2);
   (void)hipFree(a->d_tk);
@@ -116,7 +119,7 @@ hmv::PlanTuning from_public(int prec, const hmv_tuning *t) {
 }
 
 int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::PlanTuning &tune,
-          hmv_arena *out) {
+          hmv_arena *out, const std::vector<int> *seg = nullptr) {
   if (!out) {
     set_error("hmv_arena_build: null output handle");
     return HMV_ERR_ARG;
@@ -153,6 +156,21 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
       (e = upload(&a->d_codes, P.codes)) != hipSuccess) {
     release(a);
     return hip_fail(e, "hmv_arena_build: upload");
+  }
+  // multi-instance arenas: the instance of every work item (phase 1 by its
+  // first input column, phase 2 by its first row; blocks never cross instances)
+  if (seg && seg->size() > 2) {
+    auto inst_of = [&](int row) {
+      return (int)(std::upper_bound(seg->begin(), seg->end(), row) - seg->begin()) - 1;
+    };
+    std::vector<int> w1(P.work1.size()), w2(P.work2.size());
+    for (size_t i = 0; i < P.work1.size(); ++i) w1[i] = inst_of(P.zseg[P.work1[i].zseg_begin].src);
+    for (size_t i = 0; i < P.work2.size(); ++i) w2[i] = inst_of(P.work2[i].out_base);
+    if ((e = upload(&a->d_winst1, w1)) != hipSuccess ||
+        (e = upload(&a->d_winst2, w2)) != hipSuccess) {
+      release(a);
+      return hip_fail(e, "hmv_arena_build: instance tables");
+    }
   }
   // H-matmat items: every work item's 16-row tiles, in the work lists' order
   std::vector<int2> tiles1, tiles2;
@@ -195,6 +213,8 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
   L.outidx = a->d_outidx;
   L.codes = a->d_codes;
   L.t = a->d_t;
+  L.winst1 = a->d_winst1;
+  L.winst2 = a->d_winst2;
   a->tuning = to_public(tune);
   auto t1 = std::chrono::steady_clock::now();
   hmv_arena_stats &s = a->stats;
@@ -239,7 +259,70 @@ int hmv_init(int device) {
 
 int hmv_arena_build(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
                     hmv_arena *out) {
-  return build(nd, nlf, leaves, leaf_stride, precision, from_public(precision, nullptr), out);
+  int rc = build(nd, nlf, leaves, leaf_stride, precision, from_public(precision, nullptr), out);
+  if (!rc) (*out)->seg = {0, nd};
+  return rc;
+}
+
+int hmv_arena_build_multi(int ninst, const int *nd, const int *nlf, const void *const *leaves,
+                          int leaf_stride, int precision, const hmv_tuning *tuning,
+                          hmv_arena *out) {
+  if (!out || ninst < 1 || !nd || !nlf || !leaves || leaf_stride < (int)sizeof(hmv::RefLeaf)) {
+    set_error("hmv_arena_build_multi: invalid argument");
+    return HMV_ERR_ARG;
+  }
+  // one block-diagonal leaf list: instance i's rows and columns shifted by
+  // the sizes of the instances before it
+  std::vector<int> seg(ninst + 1, 0);
+  int64_t total_lf = 0;
+  for (int i = 0; i < ninst; ++i) {
+    if (nd[i] < 1 || nlf[i] < 1 || !leaves[i]) {
+      set_error("hmv_arena_build_multi: empty instance " + std::to_string(i));
+      return HMV_ERR_ARG;
+    }
+    if ((int64_t)seg[i] + nd[i] > INT32_MAX) {
+      set_error("hmv_arena_build_multi: more than 2^31 unknowns");
+      return HMV_ERR_ARG;
+    }
+    seg[i + 1] = seg[i] + nd[i];
+    total_lf += nlf[i];
+  }
+  if (total_lf > INT32_MAX) {
+    set_error("hmv_arena_build_multi: too many leaves");
+    return HMV_ERR_ARG;
+  }
+  std::vector<hmv::RefLeaf> all;
+  all.reserve((size_t)total_lf);
+  for (int i = 0; i < ninst; ++i) {
+    const char *p = static_cast<const char *>(leaves[i]);
+    for (int l = 0; l < nlf[i]; ++l) {
+      hmv::RefLeaf r;
+      std::memcpy(&r, p + (size_t)l * leaf_stride, sizeof(r));
+      if (r.nstrtl < 1 || r.nstrtt < 1 || r.nstrtl - 1 + r.ndl > nd[i] ||
+          r.nstrtt - 1 + r.ndt > nd[i]) {
+        set_error("hmv_arena_build_multi: leaf outside its instance " + std::to_string(i));
+        return HMV_ERR_ARG;
+      }
+      r.nstrtl += seg[i];
+      r.nstrtt += seg[i];
+      all.push_back(r);
+    }
+  }
+  int rc = build(seg[ninst], (int)total_lf, all.data(), (int)sizeof(hmv::RefLeaf), precision,
+                 from_public(precision, tuning), out, &seg);
+  if (!rc) (*out)->seg = seg;
+  return rc;
+}
+
+int hmv_arena_segments(hmv_arena a, int *offsets, int cap) {
+  if (!a) {
+    set_error("hmv_arena_segments: null arena");
+    return HMV_ERR_ARG;
+  }
+  const int n = (int)a->seg.size() - 1;
+  if (offsets)
+    for (int i = 0; i <= n && i < cap; ++i) offsets[i] = a->seg[i];
+  return n;
 }
 
 int hmv_default_tuning(int precision, hmv_tuning *t) {
@@ -253,7 +336,9 @@ int hmv_default_tuning(int precision, hmv_tuning *t) {
 
 int hmv_arena_build_tuned(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
                           const hmv_tuning *tuning, hmv_arena *out) {
-  return build(nd, nlf, leaves, leaf_stride, precision, from_public(precision, tuning), out);
+  int rc = build(nd, nlf, leaves, leaf_stride, precision, from_public(precision, tuning), out);
+  if (!rc) (*out)->seg = {0, nd};
+  return rc;
 }
 
 int hmv_arena_tuning_get(hmv_arena a, hmv_tuning *t) {
@@ -431,9 +516,9 @@ int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *m
       hipEvent_t e0 = ev[3 * r], e1 = ev[3 * r + 1], e2 = ev[3 * r + 2];
       hipEventRecord(e0, s);
       // phase 1 = t (rank-vector products and the chunk-partial sums)
-      if (a->nwork1 > 0) e = hmv_launch_apply(&a->launch, HMV_PHASE1, x, y, 0.0, nullptr, s);
+      if (a->nwork1 > 0) e = hmv_launch_apply(&a->launch, HMV_PHASE1, x, y, 0.0, nullptr, nullptr, s);
       hipEventRecord(e1, s);
-      if (e == hipSuccess) e = hmv_launch_apply(&a->launch, HMV_PHASE2, x, y, 0.0, nullptr, s);
+      if (e == hipSuccess) e = hmv_launch_apply(&a->launch, HMV_PHASE2, x, y, 0.0, nullptr, nullptr, s);
       hipEventRecord(e2, s);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);

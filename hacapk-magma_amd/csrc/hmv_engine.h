@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "hmv.h"
 #include "hmv_internal.h"
@@ -32,6 +33,8 @@ struct hmv_arena_s {
   int ntiles1 = 0, ntiles2 = 0;
   double *d_tk = nullptr;  // t buffer of the H-matmat (tpart_size * k doubles)
   int64_t tk_cap = 0;      // its capacity in doubles
+  std::vector<int> seg;    // instance row offsets (ninst + 1) of a multi-instance arena
+  int *d_winst1 = nullptr, *d_winst2 = nullptr;  // instance of each work item (multi only)
   hmv_tuning tuning{};     // the tuning the arena was built with
   double *d_hx = nullptr, *d_hy = nullptr;  // scratch for hmv_apply_host
   HmvSolveWork *solve = nullptr;             // solver workspace (lazy)
@@ -44,7 +47,7 @@ int hip_fail(hipError_t e, const char *what);
 // Enqueues y = H x + beta*y; `skip` (device int, may be null) makes every
 // workgroup return early when *skip != 0.
 hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta, const int *skip,
-                         hipStream_t s);
+                         hipStream_t s, const int *inst_done = nullptr);
 void free_solve_work(hmv_arena_s *a);
 // Enqueues Y = H X + beta*Y for k interleaved right-hand sides (1..128).
 hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
@@ -56,7 +59,8 @@ int ensure_tk(hmv_arena_s *a, int k);
 extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *dst, size_t bytes,
                                              hipStream_t stream);
 extern "C" hipError_t hmv_launch_apply(const HmvLaunch *L, int phases, const double *x, double *y,
-                                       double beta, const int *skip, hipStream_t stream);
+                                       double beta, const int *skip, const int *inst_done,
+                                       hipStream_t stream);
 extern "C" hipError_t hmv_launch_block(const HmvLaunch *L, const int2 *tiles1, int ntiles1,
                                        const int2 *tiles2, int ntiles2, int k, const double *x,
                                        double *t, double *y, double beta, const int *skip,

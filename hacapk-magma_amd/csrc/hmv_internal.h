@@ -91,6 +91,7 @@ struct HmvLaunch {
   const int *outidx;
   const int *codes;
   double *t;
+  const int *winst1, *winst2;  // instance of each work item (multi-instance arenas; else null)
 };
 #define HMV_PHASE1 1  // phase 1 and the chunk-partial sums
 #define HMV_PHASE2 2

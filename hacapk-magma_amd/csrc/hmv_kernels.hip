@@ -195,8 +195,11 @@ hmv_stream_kernel(
     const char *__restrict__ arena, const HmvWork *__restrict__ works,
     const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ codes,
     const double *__restrict__ t_in, const int *__restrict__ outidx, double *__restrict__ t_out,
-    double *__restrict__ y, double beta, const int *__restrict__ skip) {
+    double *__restrict__ y, double beta, const int *__restrict__ skip,
+    const int *__restrict__ winst, const int *__restrict__ inst_done) {
   if (skip && *skip) return;
+  // a finished instance of a multi-instance solve needs no more products
+  if (inst_done && inst_done[winst[blockIdx.x]]) return;
   // inputs of one chunk plus the zero slot; reused for the row reduction
   constexpr int ZCH = HMV_ZCHUNK * NT / HMV_WG_THREADS;
   __shared__ __attribute__((aligned(16))) Acc z[ZCH + 4];
@@ -286,39 +289,43 @@ __global__ __launch_bounds__(HMV_WG_THREADS) void hmv_tfinal_kernel(
 
 template <typename Acc, int PHASE, int DT0, int DT1>
 void launch_list(const HmvLaunch &L, const HmvWork *work, int n, int nsmall, int nt_small,
-                 const double *x, double *y, double beta, const int *skip, hipStream_t stream) {
+                 const double *x, double *y, double beta, const int *skip,
+                 const int *inst_done, hipStream_t stream) {
   const double *t_in = PHASE == 2 ? L.t : nullptr;
   double *t_out = PHASE == 1 ? L.t : nullptr;
+  const int *winst = PHASE == 1 ? L.winst1 : L.winst2;
+  if (!winst) inst_done = nullptr;
   if (nsmall > 0) {
     if (nt_small == 64)
       hipLaunchKernelGGL((hmv_stream_kernel<Acc, PHASE, DT0, DT1, 64>), dim3(nsmall), dim3(64), 0,
                          stream, L.arena, work, L.zseg, x, L.codes, t_in, L.outidx, t_out, y, beta,
-                         skip);
+                         skip, winst, inst_done);
     else
       hipLaunchKernelGGL((hmv_stream_kernel<Acc, PHASE, DT0, DT1, 128>), dim3(nsmall), dim3(128), 0,
                          stream, L.arena, work, L.zseg, x, L.codes, t_in, L.outidx, t_out, y, beta,
-                         skip);
+                         skip, winst, inst_done);
   }
   if (n > nsmall)
     hipLaunchKernelGGL((hmv_stream_kernel<Acc, PHASE, DT0, DT1>), dim3(n - nsmall),
                        dim3(HMV_WG_THREADS), 0, stream, L.arena, work + nsmall, L.zseg, x, L.codes,
-                       t_in, L.outidx, t_out, y, beta, skip);
+                       t_in, L.outidx, t_out, y, beta, skip, winst ? winst + nsmall : nullptr,
+                       inst_done);
 }
 
 template <typename Acc, int DT0, int DT1>
 void launch_mode(const HmvLaunch &L, int phases, const double *x, double *y, double beta,
-                 const int *skip, hipStream_t stream) {
+                 const int *skip, const int *inst_done, hipStream_t stream) {
   if (phases & HMV_PHASE1) {
     // phase 1 reads only low-rank factors: both parts use DT1's type
     launch_list<Acc, 1, DT1, DT1>(L, L.work1, L.nwork1, L.nwork1_small, L.nt1_small, x, nullptr,
-                                  0.0, skip, stream);
+                                  0.0, skip, inst_done, stream);
     if (L.ntfin > 0)
       hipLaunchKernelGGL(hmv_tfinal_kernel, dim3((L.ntfin + HMV_WG_THREADS - 1) / HMV_WG_THREADS),
                          dim3(HMV_WG_THREADS), 0, stream, L.tfin, L.ntfin, L.t, skip);
   }
   if (phases & HMV_PHASE2)
     launch_list<Acc, 2, DT0, DT1>(L, L.work2, L.nwork2, L.nwork2_small, L.nt2_small, x, y, beta,
-                                  skip, stream);
+                                  skip, inst_done, stream);
 }
 
 // Plain streaming kernels of known traffic, for bandwidth calibration.
@@ -357,13 +364,16 @@ extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *ds
 
 // Launches the phases selected by `phases` (HMV_PHASE1: t and the
 // chunk-partial sums; HMV_PHASE2: y = [A|U][x;t] + beta*y) of one apply.
+// inst_done (multi-instance arenas, may be null): workgroups of instances
+// whose flag is set return at once.
 extern "C" hipError_t hmv_launch_apply(const HmvLaunch *L, int phases, const double *x, double *y,
-                                       double beta, const int *skip, hipStream_t stream) {
+                                       double beta, const int *skip, const int *inst_done,
+                                       hipStream_t stream) {
   if (L->mode == 0)
-    launch_mode<double, HMV_DT_F64, HMV_DT_F64>(*L, phases, x, y, beta, skip, stream);
+    launch_mode<double, HMV_DT_F64, HMV_DT_F64>(*L, phases, x, y, beta, skip, inst_done, stream);
   else if (L->mode == 1)
-    launch_mode<float, HMV_DT_F32, HMV_DT_F32>(*L, phases, x, y, beta, skip, stream);
+    launch_mode<float, HMV_DT_F32, HMV_DT_F32>(*L, phases, x, y, beta, skip, inst_done, stream);
   else
-    launch_mode<float, HMV_DT_F32, HMV_DT_BF16>(*L, phases, x, y, beta, skip, stream);
+    launch_mode<float, HMV_DT_F32, HMV_DT_BF16>(*L, phases, x, y, beta, skip, inst_done, stream);
   return hipGetLastError();
 }

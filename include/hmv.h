@@ -118,6 +118,19 @@ int hmv_arena_tuning_get(hmv_arena a, hmv_tuning *t);
 int hmv_autotune(int nd, int nlf, const void *leaves, int leaf_stride, int precision, int reps,
                  hmv_tuning *best, double *best_ms, double *default_ms);
 
+/* Many independent instances in one arena: instance i (nd[i] unknowns,
+ * nlf[i] leaves at leaves[i], all with the same record stride) occupies rows
+ * and columns [off_i, off_i + nd[i]) of one block-diagonal H-matrix, off_i =
+ * nd[0] + ... + nd[i-1].  Every apply covers all instances in one launch per
+ * phase; vectors are the instances' vectors concatenated.  tuning may be
+ * NULL.  No reference counterpart (the reference holds one H-matrix). */
+int hmv_arena_build_multi(int ninst, const int *nd, const int *nlf, const void *const *leaves,
+                          int leaf_stride, int precision, const hmv_tuning *tuning,
+                          hmv_arena *out);
+/* Number of instances of an arena (1 unless built by hmv_arena_build_multi);
+ * offsets (may be NULL) receives up to cap of the ninst+1 row offsets. */
+int hmv_arena_segments(hmv_arena a, int *offsets, int cap);
+
 int hmv_arena_stats_get(hmv_arena a, hmv_arena_stats *s);
 int hmv_arena_free(hmv_arena a);
 
@@ -192,6 +205,18 @@ typedef struct {
 } hmv_column_info;
 int hmv_block_bicgstab(hmv_arena a, int k, const double *b, double *u, double eps, int maxit,
                        int check_every, hmv_column_info *cols, hmv_solve_info *info);
+
+/* One BiCGSTAB per instance of a multi-instance arena (hmv_arena_build_multi),
+ * all advancing together: one H-matvec launch per phase covers every
+ * instance, scalars and stopping are per instance (a finished instance is
+ * frozen), one hipGraph per iteration.  b, u: the instances' vectors
+ * concatenated.  inst (ninst entries, may be NULL) receives per-instance
+ * steps, status and residuals; inst_seconds (may be NULL) the wall time at
+ * which each instance was seen finished (residual read-back every
+ * check_every iterations).  No reference counterpart. */
+int hmv_multi_bicgstab(hmv_arena a, const double *b, double *u, double eps, int maxit,
+                       int check_every, hmv_column_info *inst, double *inst_seconds,
+                       hmv_solve_info *info);
 
 const char *hmv_last_error(void);
 
