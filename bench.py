@@ -15,7 +15,10 @@ Also reported: per-phase HIP-event timings and the roofline of the dominant
 kernel, BiCGSTAB time-to-1e-8 in FP64 and with FP32 / BF16 storage under
 FP64 iterative refinement, the CPU baseline (the oracle's OpenMP per-leaf
 GEMV restatement of the reference CPU path, timed on this host), and the
-leaf-rank / leaf-size histograms.
+leaf-rank / leaf-size histograms.  Unless --no-extras: on one icosphere of
+327,680 unknowns the per-precision apply rates, FP64 vs mixed-precision
+refined solves, the multi-RHS H-matmat (k = 8, 32, 128) and a 32-RHS block
+BiCGSTAB; and a batch of 48 uneven instances solved in one arena.
 """
 import argparse
 import json
@@ -57,6 +60,8 @@ def main():
     ap.add_argument("--subdiv", type=int, default=6, help="icosphere subdivisions per sphere")
     ap.add_argument("--quick", action="store_true", help="matvec only (for profiler runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 327,680-unknown precision / multi-RHS study and the 48-instance batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -183,13 +188,14 @@ def main():
         solves["fp64"] = {"seconds": round(i64["seconds"], 4), "iterations": i64["nstp"],
                           "true_relres": i64["true_relres"], "status": i64["status_name"]}
         log("BiCGSTAB FP64: %d it, %.3f s, true relres %.2e" % (i64["nstp"], i64["seconds"], i64["true_relres"]))
-        for prec in ("f32", "bf16"):
+        for prec, ie in (("f32", 1e-4), ("bf16", 1e-2)):
             alow = H.Arena(hm, prec)
-            H.bicgstab_ir(a64, alow, b, eps=1e-8, inner_eps=1e-4, max_outer=40, max_inner=2000)  # warm
-            u, inf, hist = H.bicgstab_ir(a64, alow, b, eps=1e-8, inner_eps=1e-4, max_outer=40, max_inner=2000)
+            H.bicgstab_ir(a64, alow, b, eps=1e-8, inner_eps=ie, max_outer=40, max_inner=2000)  # warm
+            u, inf, hist = H.bicgstab_ir(a64, alow, b, eps=1e-8, inner_eps=ie, max_outer=40, max_inner=2000)
             err = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
             solves["mixed_" + prec] = {
-                "seconds": round(inf["seconds"], 4), "outer": inf["outer"], "inner_iterations": inf["inner_total"],
+                "seconds": round(inf["seconds"], 4), "inner_eps": ie, "outer": inf["outer"],
+                "inner_iterations": inf["inner_total"],
                 "true_relres": inf["relres"], "status": inf["status_name"],
                 "rel_err_vs_fp64_solution": err, "outer_residuals": [float(v) for v in hist],
                 "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3),
@@ -225,7 +231,122 @@ def main():
     else:
         result["cpu_baseline"] = None
 
+    if not args.quick and not args.no_extras:
+        result["sphere_327680"] = study_327680(H, log)
+        result["instances_48"] = study_instances(H, log)
+
     print(json.dumps(result), flush=True)
+
+
+def study_327680(H, log):
+    """One icosphere s=7 (327,680 unknowns), ACA+ 1e-5: H-matvec rate per
+    storage precision, BiCGSTAB to 1e-8 in FP64 vs FP64 iterative refinement
+    with FP32 / BF16 inner solves (RHS = 1 and U[-1,1] seed 2), and the
+    multi-RHS H-matmat and block BiCGSTAB (k RHS U[-1,1] seed 3)."""
+    out = {}
+    t0 = time.perf_counter()
+    hm = H.HMatrix(*H.icosphere(7), H.baseline_params(aca_eps=1e-5))
+    nd = hm.nd
+    out["construction_s"] = round(time.perf_counter() - t0, 2)
+    arenas = {p: H.Arena(hm, p) for p in ("f64", "f32", "bf16")}
+    x = H.DeviceBuffer.from_array(H.uniform(1, nd))
+    y = H.DeviceBuffer.from_array(np.zeros(nd))
+    rates = {}
+    for p, a in arenas.items():
+        m1, m2, ma = a.apply_timed(x.ptr, y.ptr, 30)
+        rates[p] = {"ms": round(ma, 4), "storage_alg_GBs": round(a.stats["alg_bytes"] / ma / 1e6, 1),
+                    "arena_GB": round(a.stats["arena_bytes"] / 1e9, 3), "tuning": a.tuning}
+        log("s=7 %s apply %.4f ms, %.1f GB/s at storage precision" % (p, ma, rates[p]["storage_alg_GBs"]))
+    out["apply"] = rates
+    solves = {}
+    inner = {"f32": 1e-4, "bf16": 1e-2}
+    for rname, b in (("ones", np.ones(nd)), ("uniform_seed2", H.uniform(2, nd))):
+        a64 = arenas["f64"]
+        a64.bicgstab(b, eps=1e-8, maxit=3000, check_every=8)
+        u64, i64, _ = a64.bicgstab(b, eps=1e-8, maxit=3000, check_every=8)
+        row = {"fp64": {"seconds": round(i64["seconds"], 4), "iterations": i64["nstp"],
+                        "true_relres": i64["true_relres"]}}
+        for p in ("f32", "bf16"):
+            H.bicgstab_ir(a64, arenas[p], b, eps=1e-8, inner_eps=inner[p], max_outer=60, max_inner=3000)
+            u, inf, hist = H.bicgstab_ir(a64, arenas[p], b, eps=1e-8, inner_eps=inner[p], max_outer=60,
+                                         max_inner=3000)
+            row["mixed_" + p] = {
+                "seconds": round(inf["seconds"], 4), "inner_eps": inner[p], "outer": inf["outer"],
+                "inner_iterations": inf["inner_total"], "true_relres": inf["relres"],
+                "status": inf["status_name"], "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3),
+                "rel_err_vs_fp64_solution": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                "outer_residuals": [float(v) for v in hist]}
+            log("s=7 %s: FP64 %.4f s (%d it) vs %s-IR %.4f s (%.2fx), relres %.2e"
+                % (rname, i64["seconds"], i64["nstp"], p, inf["seconds"], inf["seconds"] / i64["seconds"],
+                   inf["relres"]))
+        solves[rname] = row
+    out["bicgstab"] = solves
+    # multi-RHS
+    a64 = arenas["f64"]
+    st = a64.stats
+    E = st["entries_lr"] + st["entries_dense"]
+    single_ms = rates["f64"]["ms"]
+    mm = {}
+    rng = np.random.default_rng(3)
+    for k in (8, 32, 128):
+        X = H.DeviceBuffer.from_array(rng.uniform(-1, 1, nd * k))
+        Y = H.DeviceBuffer.from_array(np.zeros(nd * k))
+        ms = a64.apply_block_timed(X.ptr, Y.ptr, k, 5)
+        mm[str(k)] = {"ms": round(ms, 4), "ms_per_rhs": round(ms / k, 4),
+                      "speedup_per_rhs_vs_single": round(single_ms * k / ms, 2),
+                      "TFLOPs": round(2.0 * E * k / ms / 1e9, 2)}
+        log("s=7 H-matmat k=%d: %.3f ms (%.2f ms/RHS, %.1fx single, %.1f TFLOP/s)"
+            % (k, ms, ms / k, single_ms * k / ms, 2.0 * E * k / ms / 1e9))
+        X.free()
+        Y.free()
+    out["hmatmat_fp64"] = mm
+    B = rng.uniform(-1, 1, (nd, 32))
+    a64.block_bicgstab(B, eps=1e-8, maxit=3000)
+    U, cols, inf = a64.block_bicgstab(B, eps=1e-8, maxit=3000, check_every=8)
+    out["block_bicgstab_k32"] = {
+        "seconds": round(inf["seconds"], 4), "iterations": inf["nstp"], "status": inf["status_name"],
+        "worst_true_relres": inf["true_relres"], "rhs_solves_per_s": round(32 / inf["seconds"], 2),
+        "single_rhs_solves_per_s": round(1.0 / solves["uniform_seed2"]["fp64"]["seconds"], 2)}
+    log("s=7 block BiCGSTAB k=32: %.3f s, %.1f RHS-solves/s (single: %.1f)"
+        % (inf["seconds"], 32 / inf["seconds"], 1.0 / solves["uniform_seed2"]["fp64"]["seconds"]))
+    for a in arenas.values():
+        a.free()
+    return out
+
+
+def study_instances(H, log):
+    """48 icospheres with s in {4,5,6} and radii U[0.5,2] (seed 4), ACA+ 1e-4,
+    FP64: one arena, one BiCGSTAB per instance to 1e-8 (RHS = 1)."""
+    rng = np.random.default_rng(4)
+    subs = rng.choice([4, 5, 6], size=48)
+    radii = rng.uniform(0.5, 2.0, size=48)
+    t0 = time.perf_counter()
+    hms = [H.HMatrix(*H.icosphere(int(s), float(r)), H.baseline_params(aca_eps=1e-4))
+           for s, r in zip(subs, radii)]
+    tcons = time.perf_counter() - t0
+    m = H.MultiArena(hms, "f64")
+    x = H.DeviceBuffer.from_array(H.uniform(1, m.nd))
+    y = H.DeviceBuffer.from_array(np.zeros(m.nd))
+    m1, m2, ma = m.apply_timed(x.ptr, y.ptr, 20)
+    b = np.ones(m.nd)
+    m.bicgstab_multi(b, eps=1e-8, maxit=3000, check_every=4)
+    u, inst, secs, inf = m.bicgstab_multi(b, eps=1e-8, maxit=3000, check_every=4)
+    steps = [d["nstp"] for d in inst]
+    out = {"instances": 48, "unknowns": int(m.nd),
+           "sizes": {"5120": int((subs == 4).sum()), "20480": int((subs == 5).sum()),
+                     "81920": int((subs == 6).sum())},
+           "construction_s": round(tcons, 2), "arena_GB": round(m.stats["arena_bytes"] / 1e9, 2),
+           "apply_all_ms": round(ma, 4), "apply_alg_GBs": round(m.stats["alg_bytes"] / ma / 1e6, 1),
+           "solve_seconds": round(inf["seconds"], 4), "status": inf["status_name"],
+           "worst_true_relres": inf["true_relres"], "steps_min": min(steps), "steps_max": max(steps),
+           "unknowns_per_s": round(m.nd / inf["seconds"], 1),
+           "instance_seconds_p50": round(float(np.percentile(secs, 50)), 4),
+           "instance_seconds_p99": round(float(np.percentile(secs, 99)), 4)}
+    log("48 instances (%d unknowns): one apply %.3f ms; solve %.4f s, %.3e unknowns/s, p50 %.4f s p99 %.4f s"
+        % (m.nd, ma, inf["seconds"], m.nd / inf["seconds"], out["instance_seconds_p50"],
+           out["instance_seconds_p99"]))
+    m.free()
+    return out
 
 
 if __name__ == "__main__":
