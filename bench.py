@@ -60,9 +60,15 @@ def main():
     ap.add_argument("--subdiv", type=int, default=6, help="icosphere subdivisions per sphere")
     ap.add_argument("--quick", action="store_true", help="matvec only (for profiler runs)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
+    ap.add_argument("--hmatmat", action="store_true",
+                    help="only the FP64 H-matmat (k = 8, 32, 128) on 327,680 unknowns (for profiler runs)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 327,680-unknown precision / multi-RHS study and the 48-instance batch")
     args = ap.parse_args()
+    if args.hmatmat:
+        import __graft_entry__ as G
+        print(json.dumps({"hmatmat_fp64_327680": hmatmat_only(G.load_package(), log)}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)
@@ -311,6 +317,26 @@ def study_327680(H, log):
         % (inf["seconds"], 32 / inf["seconds"], 1.0 / solves["uniform_seed2"]["fp64"]["seconds"]))
     for a in arenas.values():
         a.free()
+    return out
+
+
+def hmatmat_only(H, log, reps=5):
+    """FP64 H-matmat timings (k = 8, 32, 128) on one icosphere s=7."""
+    hm = H.HMatrix(*H.icosphere(7), H.baseline_params(aca_eps=1e-5))
+    a = H.Arena(hm, "f64")
+    st = a.stats
+    E = st["entries_lr"] + st["entries_dense"]
+    rng = np.random.default_rng(3)
+    out = {}
+    for k in (8, 32, 128):
+        X = H.DeviceBuffer.from_array(rng.uniform(-1, 1, hm.nd * k))
+        Y = H.DeviceBuffer.from_array(np.zeros(hm.nd * k))
+        ms = a.apply_block_timed(X.ptr, Y.ptr, k, reps)
+        out[str(k)] = {"ms": round(ms, 4), "TFLOPs": round(2.0 * E * k / ms / 1e9, 2)}
+        log("H-matmat k=%d: %.3f ms, %.1f TFLOP/s" % (k, ms, 2.0 * E * k / ms / 1e9))
+        X.free()
+        Y.free()
+    a.free()
     return out
 
 

@@ -195,6 +195,8 @@ def hmv_lib():
                                             C.POINTER(Tuning), C.POINTER(vp)]
         L.hmv_default_tuning.argtypes = [C.c_int, C.POINTER(Tuning)]
         L.hmv_apply_block.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp]
+        L.hmv_bicgstab_ru.argtypes = [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_double,
+                                      C.POINTER(SolveInfo)]
         L.hmv_arena_build_multi.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                             C.POINTER(vp), C.c_int, C.c_int, C.POINTER(Tuning),
                                             C.POINTER(vp)]
@@ -682,6 +684,18 @@ def bicgstab_ir(a64, alow, b, u0=None, eps=1e-8, inner_eps=1e-4, max_outer=30, m
            "hmv_bicgstab_ir")
     u = du.download()
     return u, info.as_dict(), hist[: info.outer + 1]
+
+
+def bicgstab_ru(a64, alow, b, u0=None, eps=1e-8, maxit=3000, check_every=4, delta=0.1):
+    """Mixed-precision BiCGSTAB with reliable updates (hmv_bicgstab_ru).
+    Returns (u, info); info["outer"] = residual replacements."""
+    b = np.ascontiguousarray(b, np.float64)
+    u0 = np.zeros_like(b) if u0 is None else np.ascontiguousarray(u0, np.float64)
+    db, du = DeviceBuffer.from_array(b), DeviceBuffer.from_array(u0)
+    info = SolveInfo()
+    _check(hmv_lib().hmv_bicgstab_ru(a64.handle, alow.handle, db.ptr, du.ptr, float(eps), int(maxit),
+                                     int(check_every), float(delta), C.byref(info)), "hmv_bicgstab_ru")
+    return du.download(), info.as_dict()
 
 
 # ---------------------------------------------------------------------------

@@ -91,11 +91,12 @@ __device__ inline void reduce_parts(int nb, const double *part, SolveState *st, 
 
 __device__ inline bool finite(double v) { return isfinite(v); }
 
-// stage: 0 setup (d0 = (r,r), d1 = (b,b)), 1 alpha, 2 zeta, 3 beta
+// stage: 0 setup (d0 = (r,r), d1 = (b,b)), 1 alpha, 2 zeta, 3 beta,
+// 4 residual replacement (d0 = (r^, r), d1 = (r, r) of the true residual)
 __global__ __launch_bounds__(kThreads) void k_final(int nb, const double *__restrict__ part,
                                                     SolveState *st, int stage,
                                                     double *__restrict__ hist) {
-  if (st->done && stage != 0) return;
+  if (st->done && stage != 0 && stage != 4) return;
   __shared__ double sh[4];
   reduce_parts(nb, part, st, sh);
   if (threadIdx.x != 0) return;
@@ -132,6 +133,25 @@ __global__ __launch_bounds__(kThreads) void k_final(int nb, const double *__rest
     st->zeta = znom / zden;
     if (zden == 0.0 || !finite(st->zeta)) {
       st->status = finite(zden) ? HMV_SOLVE_BREAKDOWN : HMV_SOLVE_NONFINITE;
+      st->done = 1;
+    }
+  } else if (stage == 4) {
+    // the iterated residual is replaced by the true one; the iteration goes
+    // on (p and r^ are kept) unless the true residual has converged
+    st->rr = st->d0;
+    st->rnorm = sqrt(st->d1);
+    st->done = 0;
+    st->status = HMV_SOLVE_MAXITER;
+    if (!finite(st->rnorm)) {
+      st->status = HMV_SOLVE_NONFINITE;
+      st->done = 1;
+    } else if (st->rnorm / st->bnorm < st->eps) {
+      st->status = HMV_SOLVE_CONVERGED;
+      st->done = 1;
+    } else if (st->step >= st->maxit) {
+      st->done = 1;
+    } else if (st->rr == 0.0) {
+      st->status = HMV_SOLVE_BREAKDOWN;
       st->done = 1;
     }
   } else {
@@ -538,6 +558,107 @@ int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, d
     info->outer = outer;
     info->inner_total = inner_total;
   }
+  return HMV_OK;
+}
+
+// Mixed-precision BiCGSTAB with reliable updates: the iteration runs with
+// the low-precision arena `alow`; u, the vectors and the scalars stay FP64;
+// whenever the iterated residual has dropped by `delta` since the last
+// replacement (or the iteration believes it has converged), r = b - A64 u
+// is recomputed with the FP64 arena and replaces the iterated residual
+// without restarting the Krylov recurrence.
+int hmv_bicgstab_ru(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
+                    int maxit, int check_every, double delta, hmv_solve_info *info) {
+  if (!a64 || !alow || !b || !u || a64->nd != alow->nd || maxit < 0 || !(delta > 0.0) ||
+      delta >= 1.0) {
+    hmv::set_error("hmv_bicgstab_ru: invalid arguments (0 < delta < 1)");
+    return HMV_ERR_ARG;
+  }
+  int rc = ensure_work(alow, maxit + 1);
+  if (rc) return rc;
+  rc = ensure_work(a64, 2);
+  if (rc) return rc;
+  if (check_every < 1) check_every = 1;
+  HmvSolveWork *w = alow->solve;
+  const int n = w->n, g = grid_for(n);
+  hipStream_t s = w->stream;
+  hipError_t e;
+  auto t0 = std::chrono::steady_clock::now();
+  SolveState init{};
+  init.eps = eps;
+  init.maxit = maxit;
+  SolveState host{};
+  // r = b - A64 u, r^ = r, p = v = 0
+  if ((e = hipMemcpyAsync(w->st, &init, sizeof(init), hipMemcpyHostToDevice, s)) != hipSuccess ||
+      (e = hmv::enqueue_apply(a64, u, w->r, 0.0, nullptr, s)) != hipSuccess)
+    return hmv::hip_fail(e, "bicgstab_ru: setup");
+  hipLaunchKernelGGL(k_init, dim3(g), dim3(kThreads), 0, s, n, b, w->r, w->rhat, w->p, w->v);
+  hipLaunchKernelGGL(k_dots, dim3(g), dim3(kThreads), 0, s, n, w->r, w->r, b, b, w->part,
+                     (const SolveState *)nullptr);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, s, g, w->part, w->st, 0, w->hist);
+  if ((e = hipGetLastError()) != hipSuccess) return hmv::hip_fail(e, "bicgstab_ru: setup");
+  if (w->g_u != u || w->g_arena != alow || !w->exec) {
+    if (w->exec) {
+      (void)hipGraphExecDestroy(w->exec);
+      (void)hipGraphDestroy(w->graph);
+      w->exec = nullptr;
+      w->graph = nullptr;
+    }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess ||
+        (e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hmv::hip_fail(e, "bicgstab_ru: capture");
+    hipError_t ec = enqueue_iteration(alow, w, u);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(s, &graph);
+    if (ec != hipSuccess) return hmv::hip_fail(ec, "bicgstab_ru: capture");
+    if (e != hipSuccess) return hmv::hip_fail(e, "bicgstab_ru: end capture");
+    w->graph = graph;
+    if ((e = hipGraphInstantiate(&w->exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+      return hmv::hip_fail(e, "bicgstab_ru: instantiate");
+    w->g_u = u;
+    w->g_arena = alow;
+  }
+  auto read_state = [&]() -> hipError_t {
+    hipError_t re = hipMemcpyAsync(&host, w->st, sizeof(host), hipMemcpyDeviceToHost, s);
+    return re == hipSuccess ? hipStreamSynchronize(s) : re;
+  };
+  if ((e = read_state()) != hipSuccess) return hmv::hip_fail(e, "bicgstab_ru: state");
+  double r_last = host.rnorm;
+  int updates = 0;
+  while (!host.done) {  // done: converged on the true residual, or stopped
+    const int burst = std::min(check_every, std::max(1, maxit - host.step));
+    for (int k = 0; k < burst; ++k)
+      if ((e = hipGraphLaunch(w->exec, s)) != hipSuccess)
+        return hmv::hip_fail(e, "bicgstab_ru: iteration launch");
+    if ((e = read_state()) != hipSuccess) return hmv::hip_fail(e, "bicgstab_ru: state");
+    const bool sloppy_done = host.done && host.status == HMV_SOLVE_CONVERGED;
+    if (host.done && !sloppy_done) break;  // breakdown, non-finite or out of steps
+    if (sloppy_done || host.rnorm < delta * r_last) {
+      // reliable update: r = b - A64 u replaces the iterated residual
+      if ((e = hmv::enqueue_apply(a64, u, w->r, 0.0, nullptr, s)) != hipSuccess)
+        return hmv::hip_fail(e, "bicgstab_ru: replacement");
+      hipLaunchKernelGGL(k_residual, dim3(g), dim3(kThreads), 0, s, n, b, w->r);
+      hipLaunchKernelGGL(k_dots, dim3(g), dim3(kThreads), 0, s, n, w->rhat, w->r, w->r, w->r,
+                         w->part, (const SolveState *)nullptr);
+      hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, s, g, w->part, w->st, 4, w->hist);
+      if ((e = read_state()) != hipSuccess) return hmv::hip_fail(e, "bicgstab_ru: state");
+      ++updates;
+      r_last = host.rnorm;
+      if (host.done) break;
+    }
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  hmv_solve_info loc{};
+  loc.nstp = host.step;
+  loc.status = host.done ? host.status : HMV_SOLVE_MAXITER;
+  loc.relres = host.rnorm / host.bnorm;
+  loc.seconds = std::chrono::duration<double>(t1 - t0).count();
+  loc.outer = updates;
+  loc.inner_total = host.step;
+  rc = true_relres(a64, b, u, &loc.true_relres);
+  if (rc) return rc;
+  if (loc.status == HMV_SOLVE_CONVERGED && !(loc.true_relres < eps)) loc.status = HMV_SOLVE_MAXITER;
+  if (info) *info = loc;
   return HMV_OK;
 }
 

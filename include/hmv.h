@@ -189,6 +189,15 @@ int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, d
                     double inner_eps, int max_outer, int max_inner, int check_every, double *hist,
                     hmv_solve_info *info);
 
+/* Mixed-precision BiCGSTAB with reliable updates: one BiCGSTAB whose
+ * products use the low-precision arena `alow`, with u, vectors and scalars
+ * in FP64; whenever the iterated residual has fallen by `delta` (0..1) since
+ * the last replacement, or the iteration reports convergence, the true
+ * residual b - A64 u (FP64 arena) replaces it without restarting.  Stops on
+ * ||b - A64 u|| / ||b|| < eps.  info->outer = number of replacements. */
+int hmv_bicgstab_ru(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
+                    int maxit, int check_every, double delta, hmv_solve_info *info);
+
 /* Block BiCGSTAB: k independent solves A u_j = b_j (k a power of two,
  * 1..128) sharing one H-matmat per step; b, u device arrays N x k
  * row-interleaved (u holds the initial guesses).  Column j stops at

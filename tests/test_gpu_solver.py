@@ -97,3 +97,19 @@ def test_mixed_precision_refinement(gpu, O, sphere4, prec):
     assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-5
     a64.free()
     alow.free()
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_reliable_update_mixed_precision(gpu, two_spheres4, prec):
+    """BiCGSTAB on the low-precision arena with FP64 residual replacement
+    reaches the FP64 true-residual bar and the FP64 solution."""
+    hm = two_spheres4
+    a64, alow = gpu.Arena(hm, "f64"), gpu.Arena(hm, prec)
+    b = gpu.uniform(11, hm.nd)
+    u64, i64, _ = a64.bicgstab(b, eps=1e-8, maxit=2000)
+    u, info = gpu.bicgstab_ru(a64, alow, b, eps=1e-8, maxit=2000, check_every=2, delta=0.1)
+    print(prec, "reliable updates:", info, "fp64 steps", i64["nstp"], flush=True)
+    assert info["status_name"] == "converged" and info["true_relres"] < 1e-8
+    assert info["outer"] >= 1
+    assert np.linalg.norm(u - u64) / np.linalg.norm(u64) < 1e-6
+    a64.free(); alow.free()
