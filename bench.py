@@ -272,6 +272,12 @@ def study_327680(H, log):
         u64, i64, _ = a64.bicgstab(b, eps=1e-8, maxit=3000, check_every=8)
         row = {"fp64": {"seconds": round(i64["seconds"], 4), "iterations": i64["nstp"],
                         "true_relres": i64["true_relres"]}}
+        a64.gcrm(b, eps=1e-8, maxit=3000, mreset=8, check_every=8, history=False)
+        _, ig, _ = a64.gcrm(b, eps=1e-8, maxit=3000, mreset=8, check_every=8, history=False)
+        row["gcrm8_fp64"] = {"seconds": round(ig["seconds"], 4), "iterations": ig["nstp"],
+                             "true_relres": ig["true_relres"], "status": ig["status_name"]}
+        log("s=7 %s: GCR(8) FP64 %.4f s (%d it), relres %.2e"
+            % (rname, ig["seconds"], ig["nstp"], ig["true_relres"]))
         for p in ("f32", "bf16"):
             H.bicgstab_ir(a64, arenas[p], b, eps=1e-8, inner_eps=inner[p], max_outer=60, max_inner=3000)
             u, inf, hist = H.bicgstab_ir(a64, arenas[p], b, eps=1e-8, inner_eps=inner[p], max_outer=60,

@@ -215,6 +215,8 @@ def hmv_lib():
         L.hmv_arena_tbytes.restype = C.c_int64
         L.hmv_apply.argtypes = [vp, vp, vp, C.c_double, vp]
         L.hmv_apply_host.argtypes = [vp, _D, _D]
+        L.hmv_gcrm.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_int, _D,
+                               C.POINTER(SolveInfo)]
         L.hmv_bicgstab.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, _D,
                                    C.POINTER(SolveInfo)]
         L.hmv_bicgstab_ir.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
@@ -579,6 +581,25 @@ class Arena:
             hist = hist[: info.nstp + 1]
         return u, info.as_dict(), hist
 
+    def gcrm(self, b, u0=None, eps=1e-8, maxit=500, mreset=8, check_every=1, history=True):
+        """GCR(mreset) on device (HACApK_gcrm_lfmtx, m_HACApK_solve.f90:506-571);
+        host b/u0 in cluster order.  Returns (u, info, hist)."""
+        b = np.ascontiguousarray(b, np.float64)
+        u0 = np.zeros_like(b) if u0 is None else np.ascontiguousarray(u0, np.float64)
+        db, du = DeviceBuffer.from_array(b), DeviceBuffer.from_array(u0)
+        hist = np.full(maxit + 1, np.nan) if history else None
+        info = SolveInfo()
+        try:
+            _check(hmv_lib().hmv_gcrm(self._a, db.ptr, du.ptr, eps, maxit, mreset, check_every,
+                                      _dptr(hist) if history else None, C.byref(info)), "hmv_gcrm")
+            u = du.download()
+        finally:
+            db.free()
+            du.free()
+        if history:
+            hist = hist[: info.nstp + 1]
+        return u, info.as_dict(), hist
+
 
 def plan_tables(hm, precision="f64", tuning=None):
     """Host-side repack of an HMatrix (hmv_plan_create_tuned): numpy copies of
@@ -751,14 +772,21 @@ def bicgstab_cax_lfmtx_gpu(leafmtxp, u, b, param):
     return nstp.value, lrtrn.value
 
 
-def solve(hm, rhs, sol0=None, ztol=1e-8, mstep=500, arena=None):
+def solve(hm, rhs, sol0=None, ztol=1e-8, mstep=500, arena=None, solver="bicgstab", mreset=8):
     """HACApK_solve (m_HACApK_use.f90:190-262) on the engine: permute with lod,
-    BiCGSTAB on the device, un-permute.  Returns (sol, info, hist)."""
+    solve on the device, un-permute.  solver selects as param(85) does
+    (m_HACApK_use.f90:235-242): "bicgstab" (1) or "gcrm" (2), the latter with
+    mreset = param(87) directions.  Returns (sol, info, hist)."""
+    if solver not in ("bicgstab", "gcrm"):
+        raise ValueError("solver must be 'bicgstab' or 'gcrm'")
     own = arena is None
     a = arena or Arena(hm, "f64")
     b = hm.to_cluster_order(rhs)
     u0 = None if sol0 is None else hm.to_cluster_order(sol0)
-    u, info, hist = a.bicgstab(b, u0, eps=ztol, maxit=mstep)
+    if solver == "gcrm":
+        u, info, hist = a.gcrm(b, u0, eps=ztol, maxit=mstep, mreset=mreset)
+    else:
+        u, info, hist = a.bicgstab(b, u0, eps=ztol, maxit=mstep)
     if own:
         a.free()
     return hm.to_original_order(u), info, hist

@@ -189,22 +189,16 @@ __device__ inline void run_cols(const Part<DT> &P, int lo, int hi, int cg, int m
 // PHASE 2: y[out_base + r] = row sums + beta * y.
 // DT0/DT1: storage types of the two column parts (fixed per arena precision).
 // NT: threads per workgroup; inputs are staged NT*8 columns at a time.
-template <typename Acc, int PHASE, int DT0, int DT1, int NT = HMV_WG_THREADS>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
-hmv_stream_kernel(
-    const char *__restrict__ arena, const HmvWork *__restrict__ works,
+// One work item w; z: LDS of ZCH + 4 accumulation-type slots (inputs of one
+// chunk plus the zero slot; reused for the row reduction).
+template <typename Acc, int PHASE, int DT0, int DT1, int NT>
+__device__ inline void stream_item(
+    const HmvWork &w, Acc *__restrict__ z, const char *__restrict__ arena,
     const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ codes,
     const double *__restrict__ t_in, const int *__restrict__ outidx, double *__restrict__ t_out,
-    double *__restrict__ y, double beta, const int *__restrict__ skip,
-    const int *__restrict__ winst, const int *__restrict__ inst_done) {
-  if (skip && *skip) return;
-  // a finished instance of a multi-instance solve needs no more products
-  if (inst_done && inst_done[winst[blockIdx.x]]) return;
-  // inputs of one chunk plus the zero slot; reused for the row reduction
+    double *__restrict__ y, double beta) {
   constexpr int ZCH = HMV_ZCHUNK * NT / HMV_WG_THREADS;
-  __shared__ __attribute__((aligned(16))) Acc z[ZCH + 4];
   static_assert(2 * NT <= ZCH, "reduction buffer fits in z");
-  const HmvWork w = works[blockIdx.x];
   const int tid = threadIdx.x;
   constexpr int vr = (DT0 == HMV_DT_F64) ? 1 : 2;
   const int q = w.rows_p / vr;
@@ -266,6 +260,22 @@ hmv_stream_kernel(
       y[row] = beta == 0.0 ? v : __builtin_fma(beta, y[row], v);
     }
   }
+}
+
+template <typename Acc, int PHASE, int DT0, int DT1, int NT = HMV_WG_THREADS>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+hmv_stream_kernel(
+    const char *__restrict__ arena, const HmvWork *__restrict__ works,
+    const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ codes,
+    const double *__restrict__ t_in, const int *__restrict__ outidx, double *__restrict__ t_out,
+    double *__restrict__ y, double beta, const int *__restrict__ skip,
+    const int *__restrict__ winst, const int *__restrict__ inst_done) {
+  if (skip && *skip) return;
+  // a finished instance of a multi-instance solve needs no more products
+  if (inst_done && inst_done[winst[blockIdx.x]]) return;
+  __shared__ __attribute__((aligned(16))) Acc z[HMV_ZCHUNK * NT / HMV_WG_THREADS + 4];
+  stream_item<Acc, PHASE, DT0, DT1, NT>(works[blockIdx.x], z, arena, zsegs, x, codes, t_in, outidx,
+                                        t_out, y, beta);
 }
 
 // t[slot] = sum of the chunk partials t[base + p], p = 0 .. np-1, in order.

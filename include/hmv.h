@@ -198,6 +198,18 @@ int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, d
 int hmv_bicgstab_ru(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
                     int maxit, int check_every, double delta, hmv_solve_info *info);
 
+/* GCR(m) (HACApK_gcrm_lfmtx, m_HACApK_solve.f90:506-571) on device vectors
+ * u (initial guess in, solution out), b: m search directions kept, restart
+ * every m steps with the reference's own restart step (the oldest slot is
+ * overwritten by the new direction before the projections, so its
+ * coefficient scales that direction).  The reference reads m from
+ * param(87), default 8.  Stops when ||r||/||b|| < eps or after maxit steps;
+ * hist (host, may be NULL) receives maxit+1 relative residuals (NaN after
+ * the last step); check_every as for hmv_bicgstab. */
+#define HMV_GCR_MAX_M 32
+int hmv_gcrm(hmv_arena a, const double *b, double *u, double eps, int maxit, int m,
+             int check_every, double *hist, hmv_solve_info *info);
+
 /* Block BiCGSTAB: k independent solves A u_j = b_j (k a power of two,
  * 1..128) sharing one H-matmat per step; b, u device arrays N x k
  * row-interleaved (u holds the initial guesses).  Column j stops at

@@ -12,6 +12,7 @@
  *                            the per-leaf GEMVs of HACApK_MAGMA.c:114-199 (CPU BLAS path)
  *   oracle_setcutthread      m_HACApK_base.f90:584-611 (HACApK_setcutthread, ktp=0)
  *   oracle_bicgstab          m_HACApK_solve.f90:342-449 (HACApK_bicgstab_cax_lfmtx_hyp)
+ *   oracle_gcrm              m_HACApK_solve.f90:506-571 (HACApK_gcrm_lfmtx)
  *   oracle_dotp              HACApK_lib.f90:26-29 (HACApK_dotp_d)
  *   oracle_dense             brute-force assembly of H from its leaves (for checks)
  *
@@ -23,6 +24,10 @@
  *     N=10,400 (ivb/job_cpu.sh.o78632:29-37), against the iterates of scipy's
  *     own BiCGSTAB (same method and shadow residual) on a small H-matrix, and
  *     the true-residual invariant on small dense systems.
+ *   - gcrm: within one restart cycle GCR minimises ||r|| over the Krylov space,
+ *     so its residual norms equal GMRES's (scipy) to rounding; convergence to the
+ *     dense solution.  The restart step is the reference's own (see the
+ *     function); no reference run covers it: parity of that step unpinned.
  *   - setcutthread: leaf ranges tile 1..nlf (invariant test).
  */
 #include <math.h>
@@ -257,6 +262,72 @@ done:
   free(zakp);
   free(zkt);
   free(zakt);
+  return nstp;
+}
+
+/* HACApK_gcrm_lfmtx (m_HACApK_solve.f90:506-571), single rank: GCR(m) with
+ * mreset = m.  Follows the reference literally, including its restart step:
+ * when the new direction goes to slot 1 (in a multiple of m), slot 1's
+ * A-direction has already been overwritten by A r, so the loop over slots
+ * 1..m projects A r on itself first (beta_1 = -(Ar,Ar)/capap(1)) and then on
+ * the old slots 2..m.  u: initial guess in / solution out; hist[0..mstep]
+ * receives ||r||/||b||.  Returns nstp (the reference's in at exit). */
+int oracle_gcrm(const void *leaves, int nlf, int stride, int nd, double *u, const double *b,
+                int mstep, int mreset, double eps, int nthr, double *hist) {
+  if (mreset < 1) return -1;
+  double *zr = (double *)malloc(sizeof(double) * nd);
+  double *zar = (double *)malloc(sizeof(double) * nd);
+  double *zp = (double *)malloc(sizeof(double) * (size_t)nd * mreset);
+  double *zap = (double *)malloc(sizeof(double) * (size_t)nd * mreset);
+  double *capap = (double *)malloc(sizeof(double) * mreset);
+  int nstp = 0;
+  if (!zr || !zar || !zp || !zap || !capap) {
+    nstp = -1;
+    goto done;
+  }
+  {
+    double bnorm = sqrt(oracle_dotp(nd, b, b));
+    oracle_adot_lfmtx_hyp(zar, leaves, nlf, stride, nd, u, nthr);
+    for (int i = 0; i < nd; i++) zr[i] = b[i] - zar[i];
+    for (int i = 0; i < nd; i++) zp[i] = zr[i];  /* zp(:,1) */
+    double zrnorm2 = oracle_dotp(nd, zr, zr);
+    double zrnorm = sqrt(zrnorm2);
+    if (hist) hist[0] = zrnorm / bnorm;
+    if (zrnorm / bnorm < eps) goto done;
+    oracle_adot_lfmtx_hyp(zap, leaves, nlf, stride, nd, zp, nthr); /* zap(:,1) */
+    int in;
+    for (in = 1; in <= mstep; in++) {
+      const int ik = (in - 1) % mreset; /* 0-based slot of zp(:,ik) */
+      double *zq = zap + (size_t)ik * nd;
+      double znom = oracle_dotp(nd, zq, zr);
+      capap[ik] = oracle_dotp(nd, zq, zq);
+      double alpha = znom / capap[ik];
+      for (int i = 0; i < nd; i++) u[i] = u[i] + alpha * zp[(size_t)ik * nd + i];
+      for (int i = 0; i < nd; i++) zr[i] = zr[i] - alpha * zq[i];
+      zrnorm2 = oracle_dotp(nd, zr, zr);
+      zrnorm = sqrt(zrnorm2);
+      if (hist) hist[in] = zrnorm / bnorm;
+      nstp = in;
+      if (zrnorm / bnorm < eps || in == mstep) break;
+      oracle_adot_lfmtx_hyp(zar, leaves, nlf, stride, nd, zr, nthr);
+      const int ikn = in % mreset; /* 0-based slot of zp(:,ikn) */
+      double *pn = zp + (size_t)ikn * nd, *apn = zap + (size_t)ikn * nd;
+      for (int i = 0; i < nd; i++) pn[i] = zr[i];
+      for (int i = 0; i < nd; i++) apn[i] = zar[i];
+      for (int il = 0; il <= ik; il++) {
+        const double *q = zap + (size_t)il * nd, *pl = zp + (size_t)il * nd;
+        double beta = -oracle_dotp(nd, q, zar) / capap[il];
+        for (int i = 0; i < nd; i++) pn[i] = pn[i] + beta * pl[i];
+        for (int i = 0; i < nd; i++) apn[i] = apn[i] + beta * q[i];
+      }
+    }
+  }
+done:
+  free(zr);
+  free(zar);
+  free(zp);
+  free(zap);
+  free(capap);
   return nstp;
 }
 

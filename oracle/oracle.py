@@ -29,6 +29,8 @@ def lib():
         L.oracle_setcutthread.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
         L.oracle_bicgstab.argtypes = [vp, C.c_int, C.c_int, C.c_int, _D, _D, C.c_int, C.c_double,
                                       C.c_int, _D]
+        L.oracle_gcrm.argtypes = [vp, C.c_int, C.c_int, C.c_int, _D, _D, C.c_int, C.c_int,
+                                  C.c_double, C.c_int, _D]
         L.oracle_dotp.argtypes = [C.c_int, _D, _D]
         L.oracle_dotp.restype = C.c_double
         L.oracle_dense.argtypes = [vp, C.c_int, C.c_int, C.c_int, _D]
@@ -91,6 +93,16 @@ def bicgstab(hm, b, u0=None, mstep=500, eps=1e-8, nthr=1):
     hist = np.full(mstep + 1, np.nan)
     nstp = lib().oracle_bicgstab(hm.leaves_ptr, hm.nlf, hm.stride, hm.nd, _d(u), _d(b), mstep, eps,
                                  nthr, _d(hist))
+    return u, nstp, hist[: nstp + 1]
+
+
+def gcrm(hm, b, u0=None, mstep=500, mreset=8, eps=1e-8, nthr=1):
+    """m_HACApK_solve.f90:506-571.  Returns (u, nstp, hist)."""
+    b = np.ascontiguousarray(b, np.float64)
+    u = np.zeros(hm.nd) if u0 is None else np.ascontiguousarray(u0, np.float64).copy()
+    hist = np.full(mstep + 1, np.nan)
+    nstp = lib().oracle_gcrm(hm.leaves_ptr, hm.nlf, hm.stride, hm.nd, _d(u), _d(b), mstep, mreset,
+                             eps, nthr, _d(hist))
     return u, nstp, hist[: nstp + 1]
 
 

@@ -84,6 +84,33 @@ def test_bicgstab_matches_scipy_iterates(O, H, sphere3):
     assert np.allclose(np.log10(hist[1:k + 1]), np.log10(res[:k]), atol=1e-6)
 
 
+def test_gcrm_first_cycle_matches_gmres(O, H, sphere3):
+    """Within one restart cycle GCR minimises ||r|| over the same Krylov space
+    as GMRES, so the oracle's residual norms equal scipy's GMRES ones while they
+    are far above rounding (m_HACApK_solve.f90:506-571)."""
+    sla = pytest.importorskip("scipy.sparse.linalg")
+    hm = sphere3
+    A = O.dense(hm)
+    b = H.uniform(6, hm.nd)
+    _, nstp, hist = O.gcrm(hm, b, None, mstep=25, mreset=30, eps=1e-13)
+    res = []
+    sla.gmres(A, b, x0=np.zeros(hm.nd), rtol=1e-14, atol=0.0, restart=30, maxiter=1,
+              callback=lambda r: res.append(r), callback_type="pr_norm")
+    k = min(20, len(res), nstp)
+    assert k >= 15
+    assert np.allclose(np.log10(hist[1:k + 1]), np.log10(res[:k]), atol=1e-8)
+
+
+def test_gcrm_converges_to_dense_solution(O, H, sphere3):
+    hm = sphere3
+    A = O.dense(hm)
+    b = H.uniform(7, hm.nd)
+    u, nstp, hist = O.gcrm(hm, b, None, mstep=500, mreset=8, eps=1e-10)
+    assert hist[-1] < 1e-10 and nstp < 500
+    assert np.linalg.norm(A @ u - b) / np.linalg.norm(b) < 1e-9
+    assert np.linalg.norm(u - np.linalg.solve(A, b)) / np.linalg.norm(u) < 1e-7
+
+
 def test_bicgstab_small_dense_terminates(O, H):
     """On a 6x6 system BiCGSTAB reaches rounding level in at most 6 steps; the
     system is passed to the oracle as one dense leaf."""
