@@ -118,7 +118,7 @@ class Tuning(C.Structure):
 WORK_DTYPE = np.dtype([("off0", "<i8"), ("off1", "<i8"), ("rows", "<i4"), ("rows_p", "<i4"),
                        ("ncol0", "<i4"), ("ncol1", "<i4"), ("dt0", "<i2"), ("dt1", "<i2"),
                        ("zseg_begin", "<i4"), ("zseg_end", "<i4"), ("out_base", "<i4")])
-ZSEG_DTYPE = np.dtype([("kind", "<i4"), ("ncols", "<i4"), ("src", "<i4"), ("pad", "<i4")])
+ZSEG_DTYPE = np.dtype([("kind", "<i4"), ("ncols", "<i4"), ("src", "<i4"), ("col", "<i4")])
 TFIN_DTYPE = np.dtype([("slot", "<i4"), ("base", "<i4"), ("np", "<i4"), ("pad", "<i4")])
 
 

@@ -46,10 +46,13 @@
 #define HMV_ZCHUNK 2048       // inputs staged in LDS per pass
 #define HMV_ARENA_ALIGN 256   // byte alignment of every block in the arena
 
-// Sources of staged inputs.
-#define HMV_Z_X 0      // z[c] = x[src + c]
+// Sources of staged inputs.  A work item's inputs are either runs (kinds X
+// and T, at most HMV_RUN_MAX of them, in column order) or one CODES segment.
+#define HMV_Z_X 0      // z[col + c] = x[src + c]
 #define HMV_Z_CODES 1  // z[c] from code = codes[src + c]:
                        //   code >= 0: x[code];  code < 0: t[-code - 1]
+#define HMV_Z_T 2      // z[col + c] = t[src + c]
+#define HMV_RUN_MAX 64
 #define HMV_MAX_TPARTS 32  // t partials per rank vector (column chunks of a group)
 
 // One workgroup's block.
@@ -101,7 +104,7 @@ struct HmvZSeg {
   int kind;   // HMV_Z_*
   int ncols;
   int src;
-  int pad;
+  int col;    // first column of the run within its work item
 };
 
 #endif

@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include "hmv_engine.h"
+#include "hmv_inputs.h"
 #include "hmv_internal.h"
 
 namespace {
@@ -36,88 +37,93 @@ __device__ inline __amdgpu_buffer_rsrc_t part_rsrc(const char *base, uint32_t nb
   return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)nbytes, 0x00020000);
 }
 
-// Per-lane element of one column part: VR consecutive rows of one column.
+// Bytes per stored entry, and rows per lane of a block whose first part is
+// stored as DT0 (the second part uses the same rows): 8-byte lane loads for
+// FP64 entries, FP32 pairs and BF16 quads (phase 1 of BF16 arenas); the
+// BF16 second part of a phase-2 block of a BF16 arena loads 4-byte pairs.
 template <int DT>
+constexpr int dt_bytes() {
+  return DT == HMV_DT_F64 ? 8 : DT == HMV_DT_F32 ? 4 : 2;
+}
+template <int DT0>
+constexpr int rows_per_lane() {
+  return DT0 == HMV_DT_F64 ? 1 : DT0 == HMV_DT_F32 ? 2 : 4;
+}
+
+typedef uint32_t uint2_nt __attribute__((ext_vector_type(2)));
+
+// Per-lane element of one column part: VR consecutive rows of one column,
+// accumulated into a[0..VR).
+template <int DT, int VR>
 struct Lane;
 template <>
-struct Lane<HMV_DT_F64> {
+struct Lane<HMV_DT_F64, 1> {
   using raw = double;
-  static constexpr int bytes = 8;
   __device__ static raw load(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kBufNT));
   }
   template <typename Acc>
-  __device__ static void fma(raw v, Acc z, Acc &a0, Acc &) {
-    a0 = __builtin_fma((Acc)v, z, a0);
+  __device__ static void fma(raw v, Acc z, Acc (&a)[1]) {
+    a[0] = __builtin_fma((Acc)v, z, a[0]);
   }
 };
 template <>
-struct Lane<HMV_DT_F32> {
+struct Lane<HMV_DT_F32, 2> {
   using raw = float2_nt;
-  static constexpr int bytes = 8;
   __device__ static raw load(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
     return __builtin_bit_cast(float2_nt, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kBufNT));
   }
   template <typename Acc>
-  __device__ static void fma(raw v, Acc z, Acc &a0, Acc &a1) {
-    a0 = __builtin_fma((Acc)v.x, z, a0);
-    a1 = __builtin_fma((Acc)v.y, z, a1);
+  __device__ static void fma(raw v, Acc z, Acc (&a)[2]) {
+    a[0] = __builtin_fma((Acc)v.x, z, a[0]);
+    a[1] = __builtin_fma((Acc)v.y, z, a[1]);
   }
 };
 template <>
-struct Lane<HMV_DT_BF16> {
+struct Lane<HMV_DT_BF16, 2> {
   using raw = uint32_t;
-  static constexpr int bytes = 4;
   __device__ static raw load(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
     return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, kBufNT);
   }
   template <typename Acc>
-  __device__ static void fma(raw v, Acc z, Acc &a0, Acc &a1) {
-    a0 = __builtin_fma((Acc)bf16_lo(v), z, a0);
-    a1 = __builtin_fma((Acc)bf16_hi(v), z, a1);
+  __device__ static void fma(raw v, Acc z, Acc (&a)[2]) {
+    a[0] = __builtin_fma((Acc)bf16_lo(v), z, a[0]);
+    a[1] = __builtin_fma((Acc)bf16_hi(v), z, a[1]);
+  }
+};
+template <>
+struct Lane<HMV_DT_BF16, 4> {
+  using raw = uint2_nt;
+  __device__ static raw load(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff) {
+    return __builtin_bit_cast(uint2_nt, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kBufNT));
+  }
+  template <typename Acc>
+  __device__ static void fma(raw v, Acc z, Acc (&a)[4]) {
+    a[0] = __builtin_fma((Acc)bf16_lo(v.x), z, a[0]);
+    a[1] = __builtin_fma((Acc)bf16_hi(v.x), z, a[1]);
+    a[2] = __builtin_fma((Acc)bf16_lo(v.y), z, a[2]);
+    a[3] = __builtin_fma((Acc)bf16_hi(v.y), z, a[3]);
   }
 };
 
-// Stages inputs of columns [c0, c0+cw) into z[0, cw).  Threads take
-// columns c0+tid, c0+tid+256, ... in batches of K: all K index loads, then
-// all K value loads are issued before any is waited for, so a batch costs
-// two memory round trips.
+// Stages inputs of item columns [c0, c0+cw) into z[0, cw).  Threads take
+// columns c0+tid, c0+tid+NT, ... in batches of K: all K codes, then all K
+// values are fetched before any is waited for.
 template <typename Acc, int NT>
-__device__ inline void stage(Acc *__restrict__ z, const HmvZSeg *__restrict__ zsegs,
-                             const HmvWork &w, int c0, int cw, const double *__restrict__ x,
-                             const int *__restrict__ codes, const double *__restrict__ t) {
+__device__ inline void stage(Acc *__restrict__ z, const Inputs &in, int c0, int cw,
+                             const double *__restrict__ x, const int *__restrict__ codes,
+                             const double *__restrict__ t) {
   constexpr int K = 4;
-  const int tid = threadIdx.x;
-  int col = 0;
-  for (int s = w.zseg_begin; s < w.zseg_end && col < c0 + cw; ++s) {
-    const HmvZSeg zs = zsegs[s];
-    const int lo = max(col, c0), hi = min(col + zs.ncols, c0 + cw);
-    for (int cb = lo + tid; cb < hi; cb += K * NT) {
-      const double *src[K];
-      if (zs.kind == HMV_Z_X) {
+  for (int cb = threadIdx.x; cb < cw; cb += K * NT) {
+    int code[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-          src[k] = x + zs.src + (min(cb + k * NT, hi - 1) - col);
-      } else {
-        int code[K];
+    for (int k = 0; k < K; ++k) code[k] = input_code(in, codes, c0 + min(cb + k * NT, cw - 1));
+    double val[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const int c = cb + k * NT;
-          code[k] = c < hi ? codes[zs.src + (c - col)] : 0;
-        }
+    for (int k = 0; k < K; ++k) val[k] = code[k] >= 0 ? x[code[k]] : t[-code[k] - 1];
 #pragma unroll
-        for (int k = 0; k < K; ++k) src[k] = code[k] >= 0 ? x + code[k] : t + (-code[k] - 1);
-      }
-      double val[K];
-#pragma unroll
-      for (int k = 0; k < K; ++k) val[k] = *src[k];
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int c = cb + k * NT;
-        if (c < hi) z[c - c0] = (Acc)val[k];
-      }
-    }
-    col += zs.ncols;
+    for (int k = 0; k < K; ++k)
+      if (cb + k * NT < cw) z[cb + k * NT] = (Acc)val[k];
   }
 }
 
@@ -126,63 +132,57 @@ __device__ inline void stage(Acc *__restrict__ z, const HmvZSeg *__restrict__ zs
 // z[zb + j*m].  Loads go in batches of U issued together; the last, partial
 // batch gives its missing columns the out-of-range offset (they load zero)
 // and the zero slot z[zmax] as input.
-template <typename Acc, int DT, int U>
+template <typename Acc, int DT, int VR, int U>
 __device__ inline void accumulate(__amdgpu_buffer_rsrc_t rs, uint32_t vb, int step, int n,
-                                  const Acc *__restrict__ z, int zb, int m, int zmax, Acc &a0,
-                                  Acc &a1) {
-  using LN = Lane<DT>;
+                                  const Acc *__restrict__ z, int zb, int m, int zmax,
+                                  Acc (&a)[VR]) {
+  using LN = Lane<DT, VR>;
   int j = 0;
   for (; j + U <= n; j += U) {
     typename LN::raw v[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) v[k] = LN::load(rs, vb, (j + k) * step);
 #pragma unroll
-    for (int k = 0; k < U; ++k) LN::fma(v[k], z[zb + (j + k) * m], a0, a1);
+    for (int k = 0; k < U; ++k) LN::fma(v[k], z[zb + (j + k) * m], a);
   }
   if (j < n) {
     typename LN::raw v[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) v[k] = LN::load(rs, j + k < n ? vb : kOffNone, (j + k) * step);
 #pragma unroll
-    for (int k = 0; k < U; ++k) LN::fma(v[k], z[min(zb + (j + k) * m, zmax)], a0, a1);
+    for (int k = 0; k < U; ++k) LN::fma(v[k], z[min(zb + (j + k) * m, zmax)], a);
   }
 }
 
 // One column part of a block: buffer resource, the lane's byte offset of its
 // first column (column cg), and the byte stride of one step of m columns.
-template <int DT>
+template <int DT, int VR>
 struct Part {
-  static constexpr int es = Lane<DT>::bytes / ((DT == HMV_DT_F64) ? 1 : 2);  // bytes per entry
+  static constexpr int es = dt_bytes<DT>();
   __amdgpu_buffer_rsrc_t rs;
   uint32_t voff;
   int step;
   __device__ Part(const char *base, int rows_p, int ncols, int rq, int cg, int m) {
-    constexpr int vr = (DT == HMV_DT_F64) ? 1 : 2;
     rs = part_rsrc(base, (uint32_t)((size_t)rows_p * ncols * es));
-    voff = (uint32_t)(rq * vr + cg * rows_p) * (uint32_t)es;
+    voff = (uint32_t)(rq * VR + cg * rows_p) * (uint32_t)es;
     step = m * rows_p * es;
   }
 };
 
+// Loads per batch (32 bytes in flight per lane for 8-byte lane elements).
+constexpr int kBatch = 4;
+
 // Lane's columns of local range [lo, hi) of a part: local column cg + j*m,
 // j in [ceil((lo-cg)/m), ceil((hi-cg)/m)); zshift maps a local column to its
 // staged slot (slot = local column + zshift).
-// Loads per batch.  8-byte lane elements: 4 (32 bytes in flight per lane).
-// 4-byte BF16 pairs: 6 in a one-part block (phase 1); in the two-part phase-2
-// block of BF16 arenas only 4 fit 64 VGPRs without spilling.
-template <int DT, bool TWO_PARTS>
-constexpr int batch_of() {
-  return (Lane<DT>::bytes == 8 || TWO_PARTS) ? 4 : 6;
-}
-
-template <typename Acc, int DT, int U>
-__device__ inline void run_cols(const Part<DT> &P, int lo, int hi, int cg, int m, int zshift,
-                                int zmax, const Acc *__restrict__ z, Acc &a0, Acc &a1) {
+template <typename Acc, int DT, int VR>
+__device__ inline void run_cols(const Part<DT, VR> &P, int lo, int hi, int cg, int m, int zshift,
+                                int zmax, const Acc *__restrict__ z, Acc (&a)[VR]) {
   const int j0 = lo > cg ? (lo - cg + m - 1) / m : 0;
   const int j1 = hi > cg ? (hi - cg + m - 1) / m : 0;
   if (j1 > j0)
-    accumulate<Acc, DT, U>(P.rs, P.voff + (uint32_t)(j0 * P.step), P.step, j1 - j0, z,
-                           cg + j0 * m + zshift, m, zmax, a0, a1);
+    accumulate<Acc, DT, VR, kBatch>(P.rs, P.voff + (uint32_t)(j0 * P.step), P.step, j1 - j0, z,
+                                    cg + j0 * m + zshift, m, zmax, a);
 }
 
 // PHASE 1: t[outidx[out_base + r]] = row sums.
@@ -193,60 +193,62 @@ __device__ inline void run_cols(const Part<DT> &P, int lo, int hi, int cg, int m
 // chunk plus the zero slot; reused for the row reduction).
 template <typename Acc, int PHASE, int DT0, int DT1, int NT>
 __device__ inline void stream_item(
-    const HmvWork &w, Acc *__restrict__ z, const char *__restrict__ arena,
+    const HmvWork &w, Acc *__restrict__ z, int *__restrict__ runs, const char *__restrict__ arena,
     const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ codes,
     const double *__restrict__ t_in, const int *__restrict__ outidx, double *__restrict__ t_out,
     double *__restrict__ y, double beta) {
   constexpr int ZCH = HMV_ZCHUNK * NT / HMV_WG_THREADS;
-  static_assert(2 * NT <= ZCH, "reduction buffer fits in z");
   const int tid = threadIdx.x;
-  constexpr int vr = (DT0 == HMV_DT_F64) ? 1 : 2;
+  constexpr int vr = rows_per_lane<DT0>();
+  static_assert(vr * NT <= ZCH, "reduction buffer fits in z");
   const int q = w.rows_p / vr;
   const int m = NT / q;
   const bool active = tid < q * m;
   const int rq = tid % q, cg = tid / q;
-  Acc a0 = Acc(0), a1 = Acc(0);
+  Acc a[vr];
+#pragma unroll
+  for (int i = 0; i < vr; ++i) a[i] = Acc(0);
   const int ncols = w.ncol0 + w.ncol1;
-  const Part<DT0> P0(arena + w.off0, w.rows_p, w.ncol0, rq, cg, m);
+  const Part<DT0, vr> P0(arena + w.off0, w.rows_p, w.ncol0, rq, cg, m);
   // The lane's first PF columns of part 0 do not depend on the staged
   // inputs: their loads are issued before staging, so the first HBM round
   // trip overlaps the gathers and the barrier.
-  constexpr int PF = 4;
+  constexpr int PF = vr == 4 ? 2 : 4;  // 4-row lanes: two, so that nothing spills
   const int lim0 = min(w.ncol0, ZCH);
   const int npf = active ? max(0, min(PF, (lim0 - cg + m - 1) / m)) : 0;
-  typename Lane<DT0>::raw pre[PF];
+  using L0 = Lane<DT0, vr>;
+  typename L0::raw pre[PF];
 #pragma unroll
-  for (int k = 0; k < PF; ++k) pre[k] = Lane<DT0>::load(P0.rs, k < npf ? P0.voff : kOffNone, k * P0.step);
+  for (int k = 0; k < PF; ++k) pre[k] = L0::load(P0.rs, k < npf ? P0.voff : kOffNone, k * P0.step);
+  const Inputs in = load_inputs<NT>(w, zsegs, runs);
   for (int c0 = 0; c0 < ncols; c0 += ZCH) {
     const int cw = min(ZCH, ncols - c0);
     if (c0 > 0) __syncthreads();
-    stage<Acc, NT>(z, zsegs, w, c0, cw, x, codes, t_in);
+    stage<Acc, NT>(z, in, c0, cw, x, codes, t_in);
     if (tid == 0) z[cw] = Acc(0);
     __syncthreads();
     if (active) {
       int lo0 = c0;
       if (c0 == 0) {
 #pragma unroll
-        for (int k = 0; k < PF; ++k) Lane<DT0>::fma(pre[k], z[min(cg + k * m, cw)], a0, a1);
+        for (int k = 0; k < PF; ++k) L0::fma(pre[k], z[min(cg + k * m, cw)], a);
         lo0 = min(npf * m, w.ncol0);  // columns < npf*m are done (for this lane: j < npf)
         lo0 = npf == PF ? lo0 : cw;   // a lane with fewer than PF columns here has none left
       }
       const int hi0 = min(c0 + cw, w.ncol0);
-      constexpr int U0 = batch_of<DT0, (DT0 != DT1)>();
-      if (lo0 < hi0) run_cols<Acc, DT0, U0>(P0, lo0, hi0, cg, m, -c0, cw, z, a0, a1);
+      if (lo0 < hi0) run_cols<Acc, DT0, vr>(P0, lo0, hi0, cg, m, -c0, cw, z, a);
       if (w.ncol1 > 0 && c0 + cw > w.ncol0) {
-        const Part<DT1> P1(arena + w.off1, w.rows_p, w.ncol1, rq, cg, m);
-        constexpr int U1 = batch_of<DT1, (DT0 != DT1)>();
-        run_cols<Acc, DT1, U1>(P1, max(c0, w.ncol0) - w.ncol0, c0 + cw - w.ncol0, cg, m,
-                               w.ncol0 - c0, cw, z, a0, a1);
+        const Part<DT1, vr> P1(arena + w.off1, w.rows_p, w.ncol1, rq, cg, m);
+        run_cols<Acc, DT1, vr>(P1, max(c0, w.ncol0) - w.ncol0, c0 + cw - w.ncol0, cg, m,
+                               w.ncol0 - c0, cw, z, a);
       }
     }
   }
   Acc *red = z;
   __syncthreads();
   if (active) {
-    red[cg * w.rows_p + rq * vr] = a0;
-    if constexpr (vr == 2) red[cg * w.rows_p + rq * vr + 1] = a1;
+#pragma unroll
+    for (int i = 0; i < vr; ++i) red[cg * w.rows_p + rq * vr + i] = a[i];
   }
   __syncthreads();
   for (int r = tid; r < w.rows; r += NT) {
@@ -274,8 +276,9 @@ hmv_stream_kernel(
   // a finished instance of a multi-instance solve needs no more products
   if (inst_done && inst_done[winst[blockIdx.x]]) return;
   __shared__ __attribute__((aligned(16))) Acc z[HMV_ZCHUNK * NT / HMV_WG_THREADS + 4];
-  stream_item<Acc, PHASE, DT0, DT1, NT>(works[blockIdx.x], z, arena, zsegs, x, codes, t_in, outidx,
-                                        t_out, y, beta);
+  __shared__ int runs[2 * HMV_RUN_MAX];
+  stream_item<Acc, PHASE, DT0, DT1, NT>(works[blockIdx.x], z, runs, arena, zsegs, x, codes, t_in,
+                                        outidx, t_out, y, beta);
 }
 
 // t[slot] = sum of the chunk partials t[base + p], p = 0 .. np-1, in order.

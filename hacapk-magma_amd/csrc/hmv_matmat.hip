@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include "hmv_engine.h"
+#include "hmv_inputs.h"
 #include "hmv_internal.h"
 
 namespace {
@@ -81,24 +82,19 @@ struct Entry<HMV_DT_BF16> {
 };
 
 // Stages z[cc][j] = input j of column c0+cc (cc < cw, j < KP; 0 for j >= k),
-// and the zero row z[cw][*].  One source per work item: an x segment
-// (phase 1) or a code table (phase 2: x index or t slot).  The chunk's column
-// sources are resolved once into `src` (LDS; element offsets, t slots
-// flagged negative), then the wave gathers SB inputs per lane per round trip.
+// and the zero row z[cw][*].  The chunk's column sources (x index or t slot
+// of the item's runs or code table) are resolved once into `src` (LDS;
+// element offsets, t slots flagged negative), then the wave gathers SB
+// inputs per lane per round trip.
 template <typename Acc, int KP>
-__device__ inline void stage(Acc *__restrict__ z, long long *__restrict__ src, const HmvZSeg &zs,
+__device__ inline void stage(Acc *__restrict__ z, long long *__restrict__ src, const Inputs &in,
                              int c0, int cw, int k, const double *__restrict__ x,
                              const int *__restrict__ codes, const double *__restrict__ t) {
   constexpr int SB = 16;
   const int lane = threadIdx.x;
   for (int cc = lane; cc < cw; cc += 64) {
-    const int c = c0 + cc;
-    if (zs.kind == HMV_Z_X) {
-      src[cc] = (long long)(zs.src + c) * k;
-    } else {
-      const int code = codes[zs.src + c];
-      src[cc] = code >= 0 ? (long long)code * k : -1 - (long long)(-code - 1) * k;
-    }
+    const int code = input_code(in, codes, c0 + cc);
+    src[cc] = code >= 0 ? (long long)code * k : -1 - (long long)(-code - 1) * k;
   }
   __syncthreads();
   const int n = (cw + 1) * KP;
@@ -172,12 +168,13 @@ __global__ __launch_bounds__(64) void hmv_matmat_kernel(
   static_assert(CH >= 4, "chunk holds at least one column quad");
   __shared__ __attribute__((aligned(16))) Acc z[(CH + 1) * KP];
   __shared__ long long zsrc[CH];
+  __shared__ int runs[2 * HMV_RUN_MAX];
   const int2 tile = tiles[blockIdx.x];
   const HmvWork w = works[tile.x];
   const int lane = threadIdx.x;
   const int r = tile.y * 16 + (lane & 15);  // this lane's A row
   const int ncols = w.ncol0 + w.ncol1;
-  const HmvZSeg zs = ncols > 0 ? zsegs[w.zseg_begin] : HmvZSeg{0, 0, 0, 0};
+  const Inputs in = load_inputs<64>(w, zsegs, runs);
   const __amdgpu_buffer_rsrc_t rs0 =
       rsrc(arena + w.off0, (uint32_t)((size_t)w.rows_p * w.ncol0 * Entry<DT0>::es));
   const __amdgpu_buffer_rsrc_t rs1 =
@@ -192,7 +189,7 @@ __global__ __launch_bounds__(64) void hmv_matmat_kernel(
   for (int c0 = 0; c0 < ncols; c0 += CH) {
     const int cw = min(CH, ncols - c0);
     if (c0 > 0) __syncthreads();
-    stage<Acc, KP>(z, zsrc, zs, c0, cw, k, x, codes, t_in);
+    stage<Acc, KP>(z, zsrc, in, c0, cw, k, x, codes, t_in);
     __syncthreads();
     if (c0 < w.ncol0)
       part_mma<Acc, DT0, KT>(acc, z, rs0, w.rows_p, r, w.ncol0, c0, min(c0 + cw, w.ncol0), c0, cw);

@@ -22,7 +22,7 @@
 namespace hmv {
 
 int dtype_size(int dt) { return dt == HMV_DT_F64 ? 8 : dt == HMV_DT_F32 ? 4 : 2; }
-int row_vector(int dt) { return dt == HMV_DT_F64 ? 1 : 2; }
+int row_vector(int dt) { return dt == HMV_DT_F64 ? 1 : dt == HMV_DT_F32 ? 2 : 4; }
 
 namespace {
 
@@ -128,7 +128,9 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   const int dt_lr = prec == kF64 ? HMV_DT_F64 : prec == kF32 ? HMV_DT_F32 : HMV_DT_BF16;
   const bool merged = dt_dense == dt_lr;
   const int es_d = dtype_size(dt_dense), es_l = dtype_size(dt_lr);
-  const int vr = row_vector(dt_lr);  // same row-vector width for both dtypes of an arena
+  // rows per lane: phase-1 blocks hold only low-rank factors (dt_lr), phase-2
+  // blocks follow their first part's type (dt_dense)
+  const int vr = row_vector(dt_lr), vr2 = row_vector(dt_dense);
 
   std::vector<const RefLeaf *> L(nlf);
   PlanStats &st = P.st;
@@ -373,7 +375,7 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     st.phase1_bytes += p.bytes;
   }
   for (auto &S : slices) {
-    const int64_t rows_p = round_up(S.b - S.a, vr);
+    const int64_t rows_p = round_up(S.b - S.a, vr2);
     if (S.ncol0 > 0) {
       S.off0 = off;
       S.bytes += rows_p * S.ncol0 * es_d;
@@ -439,35 +441,55 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     w.off0 = S.off0;
     w.off1 = S.off1;
     w.rows = S.b - S.a;
-    w.rows_p = (int)round_up(S.b - S.a, vr);
+    w.rows_p = (int)round_up(S.b - S.a, vr2);
     w.ncol0 = S.ncol0;
     w.ncol1 = S.ncol1;
     w.dt0 = (short)dt_dense;
     w.dt1 = (short)dt_lr;
     w.zseg_begin = (int)P.zseg.size();
-    const int64_t cbase = (int64_t)P.codes.size();
-    auto dense_codes = [&](const RowGroup &g) {
-      for (int i : g.dense)
-        for (int j = 0; j < L[i]->ndt; ++j) P.codes.push_back(L[i]->nstrtt - 1 + j);
+    // the block's inputs as runs of consecutive x entries (a dense leaf's
+    // columns) and t slots (a row group's rank vectors), joined when they
+    // continue each other
+    std::vector<HmvZSeg> runs;
+    int col = 0;
+    auto add_run = [&](int kind, int64_t src, int n) {
+      if (n <= 0) return;
+      if (!runs.empty() && runs.back().kind == kind && runs.back().src + runs.back().ncols == src)
+        runs.back().ncols += n;
+      else
+        runs.push_back(HmvZSeg{kind, n, (int)src, col});
+      col += n;
     };
-    auto t_codes = [&](const RowGroup &g) {
-      for (int64_t k = 0; k < g.k; ++k) P.codes.push_back(tcode(g.tbase + k));
+    auto dense_runs = [&](const RowGroup &g) {
+      for (int i : g.dense) add_run(HMV_Z_X, L[i]->nstrtt - 1, L[i]->ndt);
     };
+    auto t_runs = [&](const RowGroup &g) { add_run(HMV_Z_T, g.tbase, (int)g.k); };
     if (merged) {
       for (int gi : S.cov) {
-        dense_codes(rg[gi]);
-        t_codes(rg[gi]);
+        dense_runs(rg[gi]);
+        t_runs(rg[gi]);
       }
     } else {
-      for (int gi : S.cov) dense_codes(rg[gi]);
-      for (int gi : S.cov) t_codes(rg[gi]);
+      for (int gi : S.cov) dense_runs(rg[gi]);
+      for (int gi : S.cov) t_runs(rg[gi]);
     }
-    if (S.ncol0 + S.ncol1 > 0) {
-      if ((int64_t)P.codes.size() - cbase != S.ncol0 + S.ncol1 || cbase > INT32_MAX) {
-        err = "internal: code table does not match the block columns";
+    if (col != S.ncol0 + S.ncol1) {
+      err = "internal: input runs do not match the block columns";
+      return false;
+    }
+    if ((int)runs.size() <= HMV_RUN_MAX) {
+      P.zseg.insert(P.zseg.end(), runs.begin(), runs.end());
+    } else {
+      // too many runs for the workgroup's run table: one code per column
+      const int64_t cbase = (int64_t)P.codes.size();
+      if (cbase + col > INT32_MAX) {
+        err = "too many input codes for 32-bit indexing";
         return false;
       }
-      P.zseg.push_back(HmvZSeg{HMV_Z_CODES, S.ncol0 + S.ncol1, (int)cbase, 0});
+      for (const HmvZSeg &r : runs)
+        for (int c = 0; c < r.ncols; ++c)
+          P.codes.push_back(r.kind == HMV_Z_X ? r.src + c : tcode((int64_t)r.src + c));
+      P.zseg.push_back(HmvZSeg{HMV_Z_CODES, col, (int)cbase, 0});
     }
     w.zseg_end = (int)P.zseg.size();
     w.out_base = S.a;
@@ -491,7 +513,7 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   parallel_for((int64_t)slices.size(), [&](int64_t si) {
     const Slice &S = slices[si];
     const int rows = S.b - S.a;
-    const int64_t rows_p = round_up(rows, vr);
+    const int64_t rows_p = round_up(rows, vr2);
     if (S.off0 >= 0)
       std::memset(A + S.off0, 0, (size_t)(round_up(S.off0 + rows_p * S.ncol0 * es_d, HMV_ARENA_ALIGN) - S.off0));
     if (S.off1 >= 0)
@@ -561,13 +583,13 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   };
   // Short blocks (at most nt lanes per column) first, in their own dispatch
   // order, when the phase runs them in nt-thread workgroups.
-  auto order_split = [&](std::vector<Item> &items, int nt, std::vector<HmvWork> &out, int &nsmall,
-                         int &nt_small) {
+  auto order_split = [&](std::vector<Item> &items, int nt, int vrw, std::vector<HmvWork> &out,
+                         int &nsmall, int &nt_small) {
     nsmall = 0;
     nt_small = 0;
     if (nt == 64 || nt == 128) {
       std::vector<Item> small, large;
-      for (auto &it : items) (it.w.rows_p / vr <= nt ? small : large).push_back(it);
+      for (auto &it : items) (it.w.rows_p / vrw <= nt ? small : large).push_back(it);
       if (!small.empty()) {
         std::vector<HmvWork> a, b;
         order(small, a);
@@ -581,8 +603,8 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
     }
     order(items, out);
   };
-  order_split(it1, tune.p1_small_threads, P.work1, P.nwork1_small, P.nt1_small);
-  order_split(it2, tune.p2_small_threads, P.work2, P.nwork2_small, P.nt2_small);
+  order_split(it1, tune.p1_small_threads, vr, P.work1, P.nwork1_small, P.nt1_small);
+  order_split(it2, tune.p2_small_threads, vr2, P.work2, P.nwork2_small, P.nt2_small);
   lap("order");
   return true;
 }

@@ -12,7 +12,8 @@ ZCHUNK = 2048
 MAX_ROWS = 256
 THREADS = 256
 DT_BYTES = {0: 8, 1: 4, 2: 2}
-Z_X, Z_CODES = 0, 1
+Z_X, Z_CODES, Z_T = 0, 1, 2
+RUN_MAX = 64
 MAX_TPARTS = 32
 
 
@@ -36,15 +37,17 @@ def _part(arena, off, rows_p, ncols, dt):
 def _lane_layout(w, nt=THREADS):
     """Lane layout of a workgroup of nt threads (short blocks may run with 64
     or 128); asserts the block fits it."""
-    vr = 1 if int(w["dt0"]) == 0 else 2
+    # rows per lane follow the first part's storage type; the second part
+    # (if any) is read with the same rows per lane
+    vr = {0: 1, 1: 2, 2: 4}[int(w["dt0"])]
     rows, rows_p = int(w["rows"]), int(w["rows_p"])
     assert 0 < rows <= rows_p <= MAX_ROWS * vr and rows_p % vr == 0 and rows_p - rows < vr
     q = rows_p // vr
     assert 1 <= q <= nt, "more lanes per column than threads"
     m = nt // q
-    assert m * rows_p <= 2 * nt, "reduction buffer overflow"
-    if int(w["ncol1"]) > 0:
-        assert (1 if int(w["dt1"]) == 0 else 2) == vr, "both parts share the row-vector width"
+    assert m * rows_p <= vr * nt <= ZCHUNK * nt // THREADS, "reduction buffer overflow"
+    if int(w["ncol0"]) > 0:
+        assert int(w["off0"]) % (vr * DT_BYTES[int(w["dt0"])]) == 0, "lane loads are aligned"
     return vr, q, m
 
 
@@ -52,13 +55,23 @@ def _stage(tables, w, x, tbuf, tsize, nd):
     zsegs, codes = tables["zseg"], tables["codes"]
     zb, ze = int(w["zseg_begin"]), int(w["zseg_end"])
     assert 0 <= zb <= ze <= len(zsegs)
+    kinds = [int(k) for k in zsegs["kind"][zb:ze]]
+    # runs (x / t ranges) fit the workgroup's run table; a code table comes alone
+    assert (Z_CODES not in kinds and len(kinds) <= RUN_MAX) or kinds == [Z_CODES]
     parts = []
+    col = 0
     for zs in zsegs[zb:ze]:
         kind, n, src = int(zs["kind"]), int(zs["ncols"]), int(zs["src"])
         assert n > 0
+        if kind != Z_CODES:
+            assert int(zs["col"]) == col, "runs are in column order and cover every column"
+        col += n
         if kind == Z_X:
             assert 0 <= src and src + n <= nd
             parts.append(x[src:src + n])
+        elif kind == Z_T:
+            assert 0 <= src and src + n <= tsize, "runs address only final t slots"
+            parts.append(tbuf[src:src + n])
         else:
             assert kind == Z_CODES and 0 <= src and src + n <= codes.size
             out = np.empty(n)

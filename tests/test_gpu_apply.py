@@ -172,6 +172,26 @@ def test_edge_leaves(gpu, O, H):
     a.free()
 
 
+@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", 1e-5), ("bf16", 2e-2)])
+def test_input_runs_and_code_fallback(gpu, O, prec, tol):
+    """Blocks whose inputs come from a tabled run list (31 runs), a code table
+    (101 runs, beyond the table) and single runs, through the H-matvec and the
+    H-matmat (k = 3)."""
+    import synthetic_leaves as SL
+    hm = SL.many_runs(gpu)
+    a = gpu.Arena(hm, prec)
+    x = gpu.uniform(6, hm.nd)
+    y = a.matvec(x)
+    y_ref = O.adot_body_lfmtx(hm, x)
+    print(prec, "apply rel err %.3e" % _rel(y, y_ref), flush=True)
+    assert _rel(y, y_ref) <= tol
+    X = np.stack([gpu.uniform(7 + j, hm.nd) for j in range(3)], axis=1)
+    Y = a.matmat(X)
+    for j in range(3):
+        assert _rel(Y[:, j], O.adot_body_lfmtx(hm, X[:, j])) <= tol
+    a.free()
+
+
 def test_invalid_leaves_rejected(gpu, H):
     recs = (H.LeafRec * 1)()
     recs[0].ltmtx, recs[0].kt, recs[0].nstrtl, recs[0].ndl = 2, 0, 10, 5

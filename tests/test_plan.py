@@ -59,8 +59,11 @@ def test_long_column_groups_are_chunked(H, O):
     assert fin["base"][order][0] == T["tsize"]
     assert np.all(np.diff(fin["base"][order]) == fin["np"][order][:-1])
     assert len(np.unique(fin["slot"])) == len(fin)
-    # codes address final slots only
-    assert T["codes"].min() >= -T["tsize"]
+    # inputs address final slots only
+    zs = T["zseg"]
+    trun = zs[zs["kind"] == 2]
+    assert len(trun) > 0 and np.all(trun["src"] + trun["ncols"] <= T["tsize"])
+    assert T["codes"].size == 0 or T["codes"].min() >= -T["tsize"]
     x = H.uniform(2, hm.nd)
     _, y = PE.run(T, x, hm.nd)
     y_ref = O.adot_body_lfmtx(hm, x)
@@ -82,13 +85,34 @@ def test_tuned_plans(H, O, two_spheres4, prec, tuning):
         if tuning.get(key, 0) in (64, 128):
             assert T["nt%d_small" % k] == tuning[key] and T["nwork%d_small" % k] > 0
             w = T["work%d" % k]
-            vr = 1 if prec == "f64" else 2
+            vr = {"f64": 1, "f32": 2, "bf16": 4 if k == 1 else 2}[prec]
             assert np.all(w["rows_p"][:T["nwork%d_small" % k]] // vr <= tuning[key])
         elif tuning.get(key, 0) == -1:
             assert T["nwork%d_small" % k] == 0
     x = H.uniform(4, two_spheres4.nd)
     _, y = PE.run(T, x, two_spheres4.nd)
     y_ref = O.adot_body_lfmtx(two_spheres4, x)
+    assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["f64", "bf16"])
+def test_input_runs_and_code_fallback(H, O, prec):
+    """Phase-2 blocks take their inputs as runs (x ranges of dense leaves, t
+    ranges of row groups) up to the run table's size, and a per-column code
+    table beyond it; both give the oracle's apply."""
+    import synthetic_leaves as SL
+    hm = SL.many_runs(H)
+    T = H.plan_tables(hm, prec)
+    zs, w2 = T["zseg"], T["work2"]
+    nz = w2["zseg_end"] - w2["zseg_begin"]
+    kinds = [set(zs["kind"][b:e]) for b, e in zip(w2["zseg_begin"], w2["zseg_end"])]
+    assert any(k == {1} for k in kinds)                 # code table (101 runs)
+    assert any(n > 16 and 1 not in k for n, k in zip(nz, kinds))  # 31 runs tabled
+    assert any(2 in k for k in kinds)                   # t runs
+    assert T["codes"].size == 100 + 5  # rows 0-15 only: 100 x columns, 5 t slots
+    x = H.uniform(5, hm.nd)
+    _, y = PE.run(T, x, hm.nd)
+    y_ref = O.adot_body_lfmtx(hm, x)
     assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < TOL[prec]
 
 
