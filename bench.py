@@ -278,6 +278,12 @@ def study_327680(H, log):
                              "true_relres": ig["true_relres"], "status": ig["status_name"]}
         log("s=7 %s: GCR(8) FP64 %.4f s (%d it), relres %.2e"
             % (rname, ig["seconds"], ig["nstp"], ig["true_relres"]))
+        a64.bicgstab_pipe(b, eps=1e-8, maxit=3000, check_every=8, history=False)
+        _, ip, _ = a64.bicgstab_pipe(b, eps=1e-8, maxit=3000, check_every=8, history=False)
+        row["pipelined_bicgstab_fp64"] = {"seconds": round(ip["seconds"], 4), "iterations": ip["nstp"],
+                                          "true_relres": ip["true_relres"], "status": ip["status_name"]}
+        log("s=7 %s: pipelined BiCGSTAB FP64 %.4f s (%d it), relres %.2e"
+            % (rname, ip["seconds"], ip["nstp"], ip["true_relres"]))
         for p in ("f32", "bf16"):
             H.bicgstab_ir(a64, arenas[p], b, eps=1e-8, inner_eps=inner[p], max_outer=60, max_inner=3000)
             u, inf, hist = H.bicgstab_ir(a64, arenas[p], b, eps=1e-8, inner_eps=inner[p], max_outer=60,

@@ -8,11 +8,15 @@ missing.
 Layers exposed here:
   * mesh helpers            icosphere(), merge(), read_pbf()          (libhmvgen)
   * H-matrix construction   HParams, HMatrix                          (libhmvgen)
-  * engine                  Arena (arena build, apply, BiCGSTAB, IR)  (libhmv)
+  * engine                  Arena (arena build, apply, H-matmat, BiCGSTAB,
+                            pipelined BiCGSTAB, GCR(m), block BiCGSTAB), MultiArena,
+                            bicgstab_ir, bicgstab_ru, autotune             (libhmv)
   * reference-named calls   adot_body_lfcpy_batch_sorted, adot_body_lfmtx_batch,
-                            adot_body_lfdel_batch, bicgstab_cax_lfmtx_gpu
+                            adot_body_lfdel_batch, bicgstab_cax_lfmtx_gpu,
+                            bicgstab_cax_lfmtx_pipe
                             (HACApK_BATCH_SORT.c:86, HACApK_BATCH.c:257,812,
-                             HACApK_SOLVER.c:421) and solve() (m_HACApK_use.f90:190-262)
+                             HACApK_SOLVER.c:421, HACApK_SOLVER_PIPE.c:144) and solve()
+                            (m_HACApK_use.f90:190-262)
 """
 import ctypes as C
 import os
@@ -243,6 +247,10 @@ def hmv_lib():
         L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes = [C.POINTER(RefLeafmtxp),
                                                        C.POINTER(RefLcontrol), _D, _D, _D, _I, _I, _I]
         L.c_hacapk_bicgstab_cax_lfmtx_gpu_.restype = None
+        L.c_hacapk_bicgstab_cax_lfmtx_pipe_.argtypes = L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes
+        L.c_hacapk_bicgstab_cax_lfmtx_pipe_.restype = None
+        L.hmv_bicgstab_pipe.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, _D,
+                                        C.POINTER(SolveInfo)]
         _hmv = L
     return _hmv
 
@@ -581,6 +589,26 @@ class Arena:
             hist = hist[: info.nstp + 1]
         return u, info.as_dict(), hist
 
+    def bicgstab_pipe(self, b, u0=None, eps=1e-8, maxit=500, check_every=1, history=True):
+        """Pipelined BiCGSTAB on device (HACApK_SOLVER_PIPE.c:144-637); host b/u0
+        in cluster order.  Returns (u, info, hist)."""
+        b = np.ascontiguousarray(b, np.float64)
+        u0 = np.zeros_like(b) if u0 is None else np.ascontiguousarray(u0, np.float64)
+        db, du = DeviceBuffer.from_array(b), DeviceBuffer.from_array(u0)
+        hist = np.full(maxit + 1, np.nan) if history else None
+        info = SolveInfo()
+        try:
+            _check(hmv_lib().hmv_bicgstab_pipe(self._a, db.ptr, du.ptr, eps, maxit, check_every,
+                                               _dptr(hist) if history else None, C.byref(info)),
+                   "hmv_bicgstab_pipe")
+            u = du.download()
+        finally:
+            db.free()
+            du.free()
+        if history:
+            hist = hist[: info.nstp + 1]
+        return u, info.as_dict(), hist
+
     def gcrm(self, b, u0=None, eps=1e-8, maxit=500, mreset=8, check_every=1, history=True):
         """GCR(mreset) on device (HACApK_gcrm_lfmtx, m_HACApK_solve.f90:506-571);
         host b/u0 in cluster order.  Returns (u, info, hist)."""
@@ -756,20 +784,29 @@ def adot_body_lfdel_batch(leafmtxp):
     hmv_lib().c_hacapk_adot_body_lfdel_batch_(C.byref(leafmtxp))
 
 
-def bicgstab_cax_lfmtx_gpu(leafmtxp, u, b, param):
-    """c_hacapk_bicgstab_cax_lfmtx_gpu_ (HACApK_SOLVER.c:421): u updated in place.
-    param is the 100-slot Fortran param array seen 0-based (param[82] max
-    steps, param[90] tolerance).  Returns (nstp, lrtrn)."""
+def _ref_solver(fn, leafmtxp, u, b, param):
     assert u.dtype == np.float64 and u.flags.c_contiguous
     b = np.ascontiguousarray(b, np.float64)
     param = np.ascontiguousarray(param, np.float64)
     ctl = RefLcontrol()
     ctl.param = _dptr(param)
     nd, nstp, lrtrn = C.c_int(len(u)), C.c_int(0), C.c_int(0)
-    hmv_lib().c_hacapk_bicgstab_cax_lfmtx_gpu_(C.byref(leafmtxp), C.byref(ctl), _dptr(u), _dptr(b),
-                                               _dptr(param), C.byref(nd), C.byref(nstp),
-                                               C.byref(lrtrn))
+    fn(C.byref(leafmtxp), C.byref(ctl), _dptr(u), _dptr(b), _dptr(param), C.byref(nd), C.byref(nstp),
+       C.byref(lrtrn))
     return nstp.value, lrtrn.value
+
+
+def bicgstab_cax_lfmtx_gpu(leafmtxp, u, b, param):
+    """c_hacapk_bicgstab_cax_lfmtx_gpu_ (HACApK_SOLVER.c:421): u updated in place.
+    param is the 100-slot Fortran param array seen 0-based (param[82] max
+    steps, param[90] tolerance).  Returns (nstp, lrtrn)."""
+    return _ref_solver(hmv_lib().c_hacapk_bicgstab_cax_lfmtx_gpu_, leafmtxp, u, b, param)
+
+
+def bicgstab_cax_lfmtx_pipe(leafmtxp, u, b, param):
+    """c_hacapk_bicgstab_cax_lfmtx_pipe_ (HACApK_SOLVER_PIPE.c:144), the
+    pipelined BiCGSTAB: u receives the solution.  Returns (nstp, lrtrn)."""
+    return _ref_solver(hmv_lib().c_hacapk_bicgstab_cax_lfmtx_pipe_, leafmtxp, u, b, param)
 
 
 def solve(hm, rhs, sol0=None, ztol=1e-8, mstep=500, arena=None, solver="bicgstab", mreset=8):

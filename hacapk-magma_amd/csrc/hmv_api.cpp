@@ -777,9 +777,19 @@ void c_hacapk_adot_body_lfdel_batch_(hmv_ref_leafmtxp *H) {
   if (a) hmv_arena_free(a);
 }
 
-void c_hacapk_bicgstab_cax_lfmtx_gpu_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C, double *u_cpu,
-                                      double *b_cpu, double *param, int *nd, int *nstp,
-                                      int *lrtrn) {
+}  // extern "C"
+
+namespace {
+
+using SolveFn = int (*)(hmv_arena, const double *, double *, double, int, int, double *,
+                        hmv_solve_info *);
+
+// The reference's GPU solver entry points on host vectors: param[82] = max
+// steps, param[90] = tolerance; the arena built by lfcpy_batch_sorted_ is
+// reused, else one is built for the call.
+void ref_solve(const char *name, const char *label, SolveFn solve, hmv_ref_leafmtxp *H,
+               hmv_ref_lcontrol *C, double *u_cpu, double *b_cpu, double *param, int *nd,
+               int *nstp, int *lrtrn) {
   if (lrtrn) *lrtrn = HMV_ERR_ARG;
   if (!H || !param || !nd || !u_cpu || !b_cpu) return;
   const int mstep = (int)param[82];
@@ -791,7 +801,7 @@ void c_hacapk_bicgstab_cax_lfmtx_gpu_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C, 
   bool own = false;
   if (!a) {
     if (hmv_arena_build(*nd, H->nlf, H->st_lf, H->st_lf_stride, HMV_PREC_F64, &a) != HMV_OK) {
-      report("c_hacapk_bicgstab_cax_lfmtx_gpu_");
+      report(name);
       return;
     }
     own = true;
@@ -803,25 +813,43 @@ void c_hacapk_bicgstab_cax_lfmtx_gpu_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C, 
   if (hipMalloc(&du, nb) != hipSuccess || hipMalloc(&db, nb) != hipSuccess ||
       hipMemcpy(du, u_cpu, nb, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(db, b_cpu, nb, hipMemcpyHostToDevice) != hipSuccess) {
-    set_error("c_hacapk_bicgstab_cax_lfmtx_gpu_: device vectors");
+    set_error(std::string(name) + ": device vectors");
     rc = HMV_ERR_DEVICE;
   }
-  if (rc == HMV_OK) rc = hmv_bicgstab(a, db, du, eps, mstep, 1, nullptr, &info);
+  if (rc == HMV_OK) rc = solve(a, db, du, eps, mstep, 1, nullptr, &info);
   if (rc == HMV_OK && hipMemcpy(u_cpu, du, nb, hipMemcpyDeviceToHost) != hipSuccess) {
-    set_error("c_hacapk_bicgstab_cax_lfmtx_gpu_: copy back");
+    set_error(std::string(name) + ": copy back");
     rc = HMV_ERR_DEVICE;
   }
-  if (rc != HMV_OK) report("c_hacapk_bicgstab_cax_lfmtx_gpu_");
+  if (rc != HMV_OK) report(name);
   if (nstp) *nstp = info.nstp;
   if (lrtrn) *lrtrn = rc == HMV_OK ? info.status : rc;
   if (verbose > 0 && rc == HMV_OK) {
-    std::printf(" BiCGSTAB (hmv): steps=%d relres=%.2e true relres=%.2e time=%.5e s\n", info.nstp,
+    std::printf(" %s (hmv): steps=%d relres=%.2e true relres=%.2e time=%.5e s\n", label, info.nstp,
                 info.relres, info.true_relres, info.seconds);
     std::fflush(stdout);
   }
   (void)hipFree(du);
   (void)hipFree(db);
   if (own) hmv_arena_free(a);
+}
+
+}  // namespace
+
+extern "C" {
+
+void c_hacapk_bicgstab_cax_lfmtx_gpu_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C, double *u_cpu,
+                                      double *b_cpu, double *param, int *nd, int *nstp,
+                                      int *lrtrn) {
+  ref_solve("c_hacapk_bicgstab_cax_lfmtx_gpu_", "BiCGSTAB", hmv_bicgstab, H, C, u_cpu, b_cpu, param,
+            nd, nstp, lrtrn);
+}
+
+void c_hacapk_bicgstab_cax_lfmtx_pipe_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C, double *u_cpu,
+                                       double *b_cpu, double *param, int *nd, int *nstp,
+                                       int *lrtrn) {
+  ref_solve("c_hacapk_bicgstab_cax_lfmtx_pipe_", "pipelined BiCGSTAB", hmv_bicgstab_pipe, H, C,
+            u_cpu, b_cpu, param, nd, nstp, lrtrn);
 }
 
 }  // extern "C"

@@ -84,11 +84,8 @@ int hmv_init(int device);
 int hmv_arena_build(int nd, int nlf, const void *leaves, int leaf_stride, int precision,
                     hmv_arena *out);
 
-/* Same, with explicit repack tunables (used by the autotuner); <= 0 keeps
- * the default: p1_target_bytes = arena bytes per V^T x workgroup (96 KiB),
- * p1_row_tile = max rank vectors per V^T x workgroup (256), p2_max_rows =
- * max rows per [A|U] workgroup (256). */
-/* Repack tunables; 0 in a field = the precision's default.
+/* Repack tunables for hmv_arena_build_tuned (used by the autotuner); 0 in a
+ * field = the precision's default (96 KiB, 256, 256, and per precision).
  *   p1_target_bytes   arena bytes per phase-1 workgroup (rank-vector rows x columns)
  *   p1_row_tile       most rank vectors per phase-1 workgroup
  *   p2_max_rows       most rows per phase-2 workgroup
@@ -197,6 +194,14 @@ int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, d
  * ||b - A64 u|| / ||b|| < eps.  info->outer = number of replacements. */
 int hmv_bicgstab_ru(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
                     int maxit, int check_every, double delta, hmv_solve_info *info);
+
+/* Pipelined BiCGSTAB (c_hacapk_bicgstab_cax_lfmtx_pipe_,
+ * HACApK_SOLVER_PIPE.c:144-637) on device vectors u (initial guess in,
+ * solution out), b: the same recurrences, with the iterate started at u and
+ * returned (the reference never initialises nor returns it).  Arguments and
+ * stopping as for hmv_bicgstab. */
+int hmv_bicgstab_pipe(hmv_arena a, const double *b, double *u, double eps, int maxit,
+                      int check_every, double *hist, hmv_solve_info *info);
 
 /* GCR(m) (HACApK_gcrm_lfmtx, m_HACApK_solve.f90:506-571) on device vectors
  * u (initial guess in, solution out), b: m search directions kept, restart
@@ -331,6 +336,12 @@ void c_hacapk_adot_body_lfdel_batch_(hmv_ref_leafmtxp *st_leafmtxp);
 void c_hacapk_bicgstab_cax_lfmtx_gpu_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lcontrol *st_ctl,
                                       double *u_cpu, double *b_cpu, double *param, int *nd,
                                       int *nstp, int *lrtrn);
+/* Replaces c_hacapk_bicgstab_cax_lfmtx_pipe_ (HACApK_SOLVER_PIPE.c:144): the
+ * pipelined BiCGSTAB (hmv_bicgstab_pipe) with the same host arguments; u
+ * receives the solution. */
+void c_hacapk_bicgstab_cax_lfmtx_pipe_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lcontrol *st_ctl,
+                                       double *u_cpu, double *b_cpu, double *param, int *nd,
+                                       int *nstp, int *lrtrn);
 
 #ifdef __cplusplus
 }

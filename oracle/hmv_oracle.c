@@ -13,6 +13,8 @@
  *   oracle_setcutthread      m_HACApK_base.f90:584-611 (HACApK_setcutthread, ktp=0)
  *   oracle_bicgstab          m_HACApK_solve.f90:342-449 (HACApK_bicgstab_cax_lfmtx_hyp)
  *   oracle_gcrm              m_HACApK_solve.f90:506-571 (HACApK_gcrm_lfmtx)
+ *   oracle_bicgstab_pipe     HACApK_SOLVER_PIPE.c:144-637 (c_hacapk_bicgstab_cax_lfmtx_pipe_,
+ *                            pipelined BiCGSTAB, single process, no diagonal scaling)
  *   oracle_dotp              HACApK_lib.f90:26-29 (HACApK_dotp_d)
  *   oracle_dense             brute-force assembly of H from its leaves (for checks)
  *
@@ -28,6 +30,10 @@
  *     so its residual norms equal GMRES's (scipy) to rounding; convergence to the
  *     dense solution.  The restart step is the reference's own (see the
  *     function); no reference run covers it: parity of that step unpinned.
+ *   - bicgstab_pipe: in exact arithmetic the pipelined recurrences produce the
+ *     iterates of BiCGSTAB with the same shadow residual, so its residual norms
+ *     equal oracle_bicgstab's (itself pinned above) while far above rounding;
+ *     convergence to the dense solution.
  *   - setcutthread: leaf ranges tile 1..nlf (invariant test).
  */
 #include <math.h>
@@ -328,6 +334,68 @@ done:
   free(zp);
   free(zap);
   free(capap);
+  return nstp;
+}
+
+/* HACApK_SOLVER_PIPE.c:144-637, one process.  The reference accumulates the
+ * iterate in a device vector zx that it never initialises nor returns (it
+ * copies u, the unchanged initial guess, back; :188,483-484,595); here
+ * zx starts at u and is returned in u.  Every other recurrence is kept, in
+ * the reference's order.  hist[0] initial, hist[step] after each step. */
+int oracle_bicgstab_pipe(const void *leaves, int nlf, int stride, int nd, double *u,
+                         const double *b, int mstep, double eps, int nthr, double *hist) {
+  double *mem = (double *)malloc(sizeof(double) * (size_t)nd * 11);
+  if (!mem) return -1;
+  double *zr = mem, *zr0 = zr + nd, *zw = zr0 + nd, *zt = zw + nd, *zv = zt + nd;
+  double *zp = zv + nd, *zs = zp + nd, *zz = zs + nd, *zq = zz + nd, *zy = zq + nd;
+  double *zx = u;
+  double alpha = 0.0, beta = 0.0, zeta = 0.0;
+  const double bnorm = sqrt(oracle_dotp(nd, b, b));
+  oracle_adot_lfmtx_hyp(zv, leaves, nlf, stride, nd, u, nthr);        /* zv = A u  */
+  for (int i = 0; i < nd; i++) zr[i] = b[i] - zv[i];                  /* zr = b - zv */
+  memcpy(zr0, zr, sizeof(double) * nd);
+  oracle_adot_lfmtx_hyp(zw, leaves, nlf, stride, nd, zr, nthr);       /* zw = A zr */
+  oracle_adot_lfmtx_hyp(zt, leaves, nlf, stride, nd, zw, nthr);       /* zt = A zw */
+  double znorm = oracle_dotp(nd, zr, zr);
+  double zrnorm = oracle_dotp(nd, zw, zr);
+  alpha = znorm / zrnorm;
+  zrnorm = sqrt(znorm);
+  if (hist) hist[0] = zrnorm / bnorm;
+  int nstp = 0;
+  for (int step = 1; step <= mstep; step++) {
+    if (zrnorm / bnorm < eps) break;
+    if (step > 1) {
+      for (int i = 0; i < nd; i++) zp[i] = zr[i] + beta * (zp[i] - zeta * zs[i]);
+      for (int i = 0; i < nd; i++) zs[i] = zw[i] + beta * (zs[i] - zeta * zz[i]);
+      for (int i = 0; i < nd; i++) zz[i] = zt[i] + beta * (zz[i] - zeta * zv[i]);
+    } else {
+      memcpy(zp, zr, sizeof(double) * nd);
+      memcpy(zs, zw, sizeof(double) * nd);
+      memcpy(zz, zt, sizeof(double) * nd);
+    }
+    for (int i = 0; i < nd; i++) zq[i] = zr[i] - alpha * zs[i];
+    for (int i = 0; i < nd; i++) zy[i] = zw[i] - alpha * zz[i];
+    const double zqy = oracle_dotp(nd, zy, zq), zden = oracle_dotp(nd, zy, zy);
+    zeta = zqy / zden;
+    oracle_adot_lfmtx_hyp(zv, leaves, nlf, stride, nd, zz, nthr);     /* zv = A zz */
+    for (int i = 0; i < nd; i++) zx[i] = zx[i] + alpha * zp[i] + zeta * zq[i];
+    for (int i = 0; i < nd; i++) zr[i] = zq[i] - zeta * zy[i];
+    /* zw = zy - zeta*(zt - alpha*zv) (the code's dlacpy/daxpy/dlascl/daxpy) */
+    for (int i = 0; i < nd; i++) zw[i] = zy[i] + (-zeta) * (zt[i] - alpha * zv[i]);
+    const double znormold = znorm;
+    znorm = oracle_dotp(nd, zr, zr0);
+    const double zwnorm = oracle_dotp(nd, zw, zr0);
+    const double zsnorm = oracle_dotp(nd, zs, zr0);
+    const double zznorm = oracle_dotp(nd, zz, zr0);
+    const double rr = oracle_dotp(nd, zr, zr);
+    beta = (alpha / zeta) * (znorm / znormold);
+    alpha = znorm / (zwnorm + beta * zsnorm - beta * zeta * zznorm);
+    zrnorm = sqrt(rr);
+    nstp = step;
+    if (hist) hist[step] = zrnorm / bnorm;
+    oracle_adot_lfmtx_hyp(zt, leaves, nlf, stride, nd, zw, nthr);     /* zt = A zw */
+  }
+  free(mem);
   return nstp;
 }
 

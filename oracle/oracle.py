@@ -31,6 +31,8 @@ def lib():
                                       C.c_int, _D]
         L.oracle_gcrm.argtypes = [vp, C.c_int, C.c_int, C.c_int, _D, _D, C.c_int, C.c_int,
                                   C.c_double, C.c_int, _D]
+        L.oracle_bicgstab_pipe.argtypes = [vp, C.c_int, C.c_int, C.c_int, _D, _D, C.c_int,
+                                           C.c_double, C.c_int, _D]
         L.oracle_dotp.argtypes = [C.c_int, _D, _D]
         L.oracle_dotp.restype = C.c_double
         L.oracle_dense.argtypes = [vp, C.c_int, C.c_int, C.c_int, _D]
@@ -103,6 +105,16 @@ def gcrm(hm, b, u0=None, mstep=500, mreset=8, eps=1e-8, nthr=1):
     hist = np.full(mstep + 1, np.nan)
     nstp = lib().oracle_gcrm(hm.leaves_ptr, hm.nlf, hm.stride, hm.nd, _d(u), _d(b), mstep, mreset,
                              eps, nthr, _d(hist))
+    return u, nstp, hist[: nstp + 1]
+
+
+def bicgstab_pipe(hm, b, u0=None, mstep=500, eps=1e-8, nthr=1):
+    """HACApK_SOLVER_PIPE.c:144-637 (iterate returned).  Returns (u, nstp, hist)."""
+    b = np.ascontiguousarray(b, np.float64)
+    u = np.zeros(hm.nd) if u0 is None else np.ascontiguousarray(u0, np.float64).copy()
+    hist = np.full(mstep + 1, np.nan)
+    nstp = lib().oracle_bicgstab_pipe(hm.leaves_ptr, hm.nlf, hm.stride, hm.nd, _d(u), _d(b), mstep,
+                                      eps, nthr, _d(hist))
     return u, nstp, hist[: nstp + 1]
 
 

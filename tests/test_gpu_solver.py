@@ -113,3 +113,45 @@ def test_reliable_update_mixed_precision(gpu, two_spheres4, prec):
     assert info["outer"] >= 1
     assert np.linalg.norm(u - u64) / np.linalg.norm(u64) < 1e-6
     a64.free(); alow.free()
+
+
+def test_pipelined_bicgstab_matches_oracle(gpu, O, sphere4):
+    """hmv_bicgstab_pipe against the oracle's restatement of
+    HACApK_SOLVER_PIPE.c:144-637: same early residual history, step count
+    within a few, and the same solution.  Rounding grows fast in these
+    recurrences: on this case a 1e-15 relative change of b moves the oracle
+    BiCGSTAB's log10 residuals by 7e-9 at step 6 and by 0.4 at step 9, so
+    the histories are compared over the steps with residual above 3e-2."""
+    hm = sphere4
+    b = gpu.uniform(21, hm.nd)
+    a = gpu.Arena(hm, "f64")
+    u, info, hist = a.bicgstab_pipe(b, eps=1e-8, maxit=300)
+    u_ref, nstp_ref, hist_ref = O.bicgstab_pipe(hm, b, None, mstep=300, eps=1e-8)
+    print("gpu steps", info["nstp"], "oracle", nstp_ref, "true relres %.2e" % info["true_relres"],
+          flush=True)
+    assert info["status_name"] == "converged"
+    assert abs(info["nstp"] - nstp_ref) <= 5
+    n = min(len(hist), len(hist_ref))
+    early = (hist_ref[:n] > 3e-2) & (hist[:n] > 3e-2)
+    assert early.sum() >= 5
+    assert np.allclose(np.log10(hist[:n][early]), np.log10(hist_ref[:n][early]), atol=1e-5)
+    assert info["true_relres"] < 5e-8
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-6
+    # check_every does not change the iterates
+    u8, i8, _ = a.bicgstab_pipe(b, eps=1e-8, maxit=300, check_every=8)
+    assert i8["nstp"] == info["nstp"] and np.array_equal(u8, u)
+    a.free()
+
+
+def test_reference_named_pipelined_bicgstab(gpu, O, sphere3):
+    hm = sphere3
+    lp = gpu.make_leafmtxp(hm)
+    param = np.zeros(100)
+    param[82] = 200
+    param[90] = 1e-8
+    b = np.ones(hm.nd)
+    u = np.zeros(hm.nd)
+    nstp, lrtrn = gpu.bicgstab_cax_lfmtx_pipe(lp, u, b, param)
+    A = O.dense(hm)
+    assert lrtrn == 0 and nstp > 0
+    assert np.linalg.norm(A @ u - b) / np.linalg.norm(b) < 5e-8

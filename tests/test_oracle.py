@@ -111,6 +111,29 @@ def test_gcrm_converges_to_dense_solution(O, H, sphere3):
     assert np.linalg.norm(u - np.linalg.solve(A, b)) / np.linalg.norm(u) < 1e-7
 
 
+def test_bicgstab_pipe_matches_bicgstab(O, H, sphere3):
+    """The pipelined recurrences (HACApK_SOLVER_PIPE.c:144-637) give BiCGSTAB's
+    iterates in exact arithmetic: the residual norms equal the (pinned)
+    BiCGSTAB oracle's while rounding has not yet grown, and both reach the
+    dense solution.  On this case BiCGSTAB itself turns a 1e-15 relative
+    change of b into log10-residual differences of 2e-4 by step 9 and 1e-1
+    later, so the histories are compared over the first steps (residual
+    above 1e-2) and the step counts loosely."""
+    hm = sphere3
+    b = H.uniform(8, hm.nd)
+    u1, n1, h1 = O.bicgstab(hm, b, None, mstep=300, eps=1e-10)
+    u2, n2, h2 = O.bicgstab_pipe(hm, b, None, mstep=300, eps=1e-10)
+    n = min(len(h1), len(h2))
+    early = (h1[:n] > 1e-2) & (h2[:n] > 1e-2)
+    assert early.sum() >= 6
+    assert np.allclose(np.log10(h1[:n][early]), np.log10(h2[:n][early]), atol=1e-6)
+    assert abs(n1 - n2) <= 5
+    A = O.dense(hm)
+    x = np.linalg.solve(A, b)
+    assert np.linalg.norm(u2 - x) / np.linalg.norm(x) < 1e-8
+    assert np.linalg.norm(A @ u2 - b) / np.linalg.norm(b) < 2e-10
+
+
 def test_bicgstab_small_dense_terminates(O, H):
     """On a 6x6 system BiCGSTAB reaches rounding level in at most 6 steps; the
     system is passed to the oracle as one dense leaf."""
