@@ -238,6 +238,11 @@ def hmv_lib():
         L.hmv_plan_create_tuned.argtypes = [C.c_int, C.c_int, vp, C.c_int, C.c_int, C.POINTER(Tuning),
                                             C.POINTER(vp)]
         L.hmv_plan_free.argtypes = [vp]
+        L.hmv_plan_save.argtypes = [vp, C.c_char_p]
+        L.hmv_plan_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.hmv_arena_from_plan.argtypes = [vp, C.POINTER(vp)]
+        L.hmv_arena_save.argtypes = [vp, C.c_char_p]
+        L.hmv_arena_load.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.c_hacapk_adot_body_lfcpy_batch_sorted_.argtypes = [_I, C.POINTER(RefLeafmtxp)]
         L.c_hacapk_adot_body_lfcpy_batch_sorted_.restype = None
         L.c_hacapk_adot_body_lfmtx_batch_.argtypes = [_D, C.POINTER(RefLeafmtxp), _D, _D, _D, _D, _D]
@@ -494,6 +499,23 @@ class Arena:
                    "hmv_arena_build")
         self._finish(a)
 
+    @classmethod
+    def load(cls, path):
+        """Arena from an arena file (hmv_arena_load): no leaves, no repack."""
+        a = C.c_void_p()
+        _check(hmv_lib().hmv_arena_load(os.fsencode(path), C.byref(a)), "hmv_arena_load")
+        self = cls.__new__(cls)
+        st = _ArenaStats()
+        _check(hmv_lib().hmv_arena_stats_get(a, C.byref(st)), "arena stats")
+        self.nd = st.nd
+        self.precision = {v: k for k, v in PREC.items()}[st.precision]
+        self._finish(a)
+        return self
+
+    def save(self, path):
+        """Writes the arena file (hmv_arena_save)."""
+        _check(hmv_lib().hmv_arena_save(self._a, os.fsencode(path)), "hmv_arena_save")
+
     def _finish(self, a):
         self._a = a
         st = _ArenaStats()
@@ -629,15 +651,42 @@ class Arena:
         return u, info.as_dict(), hist
 
 
+def _plan_create(hm, precision, tuning):
+    p = C.c_void_p()
+    t = Tuning.from_dict(tuning) if tuning else None
+    _check(hmv_lib().hmv_plan_create_tuned(hm.nd, hm.nlf, hm.leaves_ptr, hm.stride, PREC[precision],
+                                           C.byref(t) if t is not None else None, C.byref(p)),
+           "hmv_plan_create")
+    return p
+
+
 def plan_tables(hm, precision="f64", tuning=None):
     """Host-side repack of an HMatrix (hmv_plan_create_tuned): numpy copies of
     the arena image and the engine's segment/work tables.  Needs no device."""
-    L = hmv_lib()
+    return _plan_view(_plan_create(hm, precision, tuning))
+
+
+def save_plan(hm, path, precision="f64", tuning=None):
+    """Repacks an HMatrix on the host and writes the arena file (hmv_plan_save).
+    Needs no device."""
+    p = _plan_create(hm, precision, tuning)
+    try:
+        _check(hmv_lib().hmv_plan_save(p, os.fsencode(path)), "hmv_plan_save")
+    finally:
+        hmv_lib().hmv_plan_free(p)
+
+
+def load_plan_tables(path):
+    """Reads and checks an arena file (hmv_plan_load); returns the tables as
+    plan_tables does.  Needs no device."""
     p = C.c_void_p()
-    t = Tuning.from_dict(tuning) if tuning else None
-    _check(L.hmv_plan_create_tuned(hm.nd, hm.nlf, hm.leaves_ptr, hm.stride, PREC[precision],
-                                   C.byref(t) if t is not None else None, C.byref(p)),
-           "hmv_plan_create")
+    _check(hmv_lib().hmv_plan_load(os.fsencode(path), C.byref(p)), "hmv_plan_load")
+    return _plan_view(p)
+
+
+def _plan_view(p):
+    """Tables of plan handle p as numpy copies; frees the plan."""
+    L = hmv_lib()
     try:
         t = _PlanTables()
         _check(L.hmv_plan_view(p, C.byref(t)), "hmv_plan_view")

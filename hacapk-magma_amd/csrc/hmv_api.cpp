@@ -118,6 +118,8 @@ hmv::PlanTuning from_public(int prec, const hmv_tuning *t) {
   return p;
 }
 
+int upload_plan(hmv::Plan &P, hmv_arena *out, std::chrono::steady_clock::time_point t0);
+
 int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::PlanTuning &tune,
           hmv_arena *out, const std::vector<int> *seg = nullptr) {
   if (!out) {
@@ -134,6 +136,14 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
     set_error("hmv_arena_build: " + err);
     return HMV_ERR_ARG;
   }
+  if (seg) P.seg = *seg;
+  return upload_plan(P, out, t0);
+}
+
+// Uploads a plan (built here or read from a file) as a device arena.
+int upload_plan(hmv::Plan &P, hmv_arena *out, std::chrono::steady_clock::time_point t0) {
+  const int nd = P.nd, prec = P.prec;
+  const std::vector<int> *seg = &P.seg;
   hmv_arena_s *a = new hmv_arena_s();
   hipGetDevice(&a->device);
   a->nd = nd;
@@ -215,7 +225,21 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
   L.t = a->d_t;
   L.winst1 = a->d_winst1;
   L.winst2 = a->d_winst2;
-  a->tuning = to_public(tune);
+  a->tuning.p1_target_bytes = P.tune.p1_target_bytes;
+  a->tuning.p1_row_tile = P.tune.p1_row_tile;
+  a->tuning.p2_max_rows = P.tune.p2_max_rows;
+  a->tuning.p1_small_threads = P.tune.p1_small_threads;
+  a->tuning.p2_small_threads = P.tune.p2_small_threads;
+  a->seg = P.seg;
+  a->nzseg = (int64_t)P.zseg.size();
+  a->noutidx = (int64_t)P.outidx.size();
+  a->ncodes = (int64_t)P.codes.size();
+  a->arena_size = P.arena_size;
+  a->nwork1_small = P.nwork1_small;
+  a->nt1_small = P.nt1_small;
+  a->nwork2_small = P.nwork2_small;
+  a->nt2_small = P.nt2_small;
+  a->pstats = P.st;
   auto t1 = std::chrono::steady_clock::now();
   hmv_arena_stats &s = a->stats;
   s.nd = nd;
@@ -310,7 +334,6 @@ int hmv_arena_build_multi(int ninst, const int *nd, const int *nlf, const void *
   }
   int rc = build(seg[ninst], (int)total_lf, all.data(), (int)sizeof(hmv::RefLeaf), precision,
                  from_public(precision, tuning), out, &seg);
-  if (!rc) (*out)->seg = seg;
   return rc;
 }
 
@@ -668,6 +691,75 @@ int hmv_plan_create_tuned(int nd, int nlf, const void *leaves, int leaf_stride, 
   }
   *out = p;
   return HMV_OK;
+}
+
+int hmv_plan_save(hmv_plan p, const char *path) {
+  if (!p || !path) {
+    set_error("hmv_plan_save: null argument");
+    return HMV_ERR_ARG;
+  }
+  std::string err;
+  if (!hmv::save_plan_file(p->P, path, err)) {
+    set_error("hmv_plan_save: " + err);
+    return HMV_ERR_ARG;
+  }
+  return HMV_OK;
+}
+
+int hmv_plan_load(const char *path, hmv_plan *out) {
+  if (!path || !out) {
+    set_error("hmv_plan_load: null argument");
+    return HMV_ERR_ARG;
+  }
+  *out = nullptr;
+  hmv_plan p = new hmv_plan_s();
+  std::string err;
+  if (!hmv::load_plan_file(path, p->P, err)) {
+    delete p;
+    set_error("hmv_plan_load: " + err);
+    return HMV_ERR_ARG;
+  }
+  *out = p;
+  return HMV_OK;
+}
+
+int hmv_arena_from_plan(hmv_plan p, hmv_arena *out) {
+  if (!p || !out) {
+    set_error("hmv_arena_from_plan: null argument");
+    return HMV_ERR_ARG;
+  }
+  *out = nullptr;
+  int rc = check_device();
+  if (rc) return rc;
+  std::string err;
+  if (!hmv::validate_plan(p->P, err)) {
+    set_error("hmv_arena_from_plan: " + err);
+    return HMV_ERR_ARG;
+  }
+  return upload_plan(p->P, out, std::chrono::steady_clock::now());
+}
+
+int hmv_arena_save(hmv_arena a, const char *path) {
+  if (!a || !path) {
+    set_error("hmv_arena_save: null argument");
+    return HMV_ERR_ARG;
+  }
+  hmv_plan_s p;
+  std::string err;
+  if (!hmv::download_plan(a, p.P, err)) {
+    set_error("hmv_arena_save: " + err);
+    return HMV_ERR_DEVICE;
+  }
+  return hmv_plan_save(&p, path);
+}
+
+int hmv_arena_load(const char *path, hmv_arena *out) {
+  hmv_plan p = nullptr;
+  int rc = hmv_plan_load(path, &p);
+  if (rc) return rc;
+  rc = hmv_arena_from_plan(p, out);
+  hmv_plan_free(p);
+  return rc;
 }
 
 int hmv_plan_create(int nd, int nlf, const void *leaves, int leaf_stride, int precision,

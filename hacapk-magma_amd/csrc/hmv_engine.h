@@ -9,6 +9,7 @@
 
 #include "hmv.h"
 #include "hmv_internal.h"
+#include "hmv_plan.h"
 
 struct HmvSolveWork;
 
@@ -39,6 +40,11 @@ struct hmv_arena_s {
   double *d_hx = nullptr, *d_hy = nullptr;  // scratch for hmv_apply_host
   HmvSolveWork *solve = nullptr;             // solver workspace (lazy)
   hmv_arena_stats stats{};
+  // what a saved arena file needs besides the device tables
+  int64_t nzseg = 0, noutidx = 0, ncodes = 0;
+  size_t arena_size = 0;
+  int nwork1_small = 0, nt1_small = 0, nwork2_small = 0, nt2_small = 0;
+  hmv::PlanStats pstats{};
 };
 
 namespace hmv {
@@ -52,6 +58,8 @@ void free_solve_work(hmv_arena_s *a);
 // Enqueues Y = H X + beta*Y for k interleaved right-hand sides (1..128).
 hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
                                const int *skip, hipStream_t s);
+// Copies an arena's device tables back into a host plan (for saving).
+bool download_plan(const hmv_arena_s *a, Plan &P, std::string &err);
 // Makes the H-matmat t buffer hold k right-hand sides.
 int ensure_tk(hmv_arena_s *a, int k);
 }  // namespace hmv

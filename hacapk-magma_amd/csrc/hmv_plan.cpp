@@ -103,6 +103,12 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   };
   P.nd = nd;
   P.prec = prec;
+  P.tune.p1_target_bytes = tune.p1_target_bytes;
+  P.tune.p1_row_tile = tune.p1_row_tile;
+  P.tune.p2_max_rows = tune.p2_max_rows;
+  P.tune.p1_small_threads = tune.p1_small_threads;
+  P.tune.p2_small_threads = tune.p2_small_threads;
+  P.seg = {0, nd};
   if (nd <= 0) {
     err = "nd must be positive";
     return false;

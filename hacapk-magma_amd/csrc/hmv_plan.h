@@ -58,6 +58,11 @@ struct Plan {
   std::vector<int> codes;          // phase-2 column input codes (HMV_Z_CODES)
   std::vector<HmvTFin> tfin;       // chunked rank vectors: t slot = sum of partials
   PlanStats st;
+  struct PlanTuningRec {           // the tunables the plan was built with
+    int64_t p1_target_bytes = 0;
+    int p1_row_tile = 0, p2_max_rows = 0, p1_small_threads = 0, p2_small_threads = 0;
+  } tune;
+  std::vector<int> seg;            // instance row offsets (ninst + 1): {0, nd} for one H-matrix
 };
 
 // Tunables of the repack (chosen by the autotuner or left at defaults).
@@ -77,6 +82,17 @@ PlanTuning default_tuning(int prec);
 // Returns false and fills `err` on invalid input.
 bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
                 const PlanTuning &tune, Plan &out, std::string &err);
+
+// Checks every index of a plan against the buffers it addresses (work
+// blocks inside the arena image, input runs, codes and outputs inside x, t
+// and y, partial sums inside t), so that a plan read from a file cannot make
+// a kernel read or write out of bounds.  Returns false with `err` set.
+bool validate_plan(const Plan &P, std::string &err);
+
+// Versioned arena file (hmv_file.cpp): write a plan, and read one back
+// (checksums and validate_plan included).
+bool save_plan_file(const Plan &P, const char *path, std::string &err);
+bool load_plan_file(const char *path, Plan &P, std::string &err);
 
 int dtype_size(int dt);
 // Rows handled per lane (each lane loads `row_vector(dt)` consecutive rows).

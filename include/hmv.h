@@ -271,8 +271,10 @@ int hmv_stream_test(int kind, const void *src, void *dst, size_t bytes, int reps
  * CPU-side validation of the arena's indexing.  Tables use the engine's
  * internal record layouts: work = {int64 off0, off1; int rows, rows_p,
  * ncol0, ncol1; int16 dt0, dt1; int zseg_begin, zseg_end, out_base}
- * (48 bytes), zseg = {int kind, ncols, src, pad} (16 bytes), codes = one
- * int per phase-2 column (>= 0: x index; < 0: t slot -1-code), tfin =
+ * (48 bytes); zseg = {int kind, ncols, src, col} (16 bytes): a run of a
+ * block's input columns starting at block column col, from x[src..] (kind
+ * 0) or t[src..] (kind 2), or (kind 1, alone) from codes[src..], one int per
+ * column (>= 0: x index; < 0: t slot -1-code), used past 64 runs; tfin =
  * {int slot, base, np, pad} (16 bytes): t[slot] = t[base] + ... +
  * t[base+np-1] for rank vectors whose column group is cut into np chunks.
  * The t buffer holds tsize slots followed by those partials (tpart_size
@@ -297,6 +299,21 @@ int hmv_plan_create_tuned(int nd, int nlf, const void *leaves, int leaf_stride, 
                           const hmv_tuning *tuning, hmv_plan *out);
 int hmv_plan_view(hmv_plan p, hmv_plan_tables *t);
 int hmv_plan_free(hmv_plan p);
+
+/* ---- arena files -------------------------------------------------------
+ * A versioned binary file of the repack (arena image and tables, format
+ * version HMV_ARENA_FILE_VERSION, checksummed sections), so that later runs
+ * upload it without the leaves.  No reference counterpart: the reference
+ * rebuilds the H-matrix and its device copy every run (param(12) "write
+ * down the H-matrix", m_HACApK_base.f90:300, is declared and unused).
+ * Loading checks the checksums and every index of the tables against the
+ * buffers it addresses; a file that fails is rejected with HMV_ERR_ARG. */
+#define HMV_ARENA_FILE_VERSION 1
+int hmv_plan_save(hmv_plan p, const char *path);        /* host only */
+int hmv_plan_load(const char *path, hmv_plan *out);     /* host only */
+int hmv_arena_from_plan(hmv_plan p, hmv_arena *out);    /* uploads a plan */
+int hmv_arena_save(hmv_arena a, const char *path);      /* copies the device tables back */
+int hmv_arena_load(const char *path, hmv_arena *out);   /* hmv_plan_load + hmv_arena_from_plan */
 
 /* ---- reference-named entry points --------------------------------------
  * For a Fortran driver built WITHOUT HAVE_MAGMA: the C view of its

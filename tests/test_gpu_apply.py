@@ -203,3 +203,36 @@ def test_invalid_leaves_rejected(gpu, H):
     recs[0].ltmtx = 3
     with pytest.raises(H.HmvError, match="ltmtx"):
         H.Arena(None, "f64", nd=20, nlf=1, leaves_ptr=C.addressof(recs), stride=C.sizeof(H.LeafRec))
+
+
+@pytest.mark.parametrize("prec", ["f64", "bf16"])
+def test_arena_file_round_trip(gpu, O, sphere4, tmp_path, prec):
+    """An arena written by hmv_arena_save and one written on the host from
+    the plan both load (hmv_arena_load) into arenas whose apply is bit-equal
+    to the original's."""
+    a = gpu.Arena(sphere4, prec)
+    x = gpu.uniform(12, sphere4.nd)
+    y = a.matvec(x)
+    a.save(tmp_path / "dev.hmv")
+    gpu.save_plan(sphere4, tmp_path / "host.hmv", prec)
+    for name in ("dev.hmv", "host.hmv"):
+        b = gpu.Arena.load(tmp_path / name)
+        assert b.nd == sphere4.nd and b.precision == prec
+        assert b.stats["arena_bytes"] == a.stats["arena_bytes"]
+        assert np.array_equal(b.matvec(x), y)
+        b.free()
+    a.free()
+    assert (tmp_path / "dev.hmv").read_bytes() == (tmp_path / "host.hmv").read_bytes()
+
+
+def test_multi_arena_file_keeps_instances(gpu, O, sphere3, sphere4, tmp_path):
+    m = gpu.MultiArena([sphere3, sphere4], "f64")
+    m.save(tmp_path / "multi.hmv")
+    b = gpu.Arena.load(tmp_path / "multi.hmv")
+    offs = (C.c_int * 3)()
+    assert gpu.hmv_lib().hmv_arena_segments(b.handle, offs, 3) == 2
+    assert list(offs) == [0, sphere3.nd, sphere3.nd + sphere4.nd]
+    x = gpu.uniform(13, b.nd)
+    assert np.array_equal(b.matvec(x), m.matvec(x))
+    b.free()
+    m.free()
