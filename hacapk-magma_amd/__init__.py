@@ -6,7 +6,8 @@ fallback: every compute call raises HmvError when the library or the GPU is
 missing.
 
 Layers exposed here:
-  * mesh helpers            icosphere(), merge(), read_pbf()          (libhmvgen)
+  * mesh helpers            icosphere(), merge(), read_pbf(), write_pbf(),
+                            write_vtk()                               (libhmvgen)
   * H-matrix construction   HParams, HMatrix                          (libhmvgen)
   * engine                  Arena (arena build, apply, H-matmat, BiCGSTAB,
                             pipelined BiCGSTAB, GCR(m), block BiCGSTAB), MultiArena,
@@ -164,6 +165,8 @@ def gen_lib():
         L.hmvgen_icosphere.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                        _I, _I, _D, _I]
         L.hmvgen_read_pbf.argtypes = [C.c_char_p, _I, _I, _D, _I, _D]
+        L.hmvgen_write_pbf.argtypes = [C.c_char_p, C.c_int, _D, C.c_int, _I, C.c_int, _D]
+        L.hmvgen_write_vtk.argtypes = [C.c_char_p, C.c_int, _D, C.c_int, _I, _D]
         L.hmvgen_build.argtypes = [C.c_int, _D, C.c_int, _I, C.POINTER(_GenParams),
                                    C.POINTER(C.c_void_p)]
         L.hmvgen_free.argtypes = [C.c_void_p]
@@ -311,6 +314,31 @@ def read_pbf(path):
     _check_gen(L.hmvgen_read_pbf(path.encode(), C.byref(nn), C.byref(nf), _dptr(nodes),
                                  faces.ctypes.data_as(_I), _dptr(para)), "read_pbf")
     return nodes, faces, para
+
+
+def write_pbf(path, nodes, faces, dble_para=None):
+    """Mesh file in the reference's .pbf layout (hmvgen_write_pbf); dble_para:
+    None, one value per face, or an (nface, ndble) array."""
+    nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1, 3)
+    faces = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
+    if dble_para is None:
+        nd, dp = 0, None
+    else:
+        dp = np.ascontiguousarray(dble_para, np.float64).reshape(len(faces), -1)
+        nd = dp.shape[1]
+    _check_gen(gen_lib().hmvgen_write_pbf(os.fsencode(path), len(nodes), _dptr(nodes), len(faces),
+                                          faces.ctypes.data_as(_I), nd,
+                                          _dptr(dp) if dp is not None else None), "write_pbf")
+
+
+def write_vtk(path, nodes, faces, sol):
+    """Solution per face as the reference's VTK output (hmvgen_write_vtk)."""
+    nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1, 3)
+    faces = np.ascontiguousarray(faces, np.int32).reshape(-1, 3)
+    sol = np.ascontiguousarray(sol, np.float64)
+    assert sol.shape == (len(faces),)
+    _check_gen(gen_lib().hmvgen_write_vtk(os.fsencode(path), len(nodes), _dptr(nodes), len(faces),
+                                          faces.ctypes.data_as(_I), _dptr(sol)), "write_vtk")
 
 
 def uniform(seed, n):

@@ -703,6 +703,96 @@ int hmvgen_read_pbf(const char *path, int *nnode, int *nface, double *nodes, int
   return 0;
 }
 
+namespace {
+
+// One value as the reference's mesh files print it (input_sample.pbf):
+// 12-digit mantissa and a signed three-digit exponent, e.g. -7.500000000000e-001.
+void put_e3(FILE *fp, double v) {
+  char buf[64];
+  std::snprintf(buf, sizeof(buf), "%.12e", v);
+  char *e = std::strchr(buf, 'e');
+  if (!e) {  // inf / nan
+    std::fputs(buf, fp);
+    return;
+  }
+  const int ex = std::atoi(e + 1);
+  *e = 0;
+  std::fprintf(fp, "%se%c%03d", buf, ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
+}
+
+}  // namespace
+
+int hmvgen_write_pbf(const char *path, int nnode, const double *nodes, int nface, const int *faces,
+                     int ndble, const double *dble_para) {
+  if (!path || nnode <= 0 || nface <= 0 || !nodes || !faces || ndble < 0 ||
+      (ndble > 0 && !dble_para))
+    return fail(-1, "write_pbf: invalid argument");
+  for (int64_t i = 0; i < 3 * (int64_t)nface; ++i)
+    if (faces[i] < 0 || faces[i] >= nnode) return fail(-1, "write_pbf: node index out of range");
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return fail(-1, std::string("write_pbf: cannot open ") + path);
+  // Read_model_data (bem-bb-fw-HACApK-0.4.2.f90:411-640) reads, list-directed:
+  // nond; nond coordinate triples; nofc, nodes per face (3), integer and
+  // double parameters per face; the faces' 0-based node triples; the
+  // integer parameters (none here); the double parameters, one face a line.
+  std::fprintf(fp, "%6d\n", nnode);
+  for (int i = 0; i < nnode; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      if (k) std::fputc('\t', fp);
+      put_e3(fp, nodes[3 * (size_t)i + k]);
+    }
+    std::fputc('\n', fp);
+  }
+  std::fprintf(fp, "%6d\n%6d\n%6d\n%6d\n", nface, 3, 0, ndble);
+  for (int i = 0; i < nface; ++i)
+    std::fprintf(fp, "%6d\t%6d\t%6d\n", faces[3 * (size_t)i], faces[3 * (size_t)i + 1],
+                 faces[3 * (size_t)i + 2]);
+  for (int i = 0; i < nface && ndble > 0; ++i) {
+    for (int k = 0; k < ndble; ++k) {
+      put_e3(fp, dble_para[(size_t)i * ndble + k]);
+      std::fputc('\t', fp);
+    }
+    std::fputc('\n', fp);
+  }
+  const bool ok = std::ferror(fp) == 0;
+  if (std::fclose(fp) != 0 || !ok) return fail(-1, std::string("write_pbf: write failed ") + path);
+  return 0;
+}
+
+int hmvgen_write_vtk(const char *path, int nnode, const double *nodes, int nface, const int *faces,
+                     const double *sol) {
+  if (!path || nnode <= 0 || nface <= 0 || !nodes || !faces || !sol)
+    return fail(-1, "write_vtk: invalid argument");
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) return fail(-1, std::string("write_vtk: cannot open ") + path);
+  // Print_result (bem-bb-fw-HACApK-0.4.2.f90:793-855): legacy ASCII
+  // unstructured grid, points as f16.13, triangles (cell type 5) with their
+  // 0-based node indices, the solution per face as f32.28.  The reference's
+  // list-directed lines (cell types, CELL_DATA, SCALARS, LOOKUP_TABLE) carry
+  // compiler-chosen blanks; here they are written with single spaces.
+  std::fprintf(fp, "# vtk DataFile Version 2.0\n");
+  std::fprintf(fp, "header : this data is results of BEM-BB\n");
+  std::fprintf(fp, "ASCII\n");
+  std::fprintf(fp, "DATASET UNSTRUCTURED_GRID\n");
+  std::fprintf(fp, "POINTS %d float\n", nnode);
+  for (int i = 0; i < nnode; ++i)
+    std::fprintf(fp, "%16.13f %16.13f %16.13f\n", nodes[3 * (size_t)i], nodes[3 * (size_t)i + 1],
+                 nodes[3 * (size_t)i + 2]);
+  std::fprintf(fp, "CELLS %d %lld\n", nface, 4LL * nface);
+  for (int i = 0; i < nface; ++i)
+    std::fprintf(fp, "3 %d %d %d\n", faces[3 * (size_t)i], faces[3 * (size_t)i + 1],
+                 faces[3 * (size_t)i + 2]);
+  std::fprintf(fp, "CELL_TYPES %d\n", nface);
+  for (int i = 0; i < nface; ++i) std::fprintf(fp, " 5\n");
+  std::fprintf(fp, " CELL_DATA %d\n", nface);
+  std::fprintf(fp, " SCALARS solve float 1\n");
+  std::fprintf(fp, " LOOKUP_TABLE default\n");
+  for (int i = 0; i < nface; ++i) std::fprintf(fp, "%32.28f\n", sol[i]);
+  const bool ok = std::ferror(fp) == 0;
+  if (std::fclose(fp) != 0 || !ok) return fail(-1, std::string("write_vtk: write failed ") + path);
+  return 0;
+}
+
 int hmvgen_build(int nnode, const double *nodes, int nface, const int *faces,
                  const hmvgen_params *pin, hmvgen_hmatrix **out) {
   if (!out) return fail(-1, "build: null output");

@@ -83,6 +83,17 @@ int hmvgen_icosphere(int subdiv, double r, double cx, double cy, double cz,
 int hmvgen_read_pbf(const char *path, int *nnode, int *nface,
                     double *nodes, int *faces, double *dble_para);
 
+/* Writes a mesh in the same format (read back by Read_model_data,
+ * bem-bb-fw-HACApK-0.4.2.f90:411-640): nodes nnode x 3, faces nface x 3
+ * (0-based), dble_para nface x ndble (face-major; NULL when ndble = 0). */
+int hmvgen_write_pbf(const char *path, int nnode, const double *nodes, int nface, const int *faces,
+                     int ndble, const double *dble_para);
+
+/* Writes the solution per face as a legacy ASCII VTK unstructured grid, the
+ * file Print_result writes (bem-bb-fw-HACApK-0.4.2.f90:793-855). */
+int hmvgen_write_vtk(const char *path, int nnode, const double *nodes, int nface, const int *faces,
+                     const double *sol);
+
 /* Builds the H-matrix for piecewise-constant collocation on the triangles. */
 int hmvgen_build(int nnode, const double *nodes, int nface, const int *faces,
                  const hmvgen_params *p, hmvgen_hmatrix **out);

@@ -8,7 +8,10 @@ data files (converted to .npz) and the figures its recorded run printed.
   input_1ts_mesh.npz     bem-bb_inputs/input_1ts.pbs  (502 nodes, 1,000 triangles)
   input_10ts_mesh.npz    bem-bb_inputs/input_10ts.pbs (5,202 nodes, 10,400 triangles)
   recorded_run_10ts.json figures of ivb/job_cpu.sh.o78632 and ivb/log0000.txt
+  input_sample_pbf.json  size and SHA-256 of input_sample.pbf with its CRLF line
+                         ends turned into LF (what a .pbf writer must reproduce)
 """
+import hashlib
 import json
 import os
 import re
@@ -34,6 +37,12 @@ def main():
         nodes, faces, para = H.read_pbf(os.path.join(REF, src))
         np.savez_compressed(os.path.join(HERE, out), nodes=nodes, faces=faces, dble_para=para)
         print("wrote", out, nodes.shape, faces.shape)
+
+    raw = open(os.path.join(REF, "input_sample.pbf"), "rb").read().replace(b"\r\n", b"\n")
+    with open(os.path.join(HERE, "input_sample_pbf.json"), "w") as f:
+        json.dump({"source": "input_sample.pbf, line ends CRLF -> LF", "bytes": len(raw),
+                   "sha256": hashlib.sha256(raw).hexdigest()}, f, indent=1)
+    print("wrote input_sample_pbf.json")
 
     log = open(os.path.join(REF, "ivb", "job_cpu.sh.o78632"), encoding="utf-8", errors="replace").read()
     log0 = open(os.path.join(REF, "ivb", "log0000.txt"), encoding="utf-8", errors="replace").read()
