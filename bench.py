@@ -327,6 +327,29 @@ def study_327680(H, log):
         "single_rhs_solves_per_s": round(1.0 / solves["uniform_seed2"]["fp64"]["seconds"], 2)}
     log("s=7 block BiCGSTAB k=32: %.3f s, %.1f RHS-solves/s (single: %.1f)"
         % (inf["seconds"], 32 / inf["seconds"], 1.0 / solves["uniform_seed2"]["fp64"]["seconds"]))
+    # the same H-matrix filled on the device, and its arena repacked there
+    mesh = H.icosphere(7)
+    t0 = time.perf_counter()
+    dh = H.DeviceHMatrix(*mesh, H.baseline_params(aca_eps=1e-5))
+    t1 = time.perf_counter()
+    ad = dh.arena("f64")
+    t2 = time.perf_counter()
+    xh = H.uniform(1, nd)
+    yd, yh = ad.matvec(xh), a64.matvec(xh)
+    out["device_construction"] = {
+        "blocks_host_s": round(dh.block_seconds, 3), "fill_device_s": round(dh.fill_seconds, 3),
+        "construct_total_s": round(t1 - t0, 3), "arena_from_device_factors_s": round(t2 - t1, 3),
+        "host_construction_s": out["construction_s"],
+        "host_arena_build_s": round(a64.stats["build_seconds"], 3),
+        "ranks_equal_host": bool(np.array_equal(dh.table, hm.leaf_table())),
+        "apply_rel_diff_vs_host_arena": float(np.linalg.norm(yd - yh) / np.linalg.norm(yh))}
+    log("s=7 device construction: blocks %.3f s (host), fill %.3f s (device), arena %.3f s; host: "
+        "construction %.3f s, arena %.3f s; ranks equal host: %s, apply diff %.1e"
+        % (dh.block_seconds, dh.fill_seconds, t2 - t1, out["construction_s"],
+           a64.stats["build_seconds"], out["device_construction"]["ranks_equal_host"],
+           out["device_construction"]["apply_rel_diff_vs_host_arena"]))
+    ad.free()
+    dh.free()
     for a in arenas.values():
         a.free()
     return out

@@ -21,6 +21,7 @@ Layers exposed here:
 """
 import ctypes as C
 import os
+import time
 
 import numpy as np
 
@@ -166,6 +167,10 @@ def gen_lib():
                                        _I, _I, _D, _I]
         L.hmvgen_read_pbf.argtypes = [C.c_char_p, _I, _I, _D, _I, _D]
         L.hmvgen_write_pbf.argtypes = [C.c_char_p, C.c_int, _D, C.c_int, _I, C.c_int, _D]
+        L.hmvgen_build_blocks.argtypes = [C.c_int, _D, C.c_int, _I, C.POINTER(_GenParams),
+                                          C.POINTER(C.c_void_p)]
+        L.hmvgen_areas.argtypes = [C.c_void_p]
+        L.hmvgen_areas.restype = _D
         L.hmvgen_write_vtk.argtypes = [C.c_char_p, C.c_int, _D, C.c_int, _I, _D]
         L.hmvgen_build.argtypes = [C.c_int, _D, C.c_int, _I, C.POINTER(_GenParams),
                                    C.POINTER(C.c_void_p)]
@@ -257,8 +262,16 @@ def hmv_lib():
         L.c_hacapk_bicgstab_cax_lfmtx_gpu_.restype = None
         L.c_hacapk_bicgstab_cax_lfmtx_pipe_.argtypes = L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes
         L.c_hacapk_bicgstab_cax_lfmtx_pipe_.restype = None
+        L.c_hacapk_bicgstab_cax_lfmtx_pipe2_.argtypes = L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes
+        L.c_hacapk_bicgstab_cax_lfmtx_pipe2_.restype = None
         L.hmv_bicgstab_pipe.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, _D,
                                         C.POINTER(SolveInfo)]
+        L.hmv_construct.argtypes = [C.c_int, _D, C.c_int, _I, _D, _D, _I, C.c_int, vp, C.c_int,
+                                    C.POINTER(_ConstructParams), C.POINTER(vp)]
+        L.hmv_dleaves_info.argtypes = [vp, _I, _I, _D]
+        L.hmv_dleaves_download.argtypes = [vp, C.c_int, _D]
+        L.hmv_arena_build_dleaves.argtypes = [vp, C.c_int, C.POINTER(Tuning), C.POINTER(vp)]
+        L.hmv_dleaves_free.argtypes = [vp]
         _hmv = L
     return _hmv
 
@@ -380,6 +393,120 @@ def baseline_params(aca_eps=1e-4, leaf_size=20, eta=2.0, kernel="centroid", nthr
     """BEM configuration of the H-matvec benchmarks: 1/(4 pi r) kernel with
     centroid quadrature off the diagonal, analytic self terms."""
     return HParams(leaf_size=leaf_size, eta=eta, aca_eps=aca_eps, kernel=kernel, nthreads=nthreads)
+
+
+class _ConstructParams(C.Structure):
+    _fields_ = [("kernel", C.c_int), ("kmax", C.c_int), ("kmin", C.c_int), ("pad", C.c_int),
+                ("aca_eps", C.c_double), ("aca_eps_factor", C.c_double)]
+
+
+class DeviceHMatrix:
+    """An H-matrix whose leaves are filled on the device (hmv_construct): the
+    host builds the cluster tree and the leaf list (hmvgen_build_blocks), the
+    device evaluates the dense leaves and runs ACA+ on the low-rank ones.
+    The factors stay in device memory; arena() repacks them without a host
+    copy, download() gives host leaves in the reference's layout."""
+
+    def __init__(self, nodes, faces, params=None):
+        L = gen_lib()
+        nodes = np.ascontiguousarray(nodes, np.float64)
+        faces = np.ascontiguousarray(faces, np.int32)
+        params = params or HParams()
+        self.params = params
+        t0 = time.perf_counter()
+        h = C.c_void_p()
+        _check_gen(L.hmvgen_build_blocks(nodes.shape[0], _dptr(nodes), faces.shape[0],
+                                         faces.ctypes.data_as(_I), C.byref(params._p), C.byref(h)),
+                   "hmvgen_build_blocks")
+        try:
+            st = _GenStats()
+            _check_gen(L.hmvgen_get_stats(h, C.byref(st)), "stats")
+            self.nd = st.nd
+            nlf, stride = C.c_int(), C.c_int()
+            specs = L.hmvgen_leaves(h, C.byref(nlf), C.byref(stride))
+            self.nlf = nlf.value
+            self.lod = np.ctypeslib.as_array(L.hmvgen_lod(h), shape=(self.nd,)).copy()
+            self.centroids = np.ctypeslib.as_array(L.hmvgen_centroids(h), shape=(self.nd, 3)).copy()
+            self.areas = np.ctypeslib.as_array(L.hmvgen_areas(h), shape=(self.nd,)).copy()
+            self.block_seconds = time.perf_counter() - t0
+            cp = _ConstructParams(kernel=params["kernel"], kmax=params["kmax"], kmin=params["kmin"],
+                                  aca_eps=params["aca_eps"], aca_eps_factor=params["aca_eps_factor"])
+            d = C.c_void_p()
+            lod = np.ascontiguousarray(self.lod, np.int32)
+            _check(hmv_lib().hmv_construct(nodes.shape[0], _dptr(nodes), faces.shape[0],
+                                           faces.ctypes.data_as(_I), _dptr(self.centroids),
+                                           _dptr(self.areas), lod.ctypes.data_as(_I), self.nlf,
+                                           specs, stride.value, C.byref(cp), C.byref(d)),
+                   "hmv_construct")
+            self._d = d
+            self.table = np.array([(r.ltmtx, 0, r.nstrtl, r.ndl, r.nstrtt, r.ndt)
+                                   for r in (LeafRec * self.nlf).from_address(specs)], np.int64)
+        finally:
+            L.hmvgen_free(h)
+        kt = np.zeros(self.nlf, np.int32)
+        sec = C.c_double()
+        _check(hmv_lib().hmv_dleaves_info(self._d, None, kt.ctypes.data_as(_I), C.byref(sec)),
+               "hmv_dleaves_info")
+        self.table[:, 1] = kt
+        self.kt = kt
+        self.fill_seconds = sec.value
+
+    def arena(self, precision="f64", tuning=None):
+        """Arena repacked from the device factors (hmv_arena_build_dleaves)."""
+        a = C.c_void_p()
+        t = Tuning.from_dict(tuning) if tuning else None
+        _check(hmv_lib().hmv_arena_build_dleaves(self._d, PREC[precision],
+                                                 C.byref(t) if t is not None else None, C.byref(a)),
+               "hmv_arena_build_dleaves")
+        ar = Arena.__new__(Arena)
+        ar.nd = self.nd
+        ar.precision = precision
+        ar._finish(a)
+        return ar
+
+    def download(self):
+        """Host copy of every leaf: an object with nd, nlf, leaves_ptr, stride
+        and leaf_table(), usable wherever an HMatrix's leaves are."""
+        return _HostLeaves(self)
+
+    def free(self):
+        if getattr(self, "_d", None):
+            hmv_lib().hmv_dleaves_free(self._d)
+            self._d = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class _HostLeaves:
+    def __init__(self, dh):
+        self.nd, self.nlf = dh.nd, dh.nlf
+        self._rec = (LeafRec * self.nlf)()
+        self._bufs = []
+        for i, (lt, kt, l0, ndl, t0, ndt) in enumerate(dh.table.tolist()):
+            n1 = ndt * kt if lt == 1 else ndt * ndl
+            n2 = ndl * kt if lt == 1 else 0
+            buf = np.zeros(max(1, n1 + n2))
+            if n1 + n2 > 0:
+                _check(hmv_lib().hmv_dleaves_download(dh._d, i, _dptr(buf)), "hmv_dleaves_download")
+            r = self._rec[i]
+            r.ltmtx, r.kt, r.nstrtl, r.ndl, r.nstrtt, r.ndt = lt, kt, l0, ndl, t0, ndt
+            r.a1size = 8 * n1
+            r.a1 = buf.ctypes.data_as(C.POINTER(C.c_double))
+            r.a2 = C.cast(buf.ctypes.data + 8 * n1, C.POINTER(C.c_double)) if lt == 1 else None
+            self._bufs.append(buf)
+        self.leaves_ptr = C.addressof(self._rec)
+        self.stride = C.sizeof(LeafRec)
+        self._table = dh.table.copy()
+
+    def leaves(self):
+        return self._rec
+
+    def leaf_table(self):
+        return self._table
 
 
 class HMatrix:
@@ -880,10 +1007,14 @@ def bicgstab_cax_lfmtx_gpu(leafmtxp, u, b, param):
     return _ref_solver(hmv_lib().c_hacapk_bicgstab_cax_lfmtx_gpu_, leafmtxp, u, b, param)
 
 
-def bicgstab_cax_lfmtx_pipe(leafmtxp, u, b, param):
-    """c_hacapk_bicgstab_cax_lfmtx_pipe_ (HACApK_SOLVER_PIPE.c:144), the
-    pipelined BiCGSTAB: u receives the solution.  Returns (nstp, lrtrn)."""
-    return _ref_solver(hmv_lib().c_hacapk_bicgstab_cax_lfmtx_pipe_, leafmtxp, u, b, param)
+def bicgstab_cax_lfmtx_pipe(leafmtxp, u, b, param, variant=1):
+    """c_hacapk_bicgstab_cax_lfmtx_pipe_ (HACApK_SOLVER_PIPE.c:144) or, with
+    variant=2, c_hacapk_bicgstab_cax_lfmtx_pipe2_ (:640, what the PIPE_BICG
+    driver calls, m_HACApK_use.f90:418): the pipelined BiCGSTAB; u receives
+    the solution.  Returns (nstp, lrtrn)."""
+    L = hmv_lib()
+    fn = L.c_hacapk_bicgstab_cax_lfmtx_pipe2_ if variant == 2 else L.c_hacapk_bicgstab_cax_lfmtx_pipe_
+    return _ref_solver(fn, leafmtxp, u, b, param)
 
 
 def solve(hm, rhs, sol0=None, ztol=1e-8, mstep=500, arena=None, solver="bicgstab", mreset=8):

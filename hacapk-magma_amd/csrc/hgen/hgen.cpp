@@ -581,6 +581,11 @@ struct hmvgen_hmatrix {
   std::vector<std::unique_ptr<double[]>> store;
 };
 
+namespace {
+int build_impl(int nnode, const double *nodes, int nface, const int *faces,
+               const hmvgen_params *pin, hmvgen_hmatrix **out, bool fill);
+}
+
 extern "C" {
 
 const char *hmvgen_last_error(void) { return g_err.c_str(); }
@@ -795,6 +800,19 @@ int hmvgen_write_vtk(const char *path, int nnode, const double *nodes, int nface
 
 int hmvgen_build(int nnode, const double *nodes, int nface, const int *faces,
                  const hmvgen_params *pin, hmvgen_hmatrix **out) {
+  return build_impl(nnode, nodes, nface, faces, pin, out, true);
+}
+
+int hmvgen_build_blocks(int nnode, const double *nodes, int nface, const int *faces,
+                        const hmvgen_params *pin, hmvgen_hmatrix **out) {
+  return build_impl(nnode, nodes, nface, faces, pin, out, false);
+}
+
+}  // extern "C"
+
+namespace {
+int build_impl(int nnode, const double *nodes, int nface, const int *faces,
+               const hmvgen_params *pin, hmvgen_hmatrix **out, bool fill) {
   if (!out) return fail(-1, "build: null output");
   *out = nullptr;
   if (nnode <= 0 || nface <= 0 || !nodes || !faces) return fail(-1, "build: empty mesh");
@@ -886,6 +904,13 @@ int hmvgen_build(int nnode, const double *nodes, int nface, const int *faces,
     lr.ndl = ls.ndl;
     lr.nstrtt = ls.nstrtt;
     lr.ndt = ls.ndt;
+    if (!fill) {  // block structure only: no factors
+      lr.kt = 0;
+      lr.a1 = nullptr;
+      lr.a2 = nullptr;
+      lr.a1size = 0;
+      continue;
+    }
     if (ls.ltmtx == 1) {
       Aca aca;
       aca.bem = &bem;
@@ -966,8 +991,13 @@ int hmvgen_build(int nnode, const double *nodes, int nface, const int *faces,
   *out = h.release();
   return 0;
 }
+}  // namespace
+
+extern "C" {
 
 void hmvgen_free(hmvgen_hmatrix *h) { delete h; }
+
+const double *hmvgen_areas(const hmvgen_hmatrix *h) { return h ? h->bem.area.data() : nullptr; }
 
 int hmvgen_get_stats(const hmvgen_hmatrix *h, hmvgen_stats *s) {
   if (!h || !s) return fail(-1, "get_stats: null argument");

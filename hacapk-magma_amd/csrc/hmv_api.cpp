@@ -140,6 +140,31 @@ int build(int nd, int nlf, const void *leaves, int stride, int prec, const hmv::
   return upload_plan(P, out, t0);
 }
 
+}  // namespace
+
+namespace hmv {
+int build_from_device_leaves(int nd, int nlf, const void *leaves, int stride, int prec,
+                             const hmv_tuning *tuning, hmv_arena *out) {
+  if (!out) {
+    set_error("hmv_arena_build_dleaves: null output handle");
+    return HMV_ERR_ARG;
+  }
+  *out = nullptr;
+  int rc = check_device();
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  hmv::Plan P;
+  std::string err;
+  if (!hmv::build_plan_device(nd, nlf, leaves, stride, prec, from_public(prec, tuning), P, err)) {
+    set_error("hmv_arena_build_dleaves: " + err);
+    return HMV_ERR_ARG;
+  }
+  return upload_plan(P, out, t0);
+}
+}  // namespace hmv
+
+namespace {
+
 // Uploads a plan (built here or read from a file) as a device arena.
 int upload_plan(hmv::Plan &P, hmv_arena *out, std::chrono::steady_clock::time_point t0) {
   const int nd = P.nd, prec = P.prec;
@@ -157,7 +182,9 @@ int upload_plan(hmv::Plan &P, hmv_arena *out, std::chrono::steady_clock::time_po
   a->ntfin = (int)P.tfin.size();
   hipError_t e;
   if ((e = hipMalloc(&a->d_arena, P.arena_size)) != hipSuccess ||
-      (e = hipMemcpy(a->d_arena, P.arena.get(), P.arena_size, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = P.arena ? hipMemcpy(a->d_arena, P.arena.get(), P.arena_size, hipMemcpyHostToDevice)
+                   : hipMemset(a->d_arena, 0, P.arena_size)) != hipSuccess ||
+      (e = P.arena ? hipSuccess : hmv::launch_fill(P.fill, a->d_arena)) != hipSuccess ||
       (e = upload(&a->d_work1, P.work1)) != hipSuccess ||
       (e = upload(&a->d_work2, P.work2)) != hipSuccess ||
       (e = upload(&a->d_tfin, P.tfin)) != hipSuccess ||
@@ -941,6 +968,13 @@ void c_hacapk_bicgstab_cax_lfmtx_pipe_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C,
                                        double *b_cpu, double *param, int *nd, int *nstp,
                                        int *lrtrn) {
   ref_solve("c_hacapk_bicgstab_cax_lfmtx_pipe_", "pipelined BiCGSTAB", hmv_bicgstab_pipe, H, C,
+            u_cpu, b_cpu, param, nd, nstp, lrtrn);
+}
+
+void c_hacapk_bicgstab_cax_lfmtx_pipe2_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C, double *u_cpu,
+                                        double *b_cpu, double *param, int *nd, int *nstp,
+                                        int *lrtrn) {
+  ref_solve("c_hacapk_bicgstab_cax_lfmtx_pipe2_", "pipelined BiCGSTAB", hmv_bicgstab_pipe, H, C,
             u_cpu, b_cpu, param, nd, nstp, lrtrn);
 }
 

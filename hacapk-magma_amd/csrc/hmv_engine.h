@@ -58,6 +58,11 @@ void free_solve_work(hmv_arena_s *a);
 // Enqueues Y = H X + beta*Y for k interleaved right-hand sides (1..128).
 hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
                                const int *skip, hipStream_t s);
+// Runs an arena's fill records on the device (image zeroed beforehand).
+hipError_t launch_fill(const std::vector<HmvFillOp> &ops, char *arena);
+// Arena from leaves whose factors are in device memory (hmv_construct).
+int build_from_device_leaves(int nd, int nlf, const void *leaves, int stride, int prec,
+                             const hmv_tuning *tuning, hmv_arena *out);
 // Copies an arena's device tables back into a host plan (for saving).
 bool download_plan(const hmv_arena_s *a, Plan &P, std::string &err);
 // Makes the H-matmat t buffer hold k right-hand sides.

@@ -107,4 +107,14 @@ struct HmvZSeg {
   int col;    // first column of the run within its work item
 };
 
+// One copy of the arena fill when the leaf factors live in device memory:
+// dst part (column-major, ld rows) at byte offset `dst`, rows r0.. and
+// columns c0.. of it, from src[r * s_r + c * s_c] (doubles), converted to dt.
+struct HmvFillOp {
+  long long dst;
+  const double *src;
+  long long s_r, s_c;
+  int ld, r0, c0, nrows, ncols, dt;
+};
+
 #endif

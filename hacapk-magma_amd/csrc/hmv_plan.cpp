@@ -85,8 +85,24 @@ PlanTuning default_tuning(int prec) {
   return t;
 }
 
+namespace {
+bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const PlanTuning &tune,
+               Plan &P, std::string &err, bool device);
+}
+
 bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
                 const PlanTuning &tune, Plan &P, std::string &err) {
+  return plan_impl(nd, nlf, leaves, stride, prec, tune, P, err, false);
+}
+
+bool build_plan_device(int nd, int nlf, const void *leaves, int stride, int prec,
+                       const PlanTuning &tune, Plan &P, std::string &err) {
+  return plan_impl(nd, nlf, leaves, stride, prec, tune, P, err, true);
+}
+
+namespace {
+bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const PlanTuning &tune,
+               Plan &P, std::string &err, bool device) {
   P = Plan();
   auto nt_ok = [](int v) { return v == -1 || v == 64 || v == 128; };
   if (!nt_ok(tune.p1_small_threads) || !nt_ok(tune.p2_small_threads)) {
@@ -397,17 +413,20 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   st.arena_bytes = std::max<int64_t>(off, HMV_ARENA_ALIGN);
   st.payload_bytes = st.entries_lr * es_l + st.entries_dense * es_d;
   st.alg_bytes = st.payload_bytes + 16LL * nd;
-  try {
-    // left uninitialised: every block (with its padding) is zeroed by the
-    // thread that fills it, so the image is first touched in parallel
-    P.arena.reset(new char[(size_t)st.arena_bytes]);
-    P.arena_size = (size_t)st.arena_bytes;
-  } catch (...) {
-    err = "out of host memory for the arena image";
-    return false;
+  P.arena_size = (size_t)st.arena_bytes;
+  char *A = nullptr;
+  if (!device) {
+    try {
+      // left uninitialised: every block (with its padding) is zeroed by the
+      // thread that fills it, so the image is first touched in parallel
+      P.arena.reset(new char[(size_t)st.arena_bytes]);
+    } catch (...) {
+      err = "out of host memory for the arena image";
+      return false;
+    }
+    A = P.arena.get();
+    if (off < st.arena_bytes) std::memset(A + off, 0, (size_t)(st.arena_bytes - off));
   }
-  char *A = P.arena.get();
-  if (off < st.arena_bytes) std::memset(A + off, 0, (size_t)(st.arena_bytes - off));
   lap("layout");
 
   // ---- tables -------------------------------------------------------------
@@ -504,6 +523,51 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   lap("tables");
 
   // ---- fill ---------------------------------------------------------------
+  if (device) {
+    // the same copies as the host fill below, as records for the device
+    // (the image is zeroed there first)
+    for (const Piece &p : pieces) {
+      const int rows_p = (int)round_up(p.nr, vr);
+      for (int r = 0; r < p.nr; ++r) {
+        const auto &rv = cg_rows[p.c][p.r0 + r];
+        const RefLeaf *l = L[rv.first];
+        P.fill.push_back(HmvFillOp{p.off, l->a1 + (size_t)rv.second * l->ndt + p.c0, 0, 1, rows_p, r,
+                                   0, 1, p.cw, dt_lr});
+      }
+    }
+    for (const Slice &S : slices) {
+      const int rows = S.b - S.a;
+      const int rows_p = (int)round_up(rows, vr2);
+      int c0 = 0, c1 = 0;
+      auto put_dense = [&](const RowGroup &g, int64_t off, int dt, int &col) {
+        for (int i : g.dense) {
+          const RefLeaf *l = L[i];
+          P.fill.push_back(HmvFillOp{off, l->a1 + (size_t)(S.a - g.s) * l->ndt, l->ndt, 1, rows_p, 0,
+                                     col, rows, l->ndt, dt});
+          col += l->ndt;
+        }
+      };
+      auto put_u = [&](const RowGroup &g, int64_t off, int dt, int &col) {
+        for (int i : g.lr) {
+          const RefLeaf *l = L[i];
+          if (l->kt > 0)
+            P.fill.push_back(HmvFillOp{off, u_factor(l) + (S.a - g.s), 1, l->ndl, rows_p, 0, col,
+                                       rows, l->kt, dt});
+          col += l->kt;
+        }
+      };
+      for (int gi : S.cov) {
+        const RowGroup &g = rg[gi];
+        if (merged) {
+          put_dense(g, S.off0, dt_dense, c0);
+          put_u(g, S.off0, dt_lr, c0);
+        } else {
+          put_dense(g, S.off0, dt_dense, c0);
+          put_u(g, S.off1, dt_lr, c1);
+        }
+      }
+    }
+  } else {
   parallel_for((int64_t)pieces.size(), [&](int64_t pi) {
     const Piece &p = pieces[pi];
     char *dst = A + p.off;
@@ -558,6 +622,7 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
       }
     }
   });
+  }
   lap("fill");
 
   // Dispatch order.  The hardware sends workgroup b to XCD b % G; give each
@@ -614,5 +679,6 @@ bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
   lap("order");
   return true;
 }
+}  // namespace
 
 }  // namespace hmv

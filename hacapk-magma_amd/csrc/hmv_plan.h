@@ -63,6 +63,9 @@ struct Plan {
     int p1_row_tile = 0, p2_max_rows = 0, p1_small_threads = 0, p2_small_threads = 0;
   } tune;
   std::vector<int> seg;            // instance row offsets (ninst + 1): {0, nd} for one H-matrix
+  // device-fill plans (leaf factors in device memory): no host image; the
+  // copies that make it, run on the device at upload
+  std::vector<HmvFillOp> fill;
 };
 
 // Tunables of the repack (chosen by the autotuner or left at defaults).
@@ -82,6 +85,10 @@ PlanTuning default_tuning(int prec);
 // Returns false and fills `err` on invalid input.
 bool build_plan(int nd, int nlf, const void *leaves, int stride, int prec,
                 const PlanTuning &tune, Plan &out, std::string &err);
+// Same, for leaves whose a1/a2 point to device memory: the arena image is
+// not built on the host; out.fill receives the copies that build it.
+bool build_plan_device(int nd, int nlf, const void *leaves, int stride, int prec,
+                       const PlanTuning &tune, Plan &out, std::string &err);
 
 // Checks every index of a plan against the buffers it addresses (work
 // blocks inside the arena image, input runs, codes and outputs inside x, t

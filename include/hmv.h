@@ -300,6 +300,41 @@ int hmv_plan_create_tuned(int nd, int nlf, const void *leaves, int leaf_stride, 
 int hmv_plan_view(hmv_plan p, hmv_plan_tables *t);
 int hmv_plan_free(hmv_plan p);
 
+/* ---- device construction -----------------------------------------------
+ * The fill of HACApK_generate on the device (m_HACApK_base.f90:1005-1051):
+ * every dense leaf's entries (HACApK_entry_ij) and every low-rank leaf's
+ * ACA+ factors (HACApK_acaplus, :783-975, param(61) = 1), one workgroup per
+ * leaf.  The block structure comes from the host (hmvgen_build_blocks: tree,
+ * admissibility, leaf list; specs are its leaf records, kt and factors
+ * ignored).  Inputs are host arrays: the mesh (nodes nnode x 3, faces
+ * nface x 3, 0-based), centroids and areas per face (original order), lod
+ * (cluster order -> 1-based original index).  kernel: 0 = the ppohBEM
+ * analytic integral for every pair, 1 = centroid 1/(4 pi r) off the diagonal
+ * (as hmvgen's HMVGEN_KERNEL_*).  The factors stay in device memory. */
+typedef struct {
+  int kernel;
+  int kmax;              /* param(63) */
+  int kmin;              /* param(64) */
+  int pad;
+  double aca_eps;        /* param(71) */
+  double aca_eps_factor; /* param(72) */
+} hmv_construct_params;
+typedef struct hmv_dleaves_s *hmv_dleaves;
+int hmv_construct(int nnode, const double *nodes, int nface, const int *faces, const double *cent,
+                  const double *area, const int *lod, int nlf, const void *specs, int stride,
+                  const hmv_construct_params *p, hmv_dleaves *out);
+/* nlf, the ranks per leaf (nlf entries; 0 for dense leaves; may be NULL) and
+ * the device fill's wall time in seconds. */
+int hmv_dleaves_info(hmv_dleaves d, int *nlf, int *kt, double *seconds);
+/* One leaf's factors to host memory in the reference's layout: dense
+ * ndt x ndl (a1), low rank V (ndt x kt) followed by U (ndl x kt). */
+int hmv_dleaves_download(hmv_dleaves d, int leaf, double *a1);
+/* Arena straight from the device factors: the repack runs on the host over
+ * the leaf shapes and ranks, the copies into the arena image on the device. */
+int hmv_arena_build_dleaves(hmv_dleaves d, int precision, const hmv_tuning *tuning,
+                            hmv_arena *out);
+int hmv_dleaves_free(hmv_dleaves d);
+
 /* ---- arena files -------------------------------------------------------
  * A versioned binary file of the repack (arena image and tables, format
  * version HMV_ARENA_FILE_VERSION, checksummed sections), so that later runs
@@ -359,6 +394,13 @@ void c_hacapk_bicgstab_cax_lfmtx_gpu_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lco
 void c_hacapk_bicgstab_cax_lfmtx_pipe_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lcontrol *st_ctl,
                                        double *u_cpu, double *b_cpu, double *param, int *nd,
                                        int *nstp, int *lrtrn);
+/* Replaces c_hacapk_bicgstab_cax_lfmtx_pipe2_ (HACApK_SOLVER_PIPE.c:640), the
+ * variant the PIPE_BICG driver calls (m_HACApK_use.f90:418): the same
+ * recurrences with the MPI all-gather overlapped with the dot products,
+ * which one GPU does not have; it runs hmv_bicgstab_pipe. */
+void c_hacapk_bicgstab_cax_lfmtx_pipe2_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lcontrol *st_ctl,
+                                        double *u_cpu, double *b_cpu, double *param, int *nd,
+                                        int *nstp, int *lrtrn);
 
 #ifdef __cplusplus
 }

@@ -98,7 +98,16 @@ int hmvgen_write_vtk(const char *path, int nnode, const double *nodes, int nface
 int hmvgen_build(int nnode, const double *nodes, int nface, const int *faces,
                  const hmvgen_params *p, hmvgen_hmatrix **out);
 
+/* The tree and the leaf list only (HACApK_generate without the fill): every
+ * leaf record has kt = 0 and no factors.  For a fill done elsewhere (the
+ * engine's device construction, hmv_construct). */
+int hmvgen_build_blocks(int nnode, const double *nodes, int nface, const int *faces,
+                        const hmvgen_params *p, hmvgen_hmatrix **out);
+
 void hmvgen_free(hmvgen_hmatrix *h);
+
+/* Face areas (original order), as the entry function uses them. */
+const double *hmvgen_areas(const hmvgen_hmatrix *h);
 
 int  hmvgen_get_stats(const hmvgen_hmatrix *h, hmvgen_stats *s);
 

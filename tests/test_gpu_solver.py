@@ -143,7 +143,8 @@ def test_pipelined_bicgstab_matches_oracle(gpu, O, sphere4):
     a.free()
 
 
-def test_reference_named_pipelined_bicgstab(gpu, O, sphere3):
+@pytest.mark.parametrize("variant", [1, 2])
+def test_reference_named_pipelined_bicgstab(gpu, O, sphere3, variant):
     hm = sphere3
     lp = gpu.make_leafmtxp(hm)
     param = np.zeros(100)
@@ -151,7 +152,7 @@ def test_reference_named_pipelined_bicgstab(gpu, O, sphere3):
     param[90] = 1e-8
     b = np.ones(hm.nd)
     u = np.zeros(hm.nd)
-    nstp, lrtrn = gpu.bicgstab_cax_lfmtx_pipe(lp, u, b, param)
+    nstp, lrtrn = gpu.bicgstab_cax_lfmtx_pipe(lp, u, b, param, variant=variant)
     A = O.dense(hm)
     assert lrtrn == 0 and nstp > 0
     assert np.linalg.norm(A @ u - b) / np.linalg.norm(b) < 5e-8
