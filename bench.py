@@ -64,10 +64,17 @@ def main():
                     help="only the FP64 H-matmat (k = 8, 32, 128) on 327,680 unknowns (for profiler runs)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 327,680-unknown precision / multi-RHS study and the 48-instance batch")
+    ap.add_argument("--construct", action="store_true",
+                    help="only the device H-matrix fill and arena repack on 327,680 unknowns "
+                         "(for profiler runs)")
     args = ap.parse_args()
     if args.hmatmat:
         import __graft_entry__ as G
         print(json.dumps({"hmatmat_fp64_327680": hmatmat_only(G.load_package(), log)}), flush=True)
+        return
+    if args.construct:
+        import __graft_entry__ as G
+        print(json.dumps({"construct_327680": construct_only(G.load_package(), log)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -372,6 +379,28 @@ def hmatmat_only(H, log, reps=5):
         X.free()
         Y.free()
     a.free()
+    return out
+
+
+def construct_only(H, log, reps=2):
+    """Device fill (hmv_construct) and arena repack of one icosphere s=7,
+    ACA+ 1e-5, centroid kernel, `reps` times."""
+    if H.hmv_lib().hmv_init(0) != 0:
+        raise SystemExit("no GPU: " + H.hmv_lib().hmv_last_error().decode())
+    mesh = H.icosphere(7)
+    out = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        dh = H.DeviceHMatrix(*mesh, H.baseline_params(aca_eps=1e-5))
+        t1 = time.perf_counter()
+        a = dh.arena("f64")
+        t2 = time.perf_counter()
+        out.append({"blocks_host_s": round(dh.block_seconds, 3), "fill_device_s": round(dh.fill_seconds, 3),
+                    "arena_s": round(t2 - t1, 3), "total_s": round(t2 - t0, 3)})
+        log("construct: blocks %.3f s (host), fill %.3f s (device), arena %.3f s"
+            % (dh.block_seconds, dh.fill_seconds, t2 - t1))
+        a.free()
+        dh.free()
     return out
 
 

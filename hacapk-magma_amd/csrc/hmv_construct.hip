@@ -161,21 +161,22 @@ __global__ __launch_bounds__(kT) void k_dense(const CLeaf *__restrict__ lv,
   }
 }
 
-// Workgroup helpers.  Every thread calls them with the same arguments; they
-// end with the result visible to all threads.
-struct Red {
-  double v[kT / 64];
-  int i[kT / 64];
-  double bv;
-  int bi;
-};
+// ACA+ runs one wave per block: the block's scalar steps are sequential, so
+// a wave keeps them cheap (reductions by lane shuffles, no workgroup
+// barrier between waves) and many blocks run at once.
+constexpr int kA = 64;
+
+// Helpers called by every lane with the same arguments; each returns the
+// same value in every lane, and ends with the lanes' writes visible.
+
+__device__ inline void wave_sync() { __syncthreads(); }
 
 // First index (1-based) of the largest (want_max) or smallest |vec|, as
 // HACApK_maxabsvalloc_d / HACApK_minabsvalloc_d scan.
-__device__ void abs_loc(const double *vec, int n, bool want_max, Red &r, double &val, int &idx) {
+__device__ void abs_loc(const double *vec, int n, bool want_max, double &val, int &idx) {
   double bv = 0.0;
   int bi = n;  // none
-  for (int a = threadIdx.x; a < n; a += kT) {
+  for (int a = threadIdx.x; a < n; a += kA) {
     const double x = fabs(vec[a]);
     if (bi == n || (want_max ? x > bv : x < bv)) {
       bv = x;
@@ -185,57 +186,60 @@ __device__ void abs_loc(const double *vec, int n, bool want_max, Red &r, double 
   for (int off = 32; off > 0; off >>= 1) {
     const double ov = __shfl_xor(bv, off, 64);
     const int oi = __shfl_xor(bi, off, 64);
-    const bool take = oi != n && (bi == n || (want_max ? ov > bv : ov < bv) || (ov == bv && oi < bi));
+    const bool take =
+        oi != n && (bi == n || (want_max ? ov > bv : ov < bv) || (ov == bv && oi < bi));
     if (take) {
       bv = ov;
       bi = oi;
     }
   }
-  if ((threadIdx.x & 63) == 0) {
-    r.v[threadIdx.x >> 6] = bv;
-    r.i[threadIdx.x >> 6] = bi;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double v0 = r.v[0];
-    int i0 = r.i[0];
-    for (int w = 1; w < kT / 64; ++w) {
-      const double ov = r.v[w];
-      const int oi = r.i[w];
-      if (oi != n && (i0 == n || (want_max ? ov > v0 : ov < v0) || (ov == v0 && oi < i0))) {
-        v0 = ov;
-        i0 = oi;
-      }
-    }
-    r.bv = v0;
-    r.bi = i0;
-  }
-  __syncthreads();
-  val = r.bv;
-  idx = r.bi + 1;
-  __syncthreads();
+  val = bv;
+  idx = bi + 1;
 }
 
-// HACApK_unrm_d, sequential (its running scale depends on the order).
-__device__ double unrm(const double *za, int nd, Red &r) {
-  if (threadIdx.x == 0) {
-    double zscl = 0.0, zzz = 1.0;
-    for (int il = 0; il < nd; ++il) {
-      const double zza = fabs(za[il]);
-      if (zza < 1.0e-30) continue;
-      if (zscl < zza) {
-        zzz = 1.0 + zzz * (zscl / zza) * (zscl / zza);
-        zscl = zza;
-      } else {
-        zzz = zzz + (zza / zscl) * (zza / zscl);
-      }
+// HACApK_unrm_d, bit for bit.  Its running scale zscl is the prefix maximum
+// of the |entries| (those below 1e-30 skipped), so every entry's ratio
+// (zscl/zza or zza/zscl) comes from a wave-wide prefix-max scan and is formed
+// by the entry's own lane; only the accumulation of zzz, whose rounding order
+// matters, runs entry by entry (uniformly in every lane).
+__device__ inline double lane_double(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ double unrm(const double *za, int nd) {
+  const int lane = threadIdx.x;
+  double zscl = 0.0, zzz = 1.0;  // same value in every lane
+  for (int c0 = 0; c0 < nd; c0 += 64) {
+    const int i = c0 + lane;
+    const double a = i < nd ? fabs(za[i]) : 0.0;
+    const bool skip = i >= nd || a < 1.0e-30;
+    double incl = skip ? 0.0 : a;  // inclusive prefix max within the chunk
+    for (int off = 1; off < 64; off <<= 1) {
+      const double o = __shfl_up(incl, off, 64);
+      if (lane >= off && o > incl) incl = o;
     }
-    r.bv = zscl * sqrt(zzz);
+    double excl = __shfl_up(incl, 1, 64);
+    if (lane == 0) excl = 0.0;
+    const double m = excl > zscl ? excl : zscl;  // zscl before entry i
+    const bool isnew = !skip && m < a;
+    const double q = skip ? 0.0 : (isnew ? m / a : a / m);
+    const unsigned long long newm = __ballot(isnew), skipm = __ballot(skip);
+    const int nl = min(64, nd - c0);
+    for (int l = 0; l < nl; ++l) {
+      if ((skipm >> l) & 1ull) continue;
+      const double ql = lane_double(q, l);
+      if ((newm >> l) & 1ull)
+        zzz = 1.0 + zzz * ql * ql;
+      else
+        zzz = zzz + ql * ql;
+    }
+    const double cm = lane_double(incl, nl - 1);
+    if (cm > zscl) zscl = cm;
   }
-  __syncthreads();
-  const double v = r.bv;
-  __syncthreads();
-  return v;
+  return zscl * sqrt(zzz);
 }
 
 // Residual column (col = true: ndl entries of column ip) or row (ndt entries
@@ -246,7 +250,7 @@ __device__ void calc_vec(const CLeaf &L, const Geo &g, bool col, int k, int ip, 
                          const double *U, const double *V) {
   const int n = col ? L.ndl : L.ndt;
   const int *m = col ? L.msk : L.msk + L.ndl;
-  for (int a = threadIdx.x; a < n; a += kT) {
+  for (int a = threadIdx.x; a < n; a += kA) {
     double e;
     if (m[a] == 0)
       e = col ? entry(g, g.lod[a + L.nstrtl - 1], g.lod[ip + L.nstrtt - 2])
@@ -265,36 +269,35 @@ __device__ void calc_vec(const CLeaf &L, const Geo &g, bool col, int k, int ip, 
     }
     vec[a] = e;
   }
-  __syncthreads();
+  wave_sync();
 }
 
 __device__ void copy_vec(double *dst, const double *src, int n) {
-  for (int a = threadIdx.x; a < n; a += kT) dst[a] = src[a];
-  __syncthreads();
+  for (int a = threadIdx.x; a < n; a += kA) dst[a] = src[a];
+  wave_sync();
 }
 
 __device__ void scale_vec(double *v, int n, double s) {
-  for (int a = threadIdx.x; a < n; a += kT) v[a] = v[a] * s;
-  __syncthreads();
+  for (int a = threadIdx.x; a < n; a += kA) v[a] = v[a] * s;
+  wave_sync();
 }
 
 // y += x * s
 __device__ void axpy_vec(double *y, const double *x, int n, double s) {
-  for (int a = threadIdx.x; a < n; a += kT) y[a] = y[a] + x[a] * s;
-  __syncthreads();
+  for (int a = threadIdx.x; a < n; a += kA) y[a] = y[a] + x[a] * s;
+  wave_sync();
 }
 
 __device__ void set_mask(int *m, int i1) {
   if (threadIdx.x == 0) m[i1 - 1] = 1;
-  __syncthreads();
+  wave_sync();
 }
 
 // HACApK_acaplus (m_HACApK_base.f90:783-975) with param(61) = 1, for one
 // block per workgroup; restates the host fill's ACA+ step for step.
-__global__ __launch_bounds__(kT) void k_aca(const CLeaf *__restrict__ lv,
+__global__ __launch_bounds__(kA) void k_aca(const CLeaf *__restrict__ lv,
                                             const int *__restrict__ ids, Geo g, AcaParams p,
                                             int *__restrict__ kt_out, int *__restrict__ status) {
-  __shared__ Red red;
   const int id = ids[blockIdx.x];
   const CLeaf L = lv[id];
   const int ndl = L.ndl, ndt = L.ndt, cap = L.cap;
@@ -305,9 +308,9 @@ __global__ __launch_bounds__(kT) void k_aca(const CLeaf *__restrict__ lv,
   auto ucol = [&](int k1) { return U + (size_t)(k1 - 1) * ndl; };
   auto vcol = [&](int k1) { return V + (size_t)(k1 - 1) * ndt; };
 
-  for (int a = threadIdx.x; a < ndl + ndt; a += kT) L.msk[a] = 0;
-  for (int a = threadIdx.x; a < ndl + ndt; a += kT) L.pa[a] = 0.0;
-  __syncthreads();
+  for (int a = threadIdx.x; a < ndl + ndt; a += kA) L.msk[a] = 0;
+  for (int a = threadIdx.x; a < ndl + ndt; a += kA) L.pa[a] = 0.0;
+  wave_sync();
 
   const double eps = p.eps;
   const double ACA_EPS = p.eps_factor * eps;
@@ -319,12 +322,12 @@ __global__ __launch_bounds__(kT) void k_aca(const CLeaf *__restrict__ lv,
 
   int j_ref = 1;
   calc_vec(L, g, true, k, j_ref, pa_ref, U, V);
-  double colnorm = unrm(pa_ref, ndl, red);
+  double colnorm = unrm(pa_ref, ndl);
   double rownorm;
   int i_ref;
-  abs_loc(pa_ref, ndl, false, red, rownorm, i_ref);
+  abs_loc(pa_ref, ndl, false, rownorm, i_ref);
   calc_vec(L, g, false, k, i_ref, pb_ref, U, V);
-  rownorm = unrm(pb_ref, ndt, red);
+  rownorm = unrm(pb_ref, ndt);
 
   double apxnorm = 0.0;
   int lstop_aca = 0;
@@ -338,21 +341,21 @@ __global__ __launch_bounds__(kT) void k_aca(const CLeaf *__restrict__ lv,
     }
     double *pcol = ucol(k + 1);
     double *prow = vcol(k + 1);
-    abs_loc(pa_ref, ndl, true, red, col_maxval, i);
-    abs_loc(pb_ref, ndt, true, red, row_maxval, j);
+    abs_loc(pa_ref, ndl, true, col_maxval, i);
+    abs_loc(pb_ref, ndt, true, row_maxval, j);
 
     if (row_maxval > col_maxval) {
       if (j != j_ref)
         calc_vec(L, g, true, k, j, pcol, U, V);
       else
         copy_vec(pcol, pa_ref, ndl);
-      abs_loc(pcol, ndl, true, red, col_maxval, i);
+      abs_loc(pcol, ndl, true, col_maxval, i);
       if (col_maxval < ACA_EPS && k >= p.kmin) {
         lstop_aca = 1;
       } else {
         calc_vec(L, g, false, k, i, prow, U, V);
         const double piv = pcol[i - 1];
-        __syncthreads();  // every thread has the pivot before the column is scaled
+        wave_sync();  // every lane has the pivot before the column is scaled
         if (fabs(piv) > piv_min) {
           scale_vec(pcol, ndl, 1.0 / piv);
         } else {
@@ -365,13 +368,13 @@ __global__ __launch_bounds__(kT) void k_aca(const CLeaf *__restrict__ lv,
         calc_vec(L, g, false, k, i, prow, U, V);
       else
         copy_vec(prow, pb_ref, ndt);
-      abs_loc(prow, ndt, true, red, row_maxval, j);
+      abs_loc(prow, ndt, true, row_maxval, j);
       if (row_maxval < ACA_EPS && k >= p.kmin) {
         lstop_aca = 1;
       } else {
         calc_vec(L, g, true, k, j, pcol, U, V);
         const double piv = prow[j - 1];
-        __syncthreads();  // every thread has the pivot before the row is scaled
+        wave_sync();  // every lane has the pivot before the row is scaled
         if (fabs(piv) > piv_min) {
           scale_vec(prow, ndt, 1.0 / piv);
         } else {
@@ -386,7 +389,7 @@ __global__ __launch_bounds__(kT) void k_aca(const CLeaf *__restrict__ lv,
     if (i != i_ref) {
       const double zinv = -pcol[i_ref - 1];
       axpy_vec(pb_ref, prow, ndt, zinv);
-      rownorm = unrm(pb_ref, ndt, red);
+      rownorm = unrm(pb_ref, ndt);
     }
     if (i == i_ref || rownorm < ACA_EPS) {
       if (i == i_ref) ntries_row = ntries_row + 1;
@@ -400,7 +403,7 @@ __global__ __launch_bounds__(kT) void k_aca(const CLeaf *__restrict__ lv,
               break;
             }
             calc_vec(L, g, false, k + 1, i, pb_ref, U, V);
-            rownorm = unrm(pb_ref, ndt, red);
+            rownorm = unrm(pb_ref, ndt);
             if (rownorm < ACA_EPS) set_mask(lrow_msk, i);
             ntries_row = ntries_row - 1;
           } else {
@@ -416,7 +419,7 @@ __global__ __launch_bounds__(kT) void k_aca(const CLeaf *__restrict__ lv,
     if (j != j_ref) {
       const double zinv = -prow[j_ref - 1];
       axpy_vec(pa_ref, pcol, ndl, zinv);
-      colnorm = unrm(pa_ref, ndl, red);
+      colnorm = unrm(pa_ref, ndl);
     }
     if (j == j_ref || colnorm < ACA_EPS) {
       if (j == j_ref) ntries_col = ntries_col + 1;
@@ -430,7 +433,7 @@ __global__ __launch_bounds__(kT) void k_aca(const CLeaf *__restrict__ lv,
               break;
             }
             calc_vec(L, g, true, k + 1, j, pa_ref, U, V);
-            colnorm = unrm(pa_ref, ndl, red);
+            colnorm = unrm(pa_ref, ndl);
             if (colnorm < ACA_EPS) set_mask(lcol_msk, j);
             ntries_col = ntries_col - 1;
           } else {
@@ -448,7 +451,7 @@ __global__ __launch_bounds__(kT) void k_aca(const CLeaf *__restrict__ lv,
       k = k + 1;
     }
     if (lstop_aca == 0) {
-      const double blknorm = unrm(pcol, ndl, red) * unrm(prow, ndt, red);
+      const double blknorm = unrm(pcol, ndl) * unrm(prow, ndt);
       if (k == 0) {
         apxnorm = blknorm;
       } else {
@@ -561,7 +564,7 @@ hipError_t run_aca(hmv_dleaves_s *d, const std::vector<int> &ids, const Geo &g,
   if (e == hipSuccess &&
       (e = hipMemcpy(d_lv, d->lv.data(), sizeof(CLeaf) * d->lv.size(), hipMemcpyHostToDevice)) ==
           hipSuccess) {
-    hipLaunchKernelGGL(k_aca, dim3((unsigned)ids.size()), dim3(kT), 0, nullptr, d_lv, d_ids, g, p,
+    hipLaunchKernelGGL(k_aca, dim3((unsigned)ids.size()), dim3(kA), 0, nullptr, d_lv, d_ids, g, p,
                        d_kt, d_st);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
