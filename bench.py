@@ -261,7 +261,7 @@ def study_327680(H, log):
     hm = H.HMatrix(*H.icosphere(7), H.baseline_params(aca_eps=1e-5))
     nd = hm.nd
     out["construction_s"] = round(time.perf_counter() - t0, 2)
-    arenas = {p: H.Arena(hm, p) for p in ("f64", "f32", "bf16")}
+    arenas = {p: H.Arena(hm, p) for p in ("f64", "f32", "bf16", "bf16_all")}
     x = H.DeviceBuffer.from_array(H.uniform(1, nd))
     y = H.DeviceBuffer.from_array(np.zeros(nd))
     rates = {}
@@ -272,7 +272,7 @@ def study_327680(H, log):
         log("s=7 %s apply %.4f ms, %.1f GB/s at storage precision" % (p, ma, rates[p]["storage_alg_GBs"]))
     out["apply"] = rates
     solves = {}
-    inner = {"f32": 1e-4, "bf16": 1e-2}
+    inner = {"f32": 1e-4, "bf16": 1e-2, "bf16_all": 1e-2}
     for rname, b in (("ones", np.ones(nd)), ("uniform_seed2", H.uniform(2, nd))):
         a64 = arenas["f64"]
         a64.bicgstab(b, eps=1e-8, maxit=3000, check_every=8)
@@ -291,7 +291,7 @@ def study_327680(H, log):
                                           "true_relres": ip["true_relres"], "status": ip["status_name"]}
         log("s=7 %s: pipelined BiCGSTAB FP64 %.4f s (%d it), relres %.2e"
             % (rname, ip["seconds"], ip["nstp"], ip["true_relres"]))
-        for p in ("f32", "bf16"):
+        for p in ("f32", "bf16", "bf16_all"):
             H.bicgstab_ir(a64, arenas[p], b, eps=1e-8, inner_eps=inner[p], max_outer=60, max_inner=3000)
             u, inf, hist = H.bicgstab_ir(a64, arenas[p], b, eps=1e-8, inner_eps=inner[p], max_outer=60,
                                          max_inner=3000)

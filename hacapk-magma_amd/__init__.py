@@ -28,7 +28,7 @@ import numpy as np
 _PKG = os.path.dirname(os.path.abspath(__file__))
 _LIBDIR = os.path.join(_PKG, "lib")
 
-PREC = {"f64": 0, "f32": 1, "bf16": 2}
+PREC = {"f64": 0, "f32": 1, "bf16": 2, "bf16_all": 3}
 KERNEL = {"ppohbem": 0, "centroid": 1}
 SOLVE_STATUS = {0: "converged", 1: "maxiter", 2: "breakdown", 3: "nonfinite"}
 

@@ -174,7 +174,7 @@ int upload_plan(hmv::Plan &P, hmv_arena *out, std::chrono::steady_clock::time_po
   a->nd = nd;
   a->prec = prec;
   a->acc64 = prec == HMV_PREC_F64 ? 1 : 0;
-  a->mode = prec == HMV_PREC_F64 ? 0 : prec == HMV_PREC_F32 ? 1 : 2;
+  a->mode = prec;  // 0 FP64, 1 FP32, 2 BF16 U/V + FP32 A, 3 BF16 throughout
   a->tsize = P.tsize;
   a->tpart_size = P.tpart_size;
   a->nwork1 = (int)P.work1.size();

@@ -295,9 +295,12 @@ bool fail(std::string &err, const std::string &msg) {
 
 bool validate_plan(const Plan &P, std::string &err) {
   if (P.nd <= 0) return fail(err, "nd must be positive");
-  if (P.prec != HMV_PREC_F64 && P.prec != HMV_PREC_F32 && P.prec != HMV_PREC_BF16)
+  if (P.prec != HMV_PREC_F64 && P.prec != HMV_PREC_F32 && P.prec != HMV_PREC_BF16 &&
+      P.prec != HMV_PREC_BF16_ALL)
     return fail(err, "unknown precision");
-  const int dt_dense = P.prec == HMV_PREC_F64 ? HMV_DT_F64 : HMV_DT_F32;
+  const int dt_dense = P.prec == HMV_PREC_F64        ? HMV_DT_F64
+                       : P.prec == HMV_PREC_BF16_ALL ? HMV_DT_BF16
+                                                     : HMV_DT_F32;
   const int dt_lr = P.prec == HMV_PREC_F64   ? HMV_DT_F64
                     : P.prec == HMV_PREC_F32 ? HMV_DT_F32
                                              : HMV_DT_BF16;
@@ -331,7 +334,7 @@ bool validate_plan(const Plan &P, std::string &err) {
       const std::string id = ph + std::to_string(i);
       const int want0 = phase == 1 ? dt_lr : dt_dense;
       if (w.dt0 != want0 || w.dt1 != dt_lr) return fail(err, id + ": storage types");
-      const int vr = row_vector(w.dt0);
+      const int vr = phase == 1 ? row_vector(w.dt0) : row_vector_p2(w.dt0);
       const int nt = (int64_t)i < nsmall ? nt_small : HMV_WG_THREADS;
       if (w.rows < 1 || w.rows > HMV_MAX_ROWS || w.rows_p < w.rows || w.rows_p % vr != 0 ||
           w.rows_p - w.rows >= vr || w.rows_p / vr > nt)

@@ -45,9 +45,9 @@ template <int DT>
 constexpr int dt_bytes() {
   return DT == HMV_DT_F64 ? 8 : DT == HMV_DT_F32 ? 4 : 2;
 }
-template <int DT0>
+template <int PHASE, int DT0>
 constexpr int rows_per_lane() {
-  return DT0 == HMV_DT_F64 ? 1 : DT0 == HMV_DT_F32 ? 2 : 4;
+  return DT0 == HMV_DT_F64 ? 1 : (DT0 == HMV_DT_F32 || PHASE == 2) ? 2 : 4;
 }
 
 typedef uint32_t uint2_nt __attribute__((ext_vector_type(2)));
@@ -199,7 +199,7 @@ __device__ inline void stream_item(
     double *__restrict__ y, double beta) {
   constexpr int ZCH = HMV_ZCHUNK * NT / HMV_WG_THREADS;
   const int tid = threadIdx.x;
-  constexpr int vr = rows_per_lane<DT0>();
+  constexpr int vr = rows_per_lane<PHASE, DT0>();
   static_assert(vr * NT <= ZCH, "reduction buffer fits in z");
   const int q = w.rows_p / vr;
   const int m = NT / q;
@@ -386,7 +386,9 @@ extern "C" hipError_t hmv_launch_apply(const HmvLaunch *L, int phases, const dou
     launch_mode<double, HMV_DT_F64, HMV_DT_F64>(*L, phases, x, y, beta, skip, inst_done, stream);
   else if (L->mode == 1)
     launch_mode<float, HMV_DT_F32, HMV_DT_F32>(*L, phases, x, y, beta, skip, inst_done, stream);
-  else
+  else if (L->mode == 2)
     launch_mode<float, HMV_DT_F32, HMV_DT_BF16>(*L, phases, x, y, beta, skip, inst_done, stream);
+  else
+    launch_mode<float, HMV_DT_BF16, HMV_DT_BF16>(*L, phases, x, y, beta, skip, inst_done, stream);
   return hipGetLastError();
 }

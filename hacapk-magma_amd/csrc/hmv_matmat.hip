@@ -289,8 +289,11 @@ extern "C" hipError_t hmv_launch_block(const HmvLaunch *L, const int2 *tiles1, i
   else if (L->mode == 1)
     launch_block<float, HMV_DT_F32, HMV_DT_F32>(*L, tiles1, ntiles1, tiles2, ntiles2, k, x, t, y,
                                                 beta, skip, stream);
-  else
+  else if (L->mode == 2)
     launch_block<float, HMV_DT_F32, HMV_DT_BF16>(*L, tiles1, ntiles1, tiles2, ntiles2, k, x, t, y,
                                                  beta, skip, stream);
+  else
+    launch_block<float, HMV_DT_BF16, HMV_DT_BF16>(*L, tiles1, ntiles1, tiles2, ntiles2, k, x, t, y,
+                                                  beta, skip, stream);
   return hipGetLastError();
 }

@@ -23,6 +23,7 @@ namespace hmv {
 
 int dtype_size(int dt) { return dt == HMV_DT_F64 ? 8 : dt == HMV_DT_F32 ? 4 : 2; }
 int row_vector(int dt) { return dt == HMV_DT_F64 ? 1 : dt == HMV_DT_F32 ? 2 : 4; }
+int row_vector_p2(int dt) { return dt == HMV_DT_F64 ? 1 : 2; }
 
 namespace {
 
@@ -78,7 +79,7 @@ const double *u_factor(const RefLeaf *l) {
 PlanTuning default_tuning(int prec) {
   PlanTuning t;
   if (prec == kF32) t.p1_small_threads = 128;
-  if (prec == kBF16) {
+  if (prec == kBF16 || prec == kBF16All) {
     t.p1_small_threads = 128;
     t.p2_small_threads = 64;
   }
@@ -137,7 +138,7 @@ bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const 
     err = "leaf stride smaller than the leaf record";
     return false;
   }
-  if (prec != kF64 && prec != kF32 && prec != kBF16) {
+  if (prec != kF64 && prec != kF32 && prec != kBF16 && prec != kBF16All) {
     err = "unknown precision";
     return false;
   }
@@ -146,13 +147,13 @@ bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const 
     err = "invalid tuning parameters";
     return false;
   }
-  const int dt_dense = prec == kF64 ? HMV_DT_F64 : HMV_DT_F32;
+  const int dt_dense = prec == kF64 ? HMV_DT_F64 : prec == kBF16All ? HMV_DT_BF16 : HMV_DT_F32;
   const int dt_lr = prec == kF64 ? HMV_DT_F64 : prec == kF32 ? HMV_DT_F32 : HMV_DT_BF16;
   const bool merged = dt_dense == dt_lr;
   const int es_d = dtype_size(dt_dense), es_l = dtype_size(dt_lr);
   // rows per lane: phase-1 blocks hold only low-rank factors (dt_lr), phase-2
   // blocks follow their first part's type (dt_dense)
-  const int vr = row_vector(dt_lr), vr2 = row_vector(dt_dense);
+  const int vr = row_vector(dt_lr), vr2 = row_vector_p2(dt_dense);
 
   std::vector<const RefLeaf *> L(nlf);
   PlanStats &st = P.st;

@@ -25,7 +25,7 @@ struct RefLeaf {
 };
 
 // Storage precision of an arena.
-enum Precision { kF64 = 0, kF32 = 1, kBF16 = 2 };
+enum Precision { kF64 = 0, kF32 = 1, kBF16 = 2, kBF16All = 3 };
 
 struct PlanStats {
   int64_t nlf = 0, nlr = 0, ndense = 0, nkt0 = 0;
@@ -102,8 +102,11 @@ bool save_plan_file(const Plan &P, const char *path, std::string &err);
 bool load_plan_file(const char *path, Plan &P, std::string &err);
 
 int dtype_size(int dt);
-// Rows handled per lane (each lane loads `row_vector(dt)` consecutive rows).
+// Rows handled per lane (each lane loads `row_vector(dt)` consecutive rows):
+// phase-1 blocks by their low-rank type, phase-2 blocks by their first
+// part's type (two-row lanes for the narrow types).
 int row_vector(int dt);
+int row_vector_p2(int dt);
 
 }  // namespace hmv
 

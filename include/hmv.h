@@ -37,6 +37,7 @@ extern "C" {
 #define HMV_PREC_F64 0   /* factors and dense blocks FP64, FP64 accumulation       */
 #define HMV_PREC_F32 1   /* factors and dense blocks FP32, FP32 accumulation       */
 #define HMV_PREC_BF16 2  /* low-rank factors BF16, dense blocks FP32, FP32 accum.  */
+#define HMV_PREC_BF16_ALL 3  /* low-rank factors and dense blocks BF16, FP32 accum. */
 
 /* Leaf record, identical to stc_HACApK_leafmtx (HACApK_MAGMA.h:12-20) and to
  * the Fortran st_HACApK_leafmtx as seen from C.  ltmtx 1 = low rank

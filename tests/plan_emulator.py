@@ -34,12 +34,12 @@ def _part(arena, off, rows_p, ncols, dt):
     return _decode(arena[off:end], dt).reshape(ncols, rows_p).T
 
 
-def _lane_layout(w, nt=THREADS):
+def _lane_layout(w, nt=THREADS, phase=1):
     """Lane layout of a workgroup of nt threads (short blocks may run with 64
     or 128); asserts the block fits it."""
-    # rows per lane follow the first part's storage type; the second part
-    # (if any) is read with the same rows per lane
-    vr = {0: 1, 1: 2, 2: 4}[int(w["dt0"])]
+    # rows per lane follow the first part's storage type (two-row lanes for
+    # the narrow types in phase 2); the second part uses the same rows
+    vr = {0: 1, 1: 2, 2: 4 if phase == 1 else 2}[int(w["dt0"])]
     rows, rows_p = int(w["rows"]), int(w["rows_p"])
     assert 0 < rows <= rows_p <= MAX_ROWS * vr and rows_p % vr == 0 and rows_p - rows < vr
     q = rows_p // vr
@@ -150,7 +150,7 @@ def run(tables, x, nd):
     y = np.full(nd, np.nan)
     wy = np.zeros(nd, bool)
     for i, w in enumerate(tables["work2"]):
-        _lane_layout(w, nts[1](i))
+        _lane_layout(w, nts[1](i), phase=2)
         z = _stage(tables, w, x, tbuf, tsize, nd)
         out = _block_product(arena, w, z) if z.size else np.zeros(int(w["rows"]))
         ob, rows = int(w["out_base"]), int(w["rows"])

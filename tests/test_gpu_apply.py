@@ -58,7 +58,7 @@ def test_apply_is_deterministic(gpu, sphere4):
     a.free()
 
 
-@pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16", 2e-2)])
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-5), ("bf16", 2e-2), ("bf16_all", 2e-2)])
 def test_low_precision_apply(gpu, O, sphere4, prec, tol):
     hm = sphere4
     a = gpu.Arena(hm, prec)
@@ -172,7 +172,8 @@ def test_edge_leaves(gpu, O, H):
     a.free()
 
 
-@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", 1e-5), ("bf16", 2e-2)])
+@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", 1e-5), ("bf16", 2e-2),
+                                      ("bf16_all", 2e-2)])
 def test_input_runs_and_code_fallback(gpu, O, prec, tol):
     """Blocks whose inputs come from a tabled run list (31 runs), a code table
     (101 runs, beyond the table) and single runs, through the H-matvec and the

@@ -83,7 +83,7 @@ def test_maxiter_and_zero_rhs(gpu, sphere3):
     a.free()
 
 
-@pytest.mark.parametrize("prec", ["f32", "bf16"])
+@pytest.mark.parametrize("prec", ["f32", "bf16", "bf16_all"])
 def test_mixed_precision_refinement(gpu, O, sphere4, prec):
     hm = sphere4
     a64, alow = gpu.Arena(hm, "f64"), gpu.Arena(hm, prec)

@@ -10,10 +10,10 @@ import pytest
 
 import plan_emulator as PE
 
-TOL = {"f64": 1e-14, "f32": 1e-6, "bf16": 1e-2}
+TOL = {"f64": 1e-14, "f32": 1e-6, "bf16": 1e-2, "bf16_all": 1e-2}
 
 
-@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16", "bf16_all"])
 @pytest.mark.parametrize("case", ["sphere3", "sphere4", "two_spheres4", "ref_mesh_sample"])
 def test_plan_emulation_matches_oracle(H, O, request, case, prec):
     hm = request.getfixturevalue(case)
