@@ -216,6 +216,20 @@ def main():
             log("BiCGSTAB-IR %s: outer %d inner %d, %.3f s (%.2fx FP64), relres %.2e, err vs FP64 sol %.2e"
                 % (prec, inf["outer"], inf["inner_total"], inf["seconds"], inf["seconds"] / i64["seconds"],
                    inf["relres"], err))
+            if prec == "bf16":
+                # refinement with GCR(32) inner solves on the same BF16 arena
+                H.gcr_ir(a64, alow, b, eps=1e-8, inner_eps=5e-3, m=32)
+                u, inf, hist = H.gcr_ir(a64, alow, b, eps=1e-8, inner_eps=5e-3, m=32)
+                err = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+                solves["gcr32_ir_bf16"] = {
+                    "seconds": round(inf["seconds"], 4), "inner_eps": 5e-3, "outer": inf["outer"],
+                    "inner_products": inf["inner_total"], "true_relres": inf["relres"],
+                    "status": inf["status_name"], "rel_err_vs_fp64_solution": err,
+                    "outer_residuals": [float(v) for v in hist],
+                    "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3)}
+                log("GCR(32)-IR bf16: outer %d, %d inner products, %.3f s (%.2fx FP64), relres %.2e, "
+                    "err vs FP64 sol %.2e" % (inf["outer"], inf["inner_total"], inf["seconds"],
+                                              inf["seconds"] / i64["seconds"], inf["relres"], err))
             alow.free()
         result["bicgstab"] = solves
 
@@ -304,6 +318,19 @@ def study_327680(H, log):
             log("s=7 %s: FP64 %.4f s (%d it) vs %s-IR %.4f s (%.2fx), relres %.2e"
                 % (rname, i64["seconds"], i64["nstp"], p, inf["seconds"], inf["seconds"] / i64["seconds"],
                    inf["relres"]))
+            if p == "f32":
+                continue
+            H.gcr_ir(a64, arenas[p], b, eps=1e-8, inner_eps=5e-3, m=32)
+            u, inf, hist = H.gcr_ir(a64, arenas[p], b, eps=1e-8, inner_eps=5e-3, m=32)
+            row["gcr32_ir_" + p] = {
+                "seconds": round(inf["seconds"], 4), "inner_eps": 5e-3, "outer": inf["outer"],
+                "inner_products": inf["inner_total"], "true_relres": inf["relres"],
+                "status": inf["status_name"], "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3),
+                "rel_err_vs_fp64_solution": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                "outer_residuals": [float(v) for v in hist]}
+            log("s=7 %s: FP64 %.4f s vs %s GCR(32)-IR %.4f s (%.2fx), %d outer, %d inner products, "
+                "relres %.2e" % (rname, i64["seconds"], p, inf["seconds"], inf["seconds"] / i64["seconds"],
+                                 inf["outer"], inf["inner_total"], inf["relres"]))
         solves[rname] = row
     out["bicgstab"] = solves
     # multi-RHS

@@ -231,6 +231,8 @@ def hmv_lib():
                                C.POINTER(SolveInfo)]
         L.hmv_bicgstab.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, _D,
                                    C.POINTER(SolveInfo)]
+        L.hmv_gcr_ir.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, _D, C.POINTER(SolveInfo)]
         L.hmv_bicgstab_ir.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.c_int, _D, C.POINTER(SolveInfo)]
         L.hmv_last_error.restype = C.c_char_p
@@ -936,6 +938,28 @@ def bicgstab_ir(a64, alow, b, u0=None, eps=1e-8, inner_eps=1e-4, max_outer=30, m
                                      C.byref(info)),
            "hmv_bicgstab_ir")
     u = du.download()
+    return u, info.as_dict(), hist[: info.outer + 1]
+
+
+def gcr_ir(a64, alow, b, u0=None, eps=1e-8, inner_eps=5e-3, m=32, max_outer=30, max_inner=1000,
+           check_every=8):
+    """Mixed-precision refinement with GCR(m) inner solves on the
+    low-precision arena (hmv_gcr_ir).  Defaults measured best at 327,680
+    unknowns with BF16 arenas (inner 5e-3, m = 32).  Returns (u, info, outer
+    residuals)."""
+    b = np.ascontiguousarray(b, np.float64)
+    u0 = np.zeros_like(b) if u0 is None else np.ascontiguousarray(u0, np.float64)
+    db, du = DeviceBuffer.from_array(b), DeviceBuffer.from_array(u0)
+    hist = np.full(max_outer + 1, np.nan)
+    info = SolveInfo()
+    try:
+        _check(hmv_lib().hmv_gcr_ir(a64.handle, alow.handle, db.ptr, du.ptr, float(eps),
+                                    float(inner_eps), int(m), int(max_outer), int(max_inner),
+                                    int(check_every), _dptr(hist), C.byref(info)), "hmv_gcr_ir")
+        u = du.download()
+    finally:
+        db.free()
+        du.free()
     return u, info.as_dict(), hist[: info.outer + 1]
 
 

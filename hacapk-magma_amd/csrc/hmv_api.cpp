@@ -78,6 +78,7 @@ hipError_t upload(T **dst, const std::vector<T> &src) {
 void release(hmv_arena_s *a) {
   if (!a) return;
   hmv::free_solve_work(a);
+  hmv::free_gcr_work(a);
   (void)hipFree(a->d_arena);
   (void)hipFree(a->d_work1);
   (void)hipFree(a->d_work2);

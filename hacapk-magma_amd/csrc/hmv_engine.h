@@ -39,6 +39,7 @@ struct hmv_arena_s {
   hmv_tuning tuning{};     // the tuning the arena was built with
   double *d_hx = nullptr, *d_hy = nullptr;  // scratch for hmv_apply_host
   HmvSolveWork *solve = nullptr;             // solver workspace (lazy)
+  struct HmvGcrWork *gcr = nullptr;          // GCR(m) workspace (lazy)
   hmv_arena_stats stats{};
   // what a saved arena file needs besides the device tables
   int64_t nzseg = 0, noutidx = 0, ncodes = 0;
@@ -55,6 +56,7 @@ int hip_fail(hipError_t e, const char *what);
 hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta, const int *skip,
                          hipStream_t s, const int *inst_done = nullptr);
 void free_solve_work(hmv_arena_s *a);
+void free_gcr_work(hmv_arena_s *a);
 // Enqueues Y = H X + beta*Y for k interleaved right-hand sides (1..128).
 hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
                                const int *skip, hipStream_t s);

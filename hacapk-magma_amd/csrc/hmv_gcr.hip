@@ -244,20 +244,73 @@ int grid_of(int n) {
   return std::max(1, std::min((n + kT - 1) / kT, kMaxB));
 }
 
-struct GcrWork {
+}  // namespace
+
+// GCR workspace, kept with the arena between calls (the iteration graph is
+// re-captured only when m, the history capacity or the solution vector change).
+struct HmvGcrWork {
+  int m = 0;
+  size_t hist_cap = 0;
   hipStream_t s = nullptr;
   double *r = nullptr, *ar = nullptr, *p = nullptr, *ap = nullptr, *part = nullptr, *hist = nullptr;
   GcrState *st = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  ~GcrWork() {
+  double *g_u = nullptr;  // solution vector the graph was captured with
+  void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    g_u = nullptr;
+  }
+  ~HmvGcrWork() {
+    drop_graph();
     if (s) (void)hipStreamDestroy(s);
     for (double *q : {r, ar, p, ap, part, hist}) (void)hipFree(q);
     (void)hipFree(st);
   }
 };
+
+namespace hmv {
+void free_gcr_work(hmv_arena_s *a) {
+  delete a->gcr;
+  a->gcr = nullptr;
+}
+}  // namespace hmv
+
+namespace {
+
+using GcrWork = HmvGcrWork;
+
+// The arena's GCR workspace for m directions and maxit steps.
+int gcr_work(hmv_arena_s *a, int m, int maxit, GcrWork *&out) {
+  GcrWork *w = a->gcr;
+  if (w && (w->m != m || w->hist_cap < (size_t)maxit + 1)) {
+    hmv::free_gcr_work(a);
+    w = nullptr;
+  }
+  if (!w) {
+    w = new GcrWork();
+    a->gcr = w;
+    const int n = a->nd, g = grid_of(n);
+    const size_t nb = sizeof(double) * (size_t)n;
+    w->m = m;
+    w->hist_cap = (size_t)maxit + 1;
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&w->s, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc(&w->r, nb)) != hipSuccess || (e = hipMalloc(&w->ar, nb)) != hipSuccess ||
+        (e = hipMalloc(&w->p, nb * m)) != hipSuccess || (e = hipMalloc(&w->ap, nb * m)) != hipSuccess ||
+        (e = hipMalloc(&w->part, sizeof(double) * (size_t)g * kMaxM)) != hipSuccess ||
+        (e = hipMalloc(&w->hist, sizeof(double) * w->hist_cap)) != hipSuccess ||
+        (e = hipMalloc(&w->st, sizeof(GcrState))) != hipSuccess) {
+      hmv::free_gcr_work(a);
+      return hmv::hip_fail(e, "hmv_gcrm: workspace");
+    }
+  }
+  out = w;
+  return HMV_OK;
+}
 
 hipError_t gcr_iteration(hmv_arena_s *a, GcrWork &w, int n, int m, double *u) {
   const int g = grid_of(n);
@@ -281,24 +334,31 @@ hipError_t gcr_iteration(hmv_arena_s *a, GcrWork &w, int n, int m, double *u) {
 
 }  // namespace
 
+namespace {
+int gcrm_run(hmv_arena a, const double *b, double *u, double eps, int maxit, int m,
+             int check_every, double *hist, hmv_solve_info *info, bool true_res);
+}
+
 extern "C" int hmv_gcrm(hmv_arena a, const double *b, double *u, double eps, int maxit, int m,
                         int check_every, double *hist, hmv_solve_info *info) {
+  return gcrm_run(a, b, u, eps, maxit, m, check_every, hist, info, true);
+}
+
+namespace {
+// true_res: recompute ||b - A u|| / ||b|| at the end (one more product).
+int gcrm_run(hmv_arena a, const double *b, double *u, double eps, int maxit, int m,
+             int check_every, double *hist, hmv_solve_info *info, bool true_res) {
   if (!a || !b || !u || maxit < 0 || m < 1 || m > kMaxM) {
     hmv::set_error("hmv_gcrm: invalid argument (1 <= m <= 32)");
     return HMV_ERR_ARG;
   }
   if (check_every < 1) check_every = 1;
   const int n = a->nd, g = grid_of(n);
-  GcrWork w;
-  const size_t nb = sizeof(double) * (size_t)n;
+  GcrWork *wp = nullptr;
+  int rc = gcr_work(a, m, maxit, wp);
+  if (rc) return rc;
+  GcrWork &w = *wp;
   hipError_t e;
-  if ((e = hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking)) != hipSuccess ||
-      (e = hipMalloc(&w.r, nb)) != hipSuccess || (e = hipMalloc(&w.ar, nb)) != hipSuccess ||
-      (e = hipMalloc(&w.p, nb * m)) != hipSuccess || (e = hipMalloc(&w.ap, nb * m)) != hipSuccess ||
-      (e = hipMalloc(&w.part, sizeof(double) * (size_t)g * kMaxM)) != hipSuccess ||
-      (e = hipMalloc(&w.hist, sizeof(double) * ((size_t)maxit + 1))) != hipSuccess ||
-      (e = hipMalloc(&w.st, sizeof(GcrState))) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_gcrm: workspace");
   auto t0 = std::chrono::steady_clock::now();
   GcrState init{};
   init.eps = eps;
@@ -312,14 +372,20 @@ extern "C" int hmv_gcrm(hmv_arena a, const double *b, double *u, double eps, int
   hipLaunchKernelGGL(g_final, dim3(1), dim3(kT), 0, w.s, g, w.part, w.st, 0, w.hist);
   // ap_1 = A p_1 (skipped when already converged)
   if ((e = hmv::enqueue_apply(a, w.p, w.ap, 0.0, &w.st->done, w.s)) != hipSuccess ||
-      (e = hipStreamSynchronize(w.s)) != hipSuccess ||
-      (e = hipStreamBeginCapture(w.s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      (e = hipStreamSynchronize(w.s)) != hipSuccess)
     return hmv::hip_fail(e, "hmv_gcrm: setup");
-  hipError_t ec = gcr_iteration(a, w, n, m, u);
-  e = hipStreamEndCapture(w.s, &w.graph);
-  if (ec != hipSuccess) return hmv::hip_fail(ec, "hmv_gcrm: capture");
-  if (e != hipSuccess || (e = hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_gcrm: graph");
+  if (!w.exec || w.g_u != u) {
+    w.drop_graph();
+    if ((e = hipStreamBeginCapture(w.s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_gcrm: capture");
+    hipError_t ec = gcr_iteration(a, w, n, m, u);
+    e = hipStreamEndCapture(w.s, &w.graph);
+    if (ec != hipSuccess) return hmv::hip_fail(ec, "hmv_gcrm: capture");
+    if (e != hipSuccess ||
+        (e = hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0)) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_gcrm: graph");
+    w.g_u = u;
+  }
   GcrState host{};
   int launched = 0;
   for (;;) {
@@ -342,7 +408,7 @@ extern "C" int hmv_gcrm(hmv_arena a, const double *b, double *u, double eps, int
   }
   // true residual ||b - A u|| / ||b||
   double tr = NAN;
-  {
+  if (true_res) {
     if ((e = hmv::enqueue_apply(a, u, w.ar, 0.0, nullptr, w.s)) != hipSuccess)
       return hmv::hip_fail(e, "hmv_gcrm: true residual");
     hipLaunchKernelGGL(g_init, dim3(g), dim3(kT), 0, w.s, n, b, w.ar, w.r, w.p);
@@ -367,6 +433,111 @@ extern "C" int hmv_gcrm(hmv_arena a, const double *b, double *u, double eps, int
     info->seconds = std::chrono::duration<double>(t1 - t0).count();
     info->outer = 0;
     info->inner_total = host.step;
+  }
+  return HMV_OK;
+}
+}  // namespace
+
+namespace {
+
+// r = b - v
+__global__ __launch_bounds__(kT) void g_residual(int n, const double *__restrict__ b,
+                                                 const double *__restrict__ v,
+                                                 double *__restrict__ r) {
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) r[i] = b[i] - v[i];
+}
+
+// u += d;  d = 0
+__global__ __launch_bounds__(kT) void g_add_clear(int n, double *__restrict__ u,
+                                                  double *__restrict__ d) {
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
+    u[i] = u[i] + d[i];
+    d[i] = 0.0;
+  }
+}
+
+struct IrBufs {
+  double *r = nullptr, *d = nullptr, *v = nullptr, *part = nullptr;
+  hipStream_t s = nullptr;
+  ~IrBufs() {
+    for (double *q : {r, d, v, part}) (void)hipFree(q);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+// r = b - A u and ||r|| / ||b|| (A: the arena a)
+int residual(hmv_arena_s *a, IrBufs &w, const double *b, const double *u, double *rel) {
+  const int n = a->nd, g = grid_of(n);
+  hipError_t e;
+  if ((e = hmv::enqueue_apply(a, u, w.v, 0.0, nullptr, w.s)) != hipSuccess)
+    return hmv::hip_fail(e, "hmv_gcr_ir: residual");
+  hipLaunchKernelGGL(g_residual, dim3(g), dim3(kT), 0, w.s, n, b, w.v, w.r);
+  hipLaunchKernelGGL(g_dots2, dim3(g), dim3(kT), 0, w.s, n, w.r, w.r, b, b, w.part);
+  std::vector<double> part(2 * (size_t)g);
+  if ((e = hipMemcpyAsync(part.data(), w.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                          w.s)) != hipSuccess ||
+      (e = hipStreamSynchronize(w.s)) != hipSuccess)
+    return hmv::hip_fail(e, "hmv_gcr_ir: residual");
+  double rr = 0.0, bb = 0.0;
+  for (int i = 0; i < g; ++i) {
+    rr += part[2 * i];
+    bb += part[2 * i + 1];
+  }
+  *rel = std::sqrt(rr) / std::sqrt(bb);
+  return HMV_OK;
+}
+
+}  // namespace
+
+extern "C" int hmv_gcr_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
+                          double inner_eps, int m, int max_outer, int max_inner, int check_every,
+                          double *hist, hmv_solve_info *info) {
+  if (!a64 || !alow || !b || !u || a64->nd != alow->nd || m < 1 || m > kMaxM || max_outer < 0 ||
+      max_inner < 0) {
+    hmv::set_error("hmv_gcr_ir: invalid arguments");
+    return HMV_ERR_ARG;
+  }
+  const int n = a64->nd, g = grid_of(n);
+  const size_t nb = sizeof(double) * (size_t)n;
+  IrBufs w;
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc(&w.r, nb)) != hipSuccess || (e = hipMalloc(&w.d, nb)) != hipSuccess ||
+      (e = hipMalloc(&w.v, nb)) != hipSuccess ||
+      (e = hipMalloc(&w.part, sizeof(double) * 2 * (size_t)g)) != hipSuccess ||
+      (e = hipMemsetAsync(w.d, 0, nb, w.s)) != hipSuccess)
+    return hmv::hip_fail(e, "hmv_gcr_ir: workspace");
+  auto t0 = std::chrono::steady_clock::now();
+  double rel = 0.0;
+  int rc = residual(a64, w, b, u, &rel);
+  if (rc) return rc;
+  if (hist) hist[0] = rel;
+  int outer = 0, inner_total = 0, status = HMV_SOLVE_MAXITER;
+  if (!std::isfinite(rel)) status = HMV_SOLVE_NONFINITE;
+  while (status == HMV_SOLVE_MAXITER && rel >= eps && outer < max_outer) {
+    // solve Alow d = r to inner_eps with GCR(m), d starting at zero
+    hmv_solve_info in{};
+    rc = gcrm_run(alow, w.r, w.d, inner_eps, max_inner, m, check_every, nullptr, &in, false);
+    if (rc) return rc;
+    inner_total += in.nstp;
+    if (in.status == HMV_SOLVE_NONFINITE) status = HMV_SOLVE_NONFINITE;
+    hipLaunchKernelGGL(g_add_clear, dim3(g), dim3(kT), 0, w.s, n, u, w.d);
+    ++outer;
+    rc = residual(a64, w, b, u, &rel);
+    if (rc) return rc;
+    if (hist) hist[outer] = rel;
+    if (!std::isfinite(rel)) status = HMV_SOLVE_NONFINITE;
+  }
+  if (status == HMV_SOLVE_MAXITER && rel < eps) status = HMV_SOLVE_CONVERGED;
+  auto t1 = std::chrono::steady_clock::now();
+  if (info) {
+    info->nstp = inner_total;
+    info->status = status;
+    info->relres = rel;
+    info->true_relres = rel;
+    info->seconds = std::chrono::duration<double>(t1 - t0).count();
+    info->outer = outer;
+    info->inner_total = inner_total;
   }
   return HMV_OK;
 }

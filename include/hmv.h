@@ -187,6 +187,16 @@ int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, d
                     double inner_eps, int max_outer, int max_inner, int check_every, double *hist,
                     hmv_solve_info *info);
 
+/* Mixed-precision iterative refinement with GCR(m) inner solves: as
+ * hmv_bicgstab_ir, with each correction solved by GCR(m) (hmv_gcrm) on the
+ * low-precision arena.  A short inner solve is mostly GCR's first cycle,
+ * which minimises the residual over the Krylov space: fewer low-precision
+ * products per decade than BiCGSTAB's two per step.  No reference
+ * counterpart. */
+int hmv_gcr_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
+               double inner_eps, int m, int max_outer, int max_inner, int check_every,
+               double *hist, hmv_solve_info *info);
+
 /* Mixed-precision BiCGSTAB with reliable updates: one BiCGSTAB whose
  * products use the low-precision arena `alow`, with u, vectors and scalars
  * in FP64; whenever the iterated residual has fallen by `delta` (0..1) since

@@ -60,6 +60,27 @@ def test_gcrm_check_every_and_edge_cases(gpu, sphere3):
     a.free()
 
 
+@pytest.mark.parametrize("prec", ["f32", "bf16", "bf16_all"])
+def test_gcr_refinement(gpu, O, sphere4, prec):
+    """Refinement with GCR(32) inner solves on a low-precision arena reaches
+    a true 1e-8 residual and the FP64 solution; the inner work is counted in
+    low-precision products (one per GCR step)."""
+    hm = sphere4
+    a64, alow = gpu.Arena(hm, "f64"), gpu.Arena(hm, prec)
+    b = gpu.uniform(14, hm.nd)
+    u, info, hist = gpu.gcr_ir(a64, alow, b, eps=1e-8, inner_eps=5e-3, m=32)
+    u_ref, _, _ = O.bicgstab(hm, b, None, mstep=500, eps=1e-11)
+    print(prec, "outer", info["outer"], "inner", info["inner_total"], "residuals", hist, flush=True)
+    assert info["status_name"] == "converged" and info["relres"] < 1e-8
+    assert np.all(np.diff(hist) < 0)
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-6
+    # a second call reuses the arena's GCR workspace and gives the same result
+    u2, info2, _ = gpu.gcr_ir(a64, alow, b, eps=1e-8, inner_eps=5e-3, m=32)
+    assert np.array_equal(u, u2) and info2["inner_total"] == info["inner_total"]
+    a64.free()
+    alow.free()
+
+
 def test_solve_with_gcrm(gpu, sphere3):
     """HACApK_solve with param(85) = 2 (GCR) and = 1 (BiCGSTAB) reach the
     same solution."""
