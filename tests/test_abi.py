@@ -44,3 +44,42 @@ def test_no_cpu_fallback(H, sphere3):
         H.Arena(sphere3, "f64")
     with pytest.raises(H.HmvError):
         H.DeviceBuffer(64)
+
+
+def test_new_reference_named_symbols_present(H):
+    L = H.hmv_lib()
+    for n in ("c_hacapk_bicgstab_cax_lfmtx_pipe_", "c_hacapk_bicgstab_cax_lfmtx_pipe2_"):
+        assert hasattr(L, n)
+
+
+def test_argument_checks_before_any_device_call(H):
+    """Invalid arguments of the solver, construction and file entry points
+    fail with HMV_ERR_ARG and a message, before touching a device."""
+    L = H.hmv_lib()
+    info = H.SolveInfo()
+    assert L.hmv_gcrm(None, None, None, 1e-8, 10, 8, 1, None, C.byref(info)) == -1
+    assert b"hmv_gcrm" in L.hmv_last_error()
+    assert L.hmv_bicgstab_pipe(None, None, None, 1e-8, 10, 1, None, C.byref(info)) == -1
+    assert L.hmv_gcr_ir(None, None, None, None, 1e-8, 1e-2, 8, 10, 10, 1, None, C.byref(info)) == -1
+    d = C.c_void_p()
+    cp = H._ConstructParams(kernel=7, kmax=10, kmin=1, aca_eps=1e-4, aca_eps_factor=1.0)
+    nodes = (C.c_double * 9)()
+    faces = (C.c_int * 3)(0, 1, 2)
+    lod = (C.c_int * 1)(1)
+    rec = (H.LeafRec * 1)()
+    assert L.hmv_construct(3, nodes, 1, faces, nodes, nodes, lod, 1, C.addressof(rec),
+                           C.sizeof(H.LeafRec), C.byref(cp), C.byref(d)) == -1
+    cp.kernel = 1
+    faces[2] = 5  # missing node
+    assert L.hmv_construct(3, nodes, 1, faces, nodes, nodes, lod, 1, C.addressof(rec),
+                           C.sizeof(H.LeafRec), C.byref(cp), C.byref(d)) == -1
+    assert b"missing node" in L.hmv_last_error()
+    faces[2] = 2
+    rec[0].ltmtx, rec[0].nstrtl, rec[0].ndl, rec[0].nstrtt, rec[0].ndt = 2, 1, 2, 1, 1  # 2 rows of 1
+    assert L.hmv_construct(3, nodes, 1, faces, nodes, nodes, lod, 1, C.addressof(rec),
+                           C.sizeof(H.LeafRec), C.byref(cp), C.byref(d)) == -1
+    assert b"outside the matrix" in L.hmv_last_error()
+    p = C.c_void_p()
+    assert L.hmv_plan_load(b"/nonexistent/arena.hmv", C.byref(p)) == -1
+    assert b"cannot open" in L.hmv_last_error()
+    assert L.hmv_arena_load(b"/nonexistent/arena.hmv", C.byref(p)) == -1
