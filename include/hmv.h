@@ -1,5 +1,6 @@
 /*
- * hmv.h -- MI355X H-matrix matvec engine and BiCGSTAB solver (C ABI).
+ * hmv.h -- MI355X H-matrix matvec engine, its Krylov solvers and device
+ * construction (C ABI).
  *
  * Plain pointers and sizes only.  Device pointers are hipMalloc'd memory of
  * the current device; `stream` is a hipStream_t passed as void* (NULL = the
