@@ -171,7 +171,7 @@ int upload_plan(hmv::Plan &P, hmv_arena *out, std::chrono::steady_clock::time_po
   const int nd = P.nd, prec = P.prec;
   const std::vector<int> *seg = &P.seg;
   hmv_arena_s *a = new hmv_arena_s();
-  hipGetDevice(&a->device);
+  (void)hipGetDevice(&a->device);  // the current device (hmv_init succeeded)
   a->nd = nd;
   a->prec = prec;
   a->acc64 = prec == HMV_PREC_F64 ? 1 : 0;
@@ -467,9 +467,9 @@ int hmv_apply_block_timed(hmv_arena a, int k, const double *x, double *y, int re
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float m = 0.f;
   if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
-  if (e1) hipEventDestroy(e1);
-  if (e0) hipEventDestroy(e0);
-  if (s) hipStreamDestroy(s);
+  if (e1) (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (s) (void)hipStreamDestroy(s);
   if (e != hipSuccess) return hip_fail(e, "hmv_apply_block_timed");
   if (ms) *ms = (double)m / reps;
   return HMV_OK;
@@ -538,9 +538,9 @@ int hmv_device_info(char *name, int name_len, int *compute_units) {
   int rc = check_device();
   if (rc) return rc;
   int dev = 0;
-  hipGetDevice(&dev);
   hipDeviceProp_t prop;
-  hipError_t e = hipGetDeviceProperties(&prop, dev);
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
   if (e != hipSuccess) return hip_fail(e, "hmv_device_info");
   if (name && name_len > 0) {
     std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
@@ -565,26 +565,27 @@ int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *m
   if (e == hipSuccess) {
     for (int r = 0; r < reps && e == hipSuccess; ++r) {
       hipEvent_t e0 = ev[3 * r], e1 = ev[3 * r + 1], e2 = ev[3 * r + 2];
-      hipEventRecord(e0, s);
+      e = hipEventRecord(e0, s);
       // phase 1 = t (rank-vector products and the chunk-partial sums)
-      if (a->nwork1 > 0) e = hmv_launch_apply(&a->launch, HMV_PHASE1, x, y, 0.0, nullptr, nullptr, s);
-      hipEventRecord(e1, s);
+      if (e == hipSuccess && a->nwork1 > 0)
+        e = hmv_launch_apply(&a->launch, HMV_PHASE1, x, y, 0.0, nullptr, nullptr, s);
+      if (e == hipSuccess) e = hipEventRecord(e1, s);
       if (e == hipSuccess) e = hmv_launch_apply(&a->launch, HMV_PHASE2, x, y, 0.0, nullptr, nullptr, s);
-      hipEventRecord(e2, s);
+      if (e == hipSuccess) e = hipEventRecord(e2, s);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     for (int r = 0; r < reps && e == hipSuccess; ++r) {
       float m1 = 0.f, m2 = 0.f;
-      hipEventElapsedTime(&m1, ev[3 * r], ev[3 * r + 1]);
-      hipEventElapsedTime(&m2, ev[3 * r + 1], ev[3 * r + 2]);
+      e = hipEventElapsedTime(&m1, ev[3 * r], ev[3 * r + 1]);
+      if (e == hipSuccess) e = hipEventElapsedTime(&m2, ev[3 * r + 1], ev[3 * r + 2]);
       t1 += m1;
       t2 += m2;
       ta += (double)m1 + (double)m2;
     }
   }
   for (auto &v : ev)
-    if (v) hipEventDestroy(v);
-  hipStreamDestroy(s);
+    if (v) (void)hipEventDestroy(v);
+  (void)hipStreamDestroy(s);
   if (e != hipSuccess) return hip_fail(e, "hmv_apply_timed");
   if (ms_phase1) *ms_phase1 = a->nwork1 > 0 ? t1 / reps : 0.0;
   if (ms_phase2) *ms_phase2 = t2 / reps;
@@ -661,9 +662,9 @@ int hmv_autotune(int nd, int nlf, const void *leaves, int leaf_stride, int preci
   search([](hmv::PlanTuning &t, int64_t v) { t.p1_target_bytes = v; }, {48 << 10, 96 << 10, 192 << 10});
   search([](hmv::PlanTuning &t, int64_t v) { t.p2_max_rows = (int)v; }, {128, 256});
   search([](hmv::PlanTuning &t, int64_t v) { t.p1_row_tile = (int)v; }, {128, 256});
-  if (e1) hipEventDestroy(e1);
-  if (e0) hipEventDestroy(e0);
-  if (s) hipStreamDestroy(s);
+  if (e1) (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (s) (void)hipStreamDestroy(s);
   (void)hipFree(dx);
   (void)hipFree(dy);
   if (rc) return rc;
@@ -681,18 +682,17 @@ int hmv_stream_test(int kind, const void *src, void *dst, size_t bytes, int reps
   hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(e, "hmv_stream_test: stream");
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  e = hmv_launch_stream_test(kind, src, dst, bytes, s);  // warm
-  hipEventRecord(e0, s);
+  if ((e = hipEventCreate(&e0)) == hipSuccess && (e = hipEventCreate(&e1)) == hipSuccess)
+    e = hmv_launch_stream_test(kind, src, dst, bytes, s);  // warm
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
   for (int r = 0; r < reps && e == hipSuccess; ++r) e = hmv_launch_stream_test(kind, src, dst, bytes, s);
-  hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   float t = 0.f;
-  if (e == hipSuccess) hipEventElapsedTime(&t, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipStreamDestroy(s);
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
   if (e != hipSuccess) return hip_fail(e, "hmv_stream_test");
   if (ms) *ms = (double)t / reps;
   return HMV_OK;
