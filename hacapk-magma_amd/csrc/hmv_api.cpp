@@ -40,8 +40,13 @@ int ensure_tk(hmv_arena_s *a, int k) {
 
 hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
                                const int *skip, hipStream_t s) {
-  return hmv_launch_block(&a->launch, a->d_tiles1, a->ntiles1, a->d_tiles2, a->ntiles2, k, x,
-                          a->d_tk, y, beta, skip, s);
+  HmvUnits u;
+  for (int p = 0; p < 2; ++p)
+    for (int w = 0; w < HMV_MM_WIDTHS; ++w) {
+      u.u[p][w] = a->d_units[p][w];
+      u.n[p][w] = a->nunits[p][w];
+    }
+  return hmv_launch_block(&a->launch, &u, k, x, a->d_tk, y, beta, skip, s);
 }
 
 hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta, const int *skip,
@@ -85,8 +90,8 @@ void release(hmv_arena_s *a) {
   (void)hipFree(a->d_tfin);
   (void)hipFree(a->d_winst1);
   (void)hipFree(a->d_winst2);
-  (void)hipFree(a->d_tiles1);
-  (void)hipFree(a->d_tiles2);
+  for (auto &ph : a->d_units)
+    for (int2 *u : ph) (void)hipFree(u);
   (void)hipFree(a->d_tk);
   (void)hipFree(a->d_zseg);
   (void)hipFree(a->d_outidx);
@@ -166,6 +171,48 @@ int build_from_device_leaves(int nd, int nlf, const void *leaves, int stride, in
 
 namespace {
 
+// Input code of the first column of item w: x index >= 0, or -1 - t slot.
+int first_input(const hmv::Plan &P, const HmvWork &w) {
+  if (w.zseg_begin >= w.zseg_end) return 0;
+  const HmvZSeg &z = P.zseg[w.zseg_begin];
+  if (z.kind == HMV_Z_CODES) return P.codes[z.src];
+  return z.kind == HMV_Z_X ? z.src : -1 - z.src;
+}
+
+// H-matmat units of group width tw: (work item, up to tw consecutive
+// 16-row tiles).  The items are sorted by locality (phase 1: first input
+// column; phase 2: first row) and the units cut into G contiguous ranges,
+// one per XCD; the list interleaves the ranges so that workgroup p, which
+// the hardware sends to XCD p % G, runs a unit of range p % G.  Each XCD
+// then walks neighbouring items together, and the k inputs of the columns
+// they share stay in its L2.
+std::vector<int2> matmat_units(const hmv::Plan &P, const std::vector<HmvWork> &wl, bool phase1,
+                               int tw) {
+  std::vector<std::pair<int64_t, int>> key(wl.size());
+  for (size_t i = 0; i < wl.size(); ++i)
+    key[i] = {phase1 ? (int64_t)first_input(P, wl[i]) : (int64_t)wl[i].out_base, (int)i};
+  std::stable_sort(key.begin(), key.end(),
+                   [](const std::pair<int64_t, int> &x, const std::pair<int64_t, int> &y) {
+                     return x.first < y.first;
+                   });
+  std::vector<int2> sorted;
+  for (const auto &kv : key) {
+    const int T = (wl[kv.second].rows + 15) / 16;
+    for (int m = 0; m < T; m += tw)
+      sorted.push_back(make_int2(kv.second, m << 8 | std::min(tw, T - m)));
+  }
+  const size_t G = 8, n = sorted.size();  // MI355X: 8 XCDs
+  const size_t q = n / G, r = n % G;      // the first r ranges take one extra unit
+  std::vector<int2> out(n);
+  size_t b = 0;
+  for (size_t g = 0; g < G; ++g) {
+    const size_t len = q + (g < r ? 1 : 0);
+    for (size_t pos = 0; pos < len; ++pos) out[pos * G + g] = sorted[b + pos];
+    b += len;
+  }
+  return out;
+}
+
 // Uploads a plan (built here or read from a file) as a device arena.
 int upload_plan(hmv::Plan &P, hmv_arena *out, std::chrono::steady_clock::time_point t0) {
   const int nd = P.nd, prec = P.prec;
@@ -202,7 +249,7 @@ int upload_plan(hmv::Plan &P, hmv_arena *out, std::chrono::steady_clock::time_po
       return (int)(std::upper_bound(seg->begin(), seg->end(), row) - seg->begin()) - 1;
     };
     std::vector<int> w1(P.work1.size()), w2(P.work2.size());
-    for (size_t i = 0; i < P.work1.size(); ++i) w1[i] = inst_of(P.zseg[P.work1[i].zseg_begin].src);
+    for (size_t i = 0; i < P.work1.size(); ++i) w1[i] = inst_of(first_input(P, P.work1[i]));
     for (size_t i = 0; i < P.work2.size(); ++i) w2[i] = inst_of(P.work2[i].out_base);
     if ((e = upload(&a->d_winst1, w1)) != hipSuccess ||
         (e = upload(&a->d_winst2, w2)) != hipSuccess) {
@@ -210,21 +257,15 @@ int upload_plan(hmv::Plan &P, hmv_arena *out, std::chrono::steady_clock::time_po
       return hip_fail(e, "hmv_arena_build: instance tables");
     }
   }
-  // H-matmat items: every work item's 16-row tiles, in the work lists' order
-  std::vector<int2> tiles1, tiles2;
-  auto tiles_of = [](const std::vector<HmvWork> &wl, std::vector<int2> &out) {
-    for (size_t i = 0; i < wl.size(); ++i)
-      for (int m = 0; m * 16 < wl[i].rows; ++m) out.push_back(make_int2((int)i, m));
-  };
-  tiles_of(P.work1, tiles1);
-  tiles_of(P.work2, tiles2);
-  a->ntiles1 = (int)tiles1.size();
-  a->ntiles2 = (int)tiles2.size();
-  if ((e = upload(&a->d_tiles1, tiles1)) != hipSuccess ||
-      (e = upload(&a->d_tiles2, tiles2)) != hipSuccess) {
-    release(a);
-    return hip_fail(e, "hmv_arena_build: tile upload");
-  }
+  for (int ph = 0; ph < 2; ++ph)
+    for (int w = 0; w < HMV_MM_WIDTHS; ++w) {
+      const std::vector<int2> u = matmat_units(P, ph == 0 ? P.work1 : P.work2, ph == 0, 1 << w);
+      a->nunits[ph][w] = (int)u.size();
+      if ((e = upload(&a->d_units[ph][w], u)) != hipSuccess) {
+        release(a);
+        return hip_fail(e, "hmv_arena_build: H-matmat unit upload");
+      }
+    }
   size_t tb = (size_t)std::max<int64_t>(1, P.tpart_size) * sizeof(double);
   if ((e = hipMalloc((void **)&a->d_t, tb)) != hipSuccess) {
     release(a);

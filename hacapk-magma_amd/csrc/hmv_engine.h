@@ -13,6 +13,15 @@
 
 struct HmvSolveWork;
 
+// H-matmat work units: one wave per (work item, group of up to 1 << w
+// consecutive 16-row tiles), unit = {item, first tile << 8 | tiles}; one
+// list per phase and width w < HMV_MM_WIDTHS.
+#define HMV_MM_WIDTHS 3
+struct HmvUnits {
+  const int2 *u[2][HMV_MM_WIDTHS];
+  int n[2][HMV_MM_WIDTHS];
+};
+
 struct hmv_arena_s {
   int device = 0;
   int nd = 0;
@@ -30,8 +39,8 @@ struct hmv_arena_s {
   int *d_outidx = nullptr, *d_codes = nullptr;
   double *d_t = nullptr;   // t slots + chunk partials (FP64 in every mode)
   HmvLaunch launch{};      // device tables of one apply
-  int2 *d_tiles1 = nullptr, *d_tiles2 = nullptr;  // (work, 16-row tile) items of the H-matmat
-  int ntiles1 = 0, ntiles2 = 0;
+  int2 *d_units[2][HMV_MM_WIDTHS] = {};  // H-matmat units per phase and tile-group width
+  int nunits[2][HMV_MM_WIDTHS] = {};
   double *d_tk = nullptr;  // t buffer of the H-matmat (tpart_size * k doubles)
   int64_t tk_cap = 0;      // its capacity in doubles
   std::vector<int> seg;    // instance row offsets (ninst + 1) of a multi-instance arena
@@ -76,9 +85,8 @@ extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *ds
 extern "C" hipError_t hmv_launch_apply(const HmvLaunch *L, int phases, const double *x, double *y,
                                        double beta, const int *skip, const int *inst_done,
                                        hipStream_t stream);
-extern "C" hipError_t hmv_launch_block(const HmvLaunch *L, const int2 *tiles1, int ntiles1,
-                                       const int2 *tiles2, int ntiles2, int k, const double *x,
-                                       double *t, double *y, double beta, const int *skip,
-                                       hipStream_t stream);
+extern "C" hipError_t hmv_launch_block(const HmvLaunch *L, const HmvUnits *units, int k,
+                                       const double *x, double *t, double *y, double beta,
+                                       const int *skip, hipStream_t stream);
 
 #endif

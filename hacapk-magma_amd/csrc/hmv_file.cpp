@@ -339,7 +339,8 @@ bool validate_plan(const Plan &P, std::string &err) {
       if (w.rows < 1 || w.rows > HMV_MAX_ROWS || w.rows_p < w.rows || w.rows_p % vr != 0 ||
           w.rows_p - w.rows >= vr || w.rows_p / vr > nt)
         return fail(err, id + ": rows");
-      if (w.ncol0 < 0 || w.ncol1 < 0 || (phase == 1 && w.ncol1 != 0))
+      // one storage type (phase 1, or dense and low rank alike): one part
+      if (w.ncol0 < 0 || w.ncol1 < 0 || ((phase == 1 || w.dt0 == w.dt1) && w.ncol1 != 0))
         return fail(err, id + ": columns");
       auto part_ok = [&](int64_t off, int ncol, int dt) {
         if (ncol == 0) return true;

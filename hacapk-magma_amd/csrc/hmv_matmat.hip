@@ -2,17 +2,21 @@
 //
 // X and Y hold k right-hand sides interleaved (row i's k values contiguous,
 // N x k row-major); t likewise (slot s's k values at t[s*k]).  The apply runs
-// over the same arena and work lists as the single-vector kernel, as
-// 16-row tiles: one wave (64 threads) per (work item, 16-row tile).
+// over the same arena and work lists as the single-vector kernel, one wave
+// (64 threads) per unit: a work item and a group of up to TW of its 16-row
+// tiles.
 //
-// Per tile the wave stages inputs z[column][rhs] (k padded to KT*16) for a
-// chunk of the block's columns in LDS, then walks the chunk four columns at
-// a time: one 8-byte (or 4/2-byte) load per lane gives the 16 x 4 A fragment
-// (lane l: row l%16 of column l/16 of the column-major block), and KT MFMAs
-// (16x16x4; FP64, or FP32 for FP32/BF16 arenas) multiply it with the 4 x 16
-// B fragments z[column][16 t + l%16] into KT 16 x 16 accumulators.  Rows and
-// columns outside the block load zero through the buffer range check
-// (out-of-range vector offset), inputs outside the chunk read a zero row.
+// The wave walks the item's columns four at a time.  Per column quad, one
+// 8-byte (or 4/2-byte) load per lane and tile gives the 16 x 4 A fragment
+// (lane l: row l%16 of column l/16 of the column-major block), and KT loads
+// per lane give the 4 x 16 B fragments x[column][16 tt + l%16] straight
+// from the interleaved inputs (L2-resident: the units of an XCD are
+// neighbouring items, so they share columns).  Each (tile, RHS tile) pair
+// then takes one MFMA (16x16x4; FP64, or FP32 for FP32/BF16 arenas) into
+// its 16 x 16 accumulator; the B fragment is shared by the group's tiles.
+// Loads run one group of U quads ahead of the MFMAs.  Rows and columns
+// outside the block load zero through the buffer range check (out-of-range
+// vector offset) and a masked input load.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -27,7 +31,6 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kOff = 0x80000000u;  // out-of-range offset: the load returns 0
 constexpr int kNT = 2;                  // nontemporal cache policy
-constexpr int kZBytes = 16384;          // LDS bytes of staged inputs per wave
 
 __device__ inline __amdgpu_buffer_rsrc_t rsrc(const char *base, uint32_t nbytes) {
   return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)nbytes, 0x00020000);
@@ -55,170 +58,261 @@ struct Mfma<float> {
   __device__ static int row(int lane, int i) { return 4 * (lane >> 4) + i; }
 };
 
-// One A entry of a part, widened to the accumulation type (0 when off).
+// One A entry of a part as loaded (0 when off), widened to the accumulation
+// type at its MFMA (so the load's wait sits at the MFMA).
 template <int DT>
 struct Entry;
 template <>
 struct Entry<HMV_DT_F64> {
   static constexpr int es = 8;
-  __device__ static double load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  using raw = double;
+  __device__ static double load_raw(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kNT));
   }
+  __device__ static double widen(double v) { return v; }
 };
 template <>
 struct Entry<HMV_DT_F32> {
   static constexpr int es = 4;
-  __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  using raw = float;
+  __device__ static float load_raw(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, kNT));
   }
+  __device__ static float widen(float v) { return v; }
 };
 template <>
 struct Entry<HMV_DT_BF16> {
   static constexpr int es = 2;
-  __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    const uint32_t v = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, kNT);
-    return __uint_as_float(v << 16);
+  using raw = uint32_t;
+  __device__ static uint32_t load_raw(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, kNT);
   }
+  __device__ static float widen(uint32_t v) { return __uint_as_float(v << 16); }
 };
 
-// Stages z[cc][j] = input j of column c0+cc (cc < cw, j < KP; 0 for j >= k),
-// and the zero row z[cw][*].  The chunk's column sources (x index or t slot
-// of the item's runs or code table) are resolved once into `src` (LDS;
-// element offsets, t slots flagged negative), then the wave gathers SB
-// inputs per lane per round trip.
-template <typename Acc, int KP>
-__device__ inline void stage(Acc *__restrict__ z, long long *__restrict__ src, const Inputs &in,
-                             int c0, int cw, int k, const double *__restrict__ x,
-                             const int *__restrict__ codes, const double *__restrict__ t) {
-  constexpr int SB = 16;
-  const int lane = threadIdx.x;
-  for (int cc = lane; cc < cw; cc += 64) {
-    const int code = input_code(in, codes, c0 + cc);
-    src[cc] = code >= 0 ? (long long)code * k : -1 - (long long)(-code - 1) * k;
-  }
-  __syncthreads();
-  const int n = (cw + 1) * KP;
-  for (int e0 = 0; e0 < n; e0 += 64 * SB) {
-    double v[SB];
-#pragma unroll
-    for (int s = 0; s < SB; ++s) {
-      const int e = e0 + s * 64 + lane;
-      const int cc = e / KP, j = e % KP;
-      v[s] = 0.0;
-      if (e < n && cc < cw && j < k) {
-        const long long o = src[cc];
-        v[s] = o >= 0 ? x[o + j] : t[-1 - o + j];
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < SB; ++s) {
-      const int e = e0 + s * 64 + lane;
-      if (e < n) z[e] = (Acc)v[s];
-    }
-  }
-}
-
-// Independent accumulator sets per tile, summed at the end: two sets for
-// two RHS tiles measured fastest on MI355X (k = 32); one set otherwise.
+// Tile-group width (tiles sharing one B fragment) by RHS tiles, and column
+// quads per prefetch group by RHS tiles and active tiles (two groups of
+// U * (NT + KT) loads per lane are in flight; U divides the code window).
 template <int KT>
-constexpr int acc_sets() {
-  return KT == 2 ? 2 : 1;
+struct Shape {
+  static constexpr int TW = KT <= 1 ? 4 : KT <= 4 ? 2 : 1;
+  static constexpr int LOG_TW = TW == 4 ? 2 : TW == 2 ? 1 : 0;
+};
+template <int KT, int NT>
+constexpr int group_quads() {
+  return KT + NT <= 3 ? 8 : KT + NT <= 6 ? 4 : 2;
 }
 
-// Accumulates the tile's product over part columns [lo, hi) (part-local;
-// the chunk's staged inputs start at part column zbase).
-template <typename Acc, int DT, int KT>
-__device__ inline void part_mma(typename Mfma<Acc>::acc (&acc)[acc_sets<KT>()][KT],
-                                const Acc *__restrict__ z, __amdgpu_buffer_rsrc_t rs, int rows_p,
-                                int r, int ncols, int lo, int hi, int zbase, int cw) {
+constexpr int kWin = 1024;  // phase-2 input codes tabled per window (LDS, per wave)
+__device__ double g_zero_inputs[128];  // B source of the columns past a part (never written)
+
+// Per-wave constants of the B loads: lane l reads input
+// jt[tt] = min(16 tt + l%16, k - 1) of RHS tile tt (inputs j >= k only
+// feed accumulator columns that are not stored; the clamp keeps the read
+// inside the column's k values).
+template <int KT>
+struct BLane {
+  int j0;      // l % 16
+  int jt[KT];
+};
+
+// One prefetch group: U column quads of A (NT tiles) and B (KT RHS tiles),
+// as loaded (widened to Acc at the MFMA).
+template <int DT, int KT, int NT, int U>
+struct Group {
+  typename Entry<DT>::raw a[U][NT];
+  double b[U][KT];
+};
+
+// Loads the group of part columns [q0, q0 + 4U) (part columns below
+// `lim`): lane l takes column q0 + 4u + l/16.  A: row 16 (tile0 + m) + l%16
+// at byte offset col * cs + r_es[m]; a column past the part lies past the
+// buffer range and loads 0, and rows past the block only feed accumulator
+// rows that are not stored.  B: the column's inputs from x or t (PHASE 1:
+// x row xs + col; PHASE 2: the code table), or zeros past `lim`.
+template <int PHASE, int DT, int KT, int NT, int U>
+__device__ inline void load_group(Group<DT, KT, NT, U> &g, __amdgpu_buffer_rsrc_t rs, uint32_t cs,
+                                 const uint32_t (&r_es)[NT], int q0, int lim, int xs,
+                                 const int *__restrict__ tbl, int tbase,
+                                 const double *__restrict__ x, const double *__restrict__ t,
+                                 int k, const BLane<KT> &bl) {
   using E = Entry<DT>;
-  constexpr int KP = KT * 16;
-  constexpr int U = 4;  // quads (4 columns) in flight
-  constexpr int NA = acc_sets<KT>();
-  const int lane = threadIdx.x;
-  const int cq = lane >> 4, n16 = lane & 15;
-  for (int q0 = lo; q0 < hi; q0 += 4 * U) {
-    Acc a[U];
+  const int cq = threadIdx.x >> 4;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int col = q0 + 4 * u + cq;
-      const bool ok = r < rows_p && col < min(hi, ncols);
-      a[u] = (Acc)E::load(rs, ok ? (uint32_t)((col * rows_p + r) * E::es) : kOff);
+  for (int u = 0; u < U; ++u) {
+    const int col = q0 + 4 * u + cq;
+    const uint32_t cb = (uint32_t)col * cs;
+#pragma unroll
+    for (int m = 0; m < NT; ++m) g.a[u][m] = E::load_raw(rs, cb + r_es[m]);
+    const double *src;
+    if constexpr (PHASE == 1) {
+      src = x + (size_t)(xs + col) * k;
+    } else {
+      const int code = tbl[min(col, lim - 1) - tbase];
+      src = code >= 0 ? x + (size_t)code * k : t + (size_t)(-1 - code) * k;
     }
+    if (col >= lim) src = g_zero_inputs;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int zr = min(q0 + 4 * u + cq - zbase, cw);  // zero row past the chunk
+    for (int tt = 0; tt < KT; ++tt) g.b[u][tt] = src[bl.jt[tt]];
+  }
+}
+
+template <typename Acc, int DT, int KT, int TW, int NT, int U>
+__device__ inline void mma_group(typename Mfma<Acc>::acc (&acc)[TW][KT],
+                                 const Group<DT, KT, NT, U> &g) {
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int m = 0; m < NT; ++m) {
+      const Acc a = (Acc)Entry<DT>::widen(g.a[u][m]);
 #pragma unroll
       for (int tt = 0; tt < KT; ++tt)
-        acc[u % NA][tt] = Mfma<Acc>::op(a[u], z[zr * KP + tt * 16 + n16], acc[u % NA][tt]);
+        acc[m][tt] = Mfma<Acc>::op(a, (Acc)g.b[u][tt], acc[m][tt]);
+    }
+}
+
+// Accumulates the product of NT tiles over part columns [lo, hi), U quads
+// at a time, the next group's loads in flight while the current one
+// multiplies.
+template <typename Acc, int PHASE, int DT, int KT, int TW, int NT, int U>
+__device__ inline void window_mma(typename Mfma<Acc>::acc (&acc)[TW][KT],
+                                  __amdgpu_buffer_rsrc_t rs, uint32_t cs,
+                                  const uint32_t (&r_es)[NT], int lo, int hi, int xs,
+                                  const int *__restrict__ tbl, int tbase,
+                                  const double *__restrict__ x, const double *__restrict__ t, int k,
+                                  const BLane<KT> &bl) {
+  constexpr int S = 4 * U;  // columns per group
+  Group<DT, KT, NT, U> g0, g1;
+  load_group<PHASE>(g0, rs, cs, r_es, lo, hi, xs, tbl, tbase, x, t, k, bl);
+  int q0 = lo;
+  for (; q0 + 2 * S < hi; q0 += 2 * S) {  // g0 holds group q0; two more follow
+    load_group<PHASE>(g1, rs, cs, r_es, q0 + S, hi, xs, tbl, tbase, x, t, k, bl);
+    mma_group<Acc>(acc, g0);
+    load_group<PHASE>(g0, rs, cs, r_es, q0 + 2 * S, hi, xs, tbl, tbase, x, t, k, bl);
+    mma_group<Acc>(acc, g1);
+  }
+  const bool last2 = q0 + S < hi;  // one or two groups left
+  if (last2) load_group<PHASE>(g1, rs, cs, r_es, q0 + S, hi, xs, tbl, tbase, x, t, k, bl);
+  mma_group<Acc>(acc, g0);
+  if (last2) mma_group<Acc>(acc, g1);
+}
+
+// One part (ncolp columns stored with DT at rs, item columns cbase + [0,
+// ncolp)) for NT tiles from tile0.  Phase 2 tables the input codes of
+// kWin columns at a time in LDS (one binary search per column), then runs
+// the window's groups.
+template <typename Acc, int PHASE, int DT, int KT, int TW, int NT>
+__device__ inline void part_mma(typename Mfma<Acc>::acc (&acc)[TW][KT], __amdgpu_buffer_rsrc_t rs,
+                                int rows_p, int tile0, int ncolp, int cbase, const Inputs &in,
+                                const int *__restrict__ codes, int *__restrict__ tbl,
+                                const double *__restrict__ x, const double *__restrict__ t, int k,
+                                const BLane<KT> &bl) {
+  if (ncolp <= 0) return;
+  constexpr int U = group_quads<KT, NT>();
+  const uint32_t cs = (uint32_t)rows_p * Entry<DT>::es;
+  uint32_t r_es[NT];
+#pragma unroll
+  for (int m = 0; m < NT; ++m) r_es[m] = (uint32_t)((tile0 + m) * 16 + bl.j0) * Entry<DT>::es;
+  if constexpr (PHASE == 1) {
+    window_mma<Acc, 1, DT, KT, TW, NT, U>(acc, rs, cs, r_es, 0, ncolp, in.z0.src, nullptr, 0, x,
+                                          t, k, bl);
+  } else {
+    for (int w0 = 0; w0 < ncolp; w0 += kWin) {
+      const int w1 = min(ncolp, w0 + kWin);
+      if (w0 > 0) __syncthreads();  // the previous window's codes are read
+      for (int c = w0 + (int)threadIdx.x; c < w1; c += 64)
+        tbl[c - w0] = input_code(in, codes, cbase + c);
+      __syncthreads();
+      window_mma<Acc, 2, DT, KT, TW, NT, U>(acc, rs, cs, r_es, w0, w1, 0, tbl, w0, x, t, k, bl);
     }
   }
 }
 
-// PHASE 1: t[outidx[out_base + row] * k + j];  PHASE 2: y[(out_base + row) * k + j].
+// Both parts of the item for NT tiles (the unit's tile count, fixed at
+// compile time so that the accumulators stay in place); an item stored
+// with one type has one part.
+template <typename Acc, int PHASE, int DT0, int DT1, int KT, int TW, int NT>
+__device__ inline void unit_mma(typename Mfma<Acc>::acc (&acc)[TW][KT], const HmvWork &w,
+                                __amdgpu_buffer_rsrc_t rs0, __amdgpu_buffer_rsrc_t rs1, int tile0,
+                                const Inputs &in, const int *__restrict__ codes,
+                                int *__restrict__ tbl, const double *__restrict__ x,
+                                const double *__restrict__ t, int k, const BLane<KT> &bl) {
+  part_mma<Acc, PHASE, DT0, KT, TW, NT>(acc, rs0, w.rows_p, tile0, w.ncol0, 0, in, codes, tbl, x,
+                                        t, k, bl);
+  if constexpr (DT0 != DT1)
+    part_mma<Acc, PHASE, DT1, KT, TW, NT>(acc, rs1, w.rows_p, tile0, w.ncol1, w.ncol0, in, codes,
+                                          tbl, x, t, k, bl);
+}
+
+// One wave per unit {item, tile0 << 8 | nt}: tiles tile0 .. tile0+nt-1 of
+// the item.  PHASE 1: t[outidx[out_base + row] * k + j];  PHASE 2:
+// y[(out_base + row) * k + j].
 template <typename Acc, int PHASE, int DT0, int DT1, int KT>
 __global__ __launch_bounds__(64) void hmv_matmat_kernel(
-    const char *__restrict__ arena, const HmvWork *__restrict__ works, const int2 *__restrict__ tiles,
+    const char *__restrict__ arena, const HmvWork *__restrict__ works, const int2 *__restrict__ units,
     const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ codes,
     const double *__restrict__ t_in, const int *__restrict__ outidx, double *__restrict__ t_out,
     double *__restrict__ y, int k, double beta, const int *__restrict__ skip) {
   if (skip && *skip) return;
-  constexpr int KP = KT * 16;
-  constexpr int CH = (kZBytes / (int)sizeof(Acc) / KP - 1) & ~3;  // columns per chunk (multiple of 4)
-  static_assert(CH >= 4, "chunk holds at least one column quad");
-  __shared__ __attribute__((aligned(16))) Acc z[(CH + 1) * KP];
-  __shared__ long long zsrc[CH];
+  constexpr int TW = Shape<KT>::TW;
   __shared__ int runs[2 * HMV_RUN_MAX];
-  const int2 tile = tiles[blockIdx.x];
-  const HmvWork w = works[tile.x];
+  __shared__ int tbl[PHASE == 2 ? kWin : 1];
+  const int2 unit = units[blockIdx.x];
+  const HmvWork w = works[unit.x];
+  const int tile0 = unit.y >> 8, nt = unit.y & 255;
   const int lane = threadIdx.x;
-  const int r = tile.y * 16 + (lane & 15);  // this lane's A row
-  const int ncols = w.ncol0 + w.ncol1;
+  BLane<KT> bl;
+  bl.j0 = lane & 15;
+#pragma unroll
+  for (int tt = 0; tt < KT; ++tt) bl.jt[tt] = min(16 * tt + bl.j0, k - 1);
   const Inputs in = load_inputs<64>(w, zsegs, runs);
   const __amdgpu_buffer_rsrc_t rs0 =
       rsrc(arena + w.off0, (uint32_t)((size_t)w.rows_p * w.ncol0 * Entry<DT0>::es));
   const __amdgpu_buffer_rsrc_t rs1 =
       rsrc(arena + (w.ncol1 > 0 ? w.off1 : w.off0),
            (uint32_t)((size_t)w.rows_p * w.ncol1 * Entry<DT1>::es));
-  constexpr int NA = acc_sets<KT>();
-  typename Mfma<Acc>::acc acc[NA][KT];
+  typename Mfma<Acc>::acc acc[TW][KT];
 #pragma unroll
-  for (int sa = 0; sa < NA; ++sa)
+  for (int m = 0; m < TW; ++m)
 #pragma unroll
-    for (int tt = 0; tt < KT; ++tt) acc[sa][tt] = typename Mfma<Acc>::acc{0, 0, 0, 0};
-  for (int c0 = 0; c0 < ncols; c0 += CH) {
-    const int cw = min(CH, ncols - c0);
-    if (c0 > 0) __syncthreads();
-    stage<Acc, KP>(z, zsrc, in, c0, cw, k, x, codes, t_in);
-    __syncthreads();
-    if (c0 < w.ncol0)
-      part_mma<Acc, DT0, KT>(acc, z, rs0, w.rows_p, r, w.ncol0, c0, min(c0 + cw, w.ncol0), c0, cw);
-    if (c0 + cw > w.ncol0) {
-      const int lo = max(c0, w.ncol0) - w.ncol0;
-      part_mma<Acc, DT1, KT>(acc, z, rs1, w.rows_p, r, w.ncol1, lo, c0 + cw - w.ncol0,
-                             c0 - w.ncol0, cw);
-    }
+    for (int tt = 0; tt < KT; ++tt) acc[m][tt] = typename Mfma<Acc>::acc{0, 0, 0, 0};
+  if (nt >= TW)
+    unit_mma<Acc, PHASE, DT0, DT1, KT, TW, TW>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in, k,
+                                               bl);
+  else if constexpr (TW == 2)
+    unit_mma<Acc, PHASE, DT0, DT1, KT, TW, 1>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in, k,
+                                              bl);
+  else if constexpr (TW == 4) {
+    if (nt == 1)
+      unit_mma<Acc, PHASE, DT0, DT1, KT, TW, 1>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
+                                                k, bl);
+    else if (nt == 2)
+      unit_mma<Acc, PHASE, DT0, DT1, KT, TW, 2>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
+                                                k, bl);
+    else
+      unit_mma<Acc, PHASE, DT0, DT1, KT, TW, 3>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
+                                                k, bl);
   }
-#pragma unroll
-  for (int sa = 1; sa < NA; ++sa)
-#pragma unroll
-    for (int tt = 0; tt < KT; ++tt) acc[0][tt] += acc[sa][tt];
   // lane holds rows Mfma::row(lane, i), column lane%16 of each 16 x 16 tile
   const int j0 = lane & 15;
 #pragma unroll
-  for (int tt = 0; tt < KT; ++tt) {
-    const int j = tt * 16 + j0;
-    if (j >= k) continue;
+  for (int m = 0; m < TW; ++m) {
+    if (m >= nt) continue;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = tile.y * 16 + Mfma<Acc>::row(lane, i);
-      if (row >= w.rows) continue;
-      const double v = (double)acc[0][tt][i];
-      if constexpr (PHASE == 1) {
-        t_out[(size_t)outidx[w.out_base + row] * k + j] = v;
-      } else {
-        double *py = y + (size_t)(w.out_base + row) * k + j;
-        *py = beta == 0.0 ? v : __builtin_fma(beta, *py, v);
+    for (int tt = 0; tt < KT; ++tt) {
+      const int j = tt * 16 + j0;
+      if (j >= k) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = (tile0 + m) * 16 + Mfma<Acc>::row(lane, i);
+        if (row >= w.rows) continue;
+        const double v = (double)acc[m][tt][i];
+        if constexpr (PHASE == 1) {
+          t_out[(size_t)outidx[w.out_base + row] * k + j] = v;
+        } else {
+          double *py = y + (size_t)(w.out_base + row) * k + j;
+          *py = beta == 0.0 ? v : __builtin_fma(beta, *py, v);
+        }
       }
     }
   }
@@ -240,16 +334,19 @@ __global__ __launch_bounds__(256) void hmv_tfinal_block_kernel(const HmvTFin *__
 }
 
 template <typename Acc, int PHASE, int DT0, int DT1>
-void launch_tiles(int kt, const HmvLaunch &L, const int2 *tiles, int ntiles, const HmvWork *works,
+void launch_units(int kt, const HmvLaunch &L, const HmvUnits &U, const HmvWork *works,
                   const double *x, double *t, double *y, int k, double beta, const int *skip,
                   hipStream_t s) {
-  if (ntiles <= 0) return;
   const double *t_in = PHASE == 2 ? t : nullptr;
   double *t_out = PHASE == 1 ? t : nullptr;
 #define HMV_MM_LAUNCH(KT)                                                                         \
-  hipLaunchKernelGGL((hmv_matmat_kernel<Acc, PHASE, DT0, DT1, KT>), dim3(ntiles), dim3(64), 0, s, \
-                     L.arena, works, tiles, L.zseg, x, L.codes, t_in, L.outidx, t_out, y, k, beta,  \
-                     skip)
+  do {                                                                                            \
+    const int w = Shape<KT>::LOG_TW, n = U.n[PHASE - 1][w];                                       \
+    if (n > 0)                                                                                    \
+      hipLaunchKernelGGL((hmv_matmat_kernel<Acc, PHASE, DT0, DT1, KT>), dim3(n), dim3(64), 0, s,  \
+                         L.arena, works, U.u[PHASE - 1][w], L.zseg, x, L.codes, t_in, L.outidx,   \
+                         t_out, y, k, beta, skip);                                                \
+  } while (0)
   if (kt == 1)
     HMV_MM_LAUNCH(1);
   else if (kt == 2)
@@ -262,38 +359,32 @@ void launch_tiles(int kt, const HmvLaunch &L, const int2 *tiles, int ntiles, con
 }
 
 template <typename Acc, int DT0, int DT1>
-void launch_block(const HmvLaunch &L, const int2 *tiles1, int ntiles1, const int2 *tiles2,
-                  int ntiles2, int k, const double *x, double *t, double *y, double beta,
-                  const int *skip, hipStream_t s) {
+void launch_block(const HmvLaunch &L, const HmvUnits &U, int k, const double *x, double *t,
+                  double *y, double beta, const int *skip, hipStream_t s) {
   const int kt = (k + 15) / 16;
-  launch_tiles<Acc, 1, DT1, DT1>(kt, L, tiles1, ntiles1, L.work1, x, t, nullptr, k, 0.0, skip, s);
+  launch_units<Acc, 1, DT1, DT1>(kt, L, U, L.work1, x, t, nullptr, k, 0.0, skip, s);
   if (L.ntfin > 0) {
     const int64_t n = (int64_t)L.ntfin * k;
     hipLaunchKernelGGL(hmv_tfinal_block_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        L.tfin, L.ntfin, k, t, skip);
   }
-  launch_tiles<Acc, 2, DT0, DT1>(kt, L, tiles2, ntiles2, L.work2, x, t, y, k, beta, skip, s);
+  launch_units<Acc, 2, DT0, DT1>(kt, L, U, L.work2, x, t, y, k, beta, skip, s);
 }
 
 }  // namespace
 
 // Y = H X + beta*Y for k (1..128) interleaved right-hand sides; t holds
 // tpart_size * k doubles.
-extern "C" hipError_t hmv_launch_block(const HmvLaunch *L, const int2 *tiles1, int ntiles1,
-                                       const int2 *tiles2, int ntiles2, int k, const double *x,
-                                       double *t, double *y, double beta, const int *skip,
-                                       hipStream_t stream) {
+extern "C" hipError_t hmv_launch_block(const HmvLaunch *L, const HmvUnits *units, int k,
+                                       const double *x, double *t, double *y, double beta,
+                                       const int *skip, hipStream_t stream) {
   if (L->mode == 0)
-    launch_block<double, HMV_DT_F64, HMV_DT_F64>(*L, tiles1, ntiles1, tiles2, ntiles2, k, x, t, y,
-                                                 beta, skip, stream);
+    launch_block<double, HMV_DT_F64, HMV_DT_F64>(*L, *units, k, x, t, y, beta, skip, stream);
   else if (L->mode == 1)
-    launch_block<float, HMV_DT_F32, HMV_DT_F32>(*L, tiles1, ntiles1, tiles2, ntiles2, k, x, t, y,
-                                                beta, skip, stream);
+    launch_block<float, HMV_DT_F32, HMV_DT_F32>(*L, *units, k, x, t, y, beta, skip, stream);
   else if (L->mode == 2)
-    launch_block<float, HMV_DT_F32, HMV_DT_BF16>(*L, tiles1, ntiles1, tiles2, ntiles2, k, x, t, y,
-                                                 beta, skip, stream);
+    launch_block<float, HMV_DT_F32, HMV_DT_BF16>(*L, *units, k, x, t, y, beta, skip, stream);
   else
-    launch_block<float, HMV_DT_BF16, HMV_DT_BF16>(*L, tiles1, ntiles1, tiles2, ntiles2, k, x, t, y,
-                                                  beta, skip, stream);
+    launch_block<float, HMV_DT_BF16, HMV_DT_BF16>(*L, *units, k, x, t, y, beta, skip, stream);
   return hipGetLastError();
 }
