@@ -172,9 +172,10 @@ def main():
     ms_read = H.stream_test(0, cal_bytes, reps=10)
     ms_copy = H.stream_test(1, cal_bytes // 2, reps=10)
     calibration = {"read_GBs": round(cal_bytes / ms_read / 1e6, 1),
-                   "copy_GBs": round(2 * (cal_bytes // 2) / ms_copy / 1e6, 1), "bytes": cal_bytes}
-    log("calibration: plain read %.1f GB/s, plain copy %.1f GB/s"
-        % (calibration["read_GBs"], calibration["copy_GBs"]))
+                   "copy_GBs": round(2 * (cal_bytes // 2) / ms_copy / 1e6, 1), "bytes": cal_bytes,
+                   "mfma_f64_TFLOPs": round(H.mfma_rate(), 1)}
+    log("calibration: plain read %.1f GB/s, plain copy %.1f GB/s, FP64 MFMA %.1f TFLOP/s"
+        % (calibration["read_GBs"], calibration["copy_GBs"], calibration["mfma_f64_TFLOPs"]))
 
     result = {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
@@ -338,17 +339,20 @@ def study_327680(H, log):
     st = a64.stats
     E = st["entries_lr"] + st["entries_dense"]
     single_ms = rates["f64"]["ms"]
-    mm = {}
+    mfma_tf = H.mfma_rate()  # measured FP64 MFMA ceiling
+    mm = {"mfma_f64_TFLOPs_measured": round(mfma_tf, 1)}
     rng = np.random.default_rng(3)
     for k in (8, 32, 128):
         X = H.DeviceBuffer.from_array(rng.uniform(-1, 1, nd * k))
         Y = H.DeviceBuffer.from_array(np.zeros(nd * k))
         ms = a64.apply_block_timed(X.ptr, Y.ptr, k, 5)
+        tf = 2.0 * E * k / ms / 1e9
         mm[str(k)] = {"ms": round(ms, 4), "ms_per_rhs": round(ms / k, 4),
                       "speedup_per_rhs_vs_single": round(single_ms * k / ms, 2),
-                      "TFLOPs": round(2.0 * E * k / ms / 1e9, 2)}
-        log("s=7 H-matmat k=%d: %.3f ms (%.2f ms/RHS, %.1fx single, %.1f TFLOP/s)"
-            % (k, ms, ms / k, single_ms * k / ms, 2.0 * E * k / ms / 1e9))
+                      "TFLOPs": round(tf, 2), "frac_of_measured_mfma": round(tf / mfma_tf, 3)}
+        log("s=7 H-matmat k=%d: %.3f ms (%.2f ms/RHS, %.1fx single, %.1f TFLOP/s = %.0f%% of the "
+            "measured FP64 MFMA rate %.1f)" % (k, ms, ms / k, single_ms * k / ms, tf,
+                                               100 * tf / mfma_tf, mfma_tf))
         X.free()
         Y.free()
     out["hmatmat_fp64"] = mm
@@ -396,7 +400,9 @@ def hmatmat_only(H, log, reps=5):
     st = a.stats
     E = st["entries_lr"] + st["entries_dense"]
     rng = np.random.default_rng(3)
-    out = {}
+    mfma_tf = H.mfma_rate()
+    log("measured FP64 MFMA rate: %.1f TFLOP/s" % mfma_tf)
+    out = {"mfma_f64_TFLOPs_measured": round(mfma_tf, 1)}
     for k in (8, 32, 128):
         X = H.DeviceBuffer.from_array(rng.uniform(-1, 1, hm.nd * k))
         Y = H.DeviceBuffer.from_array(np.zeros(hm.nd * k))

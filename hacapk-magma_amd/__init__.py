@@ -625,6 +625,17 @@ def stream_test(kind, nbytes, reps=20):
     return ms.value
 
 
+def mfma_rate(mfmas_per_wave=65536, reps=5):
+    """Measured FP64 MFMA rate in TFLOP/s (hmv_stream_test kind 2: 16x16x4
+    FP64 MFMAs on register operands, 4 waves per SIMD)."""
+    dst = DeviceBuffer(64)
+    ms = C.c_double()
+    _check(hmv_lib().hmv_stream_test(2, None, dst.ptr, int(mfmas_per_wave), int(reps),
+                                     C.byref(ms)), "hmv_stream_test")
+    dst.free()
+    return 2048.0 * mfmas_per_wave * 4096 / (ms.value * 1e-3) / 1e12
+
+
 def device_info():
     L = hmv_lib()
     buf = C.create_string_buffer(256)

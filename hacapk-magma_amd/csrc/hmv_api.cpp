@@ -715,7 +715,8 @@ int hmv_autotune(int nd, int nlf, const void *leaves, int leaf_stride, int preci
 }
 
 int hmv_stream_test(int kind, const void *src, void *dst, size_t bytes, int reps, double *ms) {
-  if (!src || (kind != 0 && !dst) || reps < 1 || bytes < 16) {
+  if (kind < 0 || kind > 2 || (kind != 2 && !src) || (kind != 0 && !dst) || reps < 1 ||
+      bytes < 16 || (kind == 2 && (bytes % 8 != 0 || bytes / 8 > (size_t)1 << 24))) {
     set_error("hmv_stream_test: invalid argument");
     return HMV_ERR_ARG;
   }

@@ -80,6 +80,7 @@ bool download_plan(const hmv_arena_s *a, Plan &P, std::string &err);
 int ensure_tk(hmv_arena_s *a, int k);
 }  // namespace hmv
 
+#define HMV_MFMA_RATE_WAVES (256 * 4 * 4)  // waves of the MFMA-rate calibration
 extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *dst, size_t bytes,
                                              hipStream_t stream);
 extern "C" hipError_t hmv_launch_apply(const HmvLaunch *L, int phases, const double *x, double *y,

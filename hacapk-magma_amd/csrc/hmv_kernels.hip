@@ -360,6 +360,24 @@ __global__ __launch_bounds__(HMV_WG_THREADS) void hmv_copy_kernel(const double2 
     dst[i] = src[i];
 }
 
+// FP64 MFMA rate: `rounds` x 8 independent 16x16x4 FP64 MFMAs per wave on
+// register operands (eight accumulator chains, so that no MFMA waits for
+// its predecessor's result).
+typedef double hmv_d4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(64) void hmv_mfma_rate_kernel(int rounds, double *__restrict__ out) {
+  hmv_d4 acc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) acc[c] = hmv_d4{0, 0, 0, 0};
+  const double a = 1.0 + threadIdx.x * 1e-9, b = 1e-3;
+  for (int i = 0; i < rounds; ++i)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) s += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
+  if (s == 1.2345e-300) out[0] = s;  // keeps the MFMAs alive; never true here
+}
+
 }  // namespace
 
 extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *dst, size_t bytes,
@@ -369,9 +387,12 @@ extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *ds
   if (kind == 0)
     hipLaunchKernelGGL(hmv_read_kernel, dim3(grid), dim3(HMV_WG_THREADS), 0, stream,
                        (const double2 *)src, n, (double *)dst);
-  else
+  else if (kind == 1)
     hipLaunchKernelGGL(hmv_copy_kernel, dim3(grid), dim3(HMV_WG_THREADS), 0, stream,
                        (const double2 *)src, (double2 *)dst, n);
+  else  // 4 waves per SIMD on 256 CUs; `bytes` counts MFMAs per wave
+    hipLaunchKernelGGL(hmv_mfma_rate_kernel, dim3(HMV_MFMA_RATE_WAVES), dim3(64), 0, stream,
+                       (int)(bytes / 8), (double *)dst);
   return hipGetLastError();
 }
 

@@ -10,13 +10,13 @@
 // 8-byte (or 4/2-byte) load per lane and tile gives the 16 x 4 A fragment
 // (lane l: row l%16 of column l/16 of the column-major block), and KT loads
 // per lane give the 4 x 16 B fragments x[column][16 tt + l%16] straight
-// from the interleaved inputs (L2-resident: the units of an XCD are
-// neighbouring items, so they share columns).  Each (tile, RHS tile) pair
-// then takes one MFMA (16x16x4; FP64, or FP32 for FP32/BF16 arenas) into
-// its 16 x 16 accumulator; the B fragment is shared by the group's tiles.
-// Loads run one group of U quads ahead of the MFMAs.  Rows and columns
-// outside the block load zero through the buffer range check (out-of-range
-// vector offset) and a masked input load.
+// from the interleaved inputs, through a per-wave LDS table of the columns'
+// input addresses (L2-resident inputs: the units of an XCD are neighbouring
+// items, so they share columns).  Each (tile, RHS tile) pair then takes one
+// MFMA (16x16x4; FP64, or FP32 for FP32/BF16 arenas) into its 16 x 16
+// accumulator; the B fragment is shared by the group's tiles.  Loads run
+// one group of U quads ahead of the MFMAs.  Columns past the block load
+// A = 0 through the buffer range check and B = 0 from a zero row.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -92,7 +92,8 @@ struct Entry<HMV_DT_BF16> {
 
 // Tile-group width (tiles sharing one B fragment) by RHS tiles, and column
 // quads per prefetch group by RHS tiles and active tiles (two groups of
-// U * (NT + KT) loads per lane are in flight; U divides the code window).
+// U * (NT + KT) loads per lane are live; measured on MI355X, deeper or
+// wider groups do not gain).
 template <int KT>
 struct Shape {
   static constexpr int TW = KT <= 1 ? 4 : KT <= 4 ? 2 : 1;
@@ -100,20 +101,28 @@ struct Shape {
 };
 template <int KT, int NT>
 constexpr int group_quads() {
-  return KT + NT <= 3 ? 8 : KT + NT <= 6 ? 4 : 2;
+  return KT + NT <= 6 ? 4 : 2;
 }
 
-constexpr int kWin = 1024;  // phase-2 input codes tabled per window (LDS, per wave)
-__device__ double g_zero_inputs[128];  // B source of the columns past a part (never written)
+// Global-address-space inputs: loads through these are global loads, which
+// count on vmcnt only (a generic pointer read from LDS would give flat
+// loads, which also count on lgkmcnt and so serialise the LDS reads with
+// the inputs in flight).
+typedef const double __attribute__((address_space(1))) gdouble;
 
-// Per-wave constants of the B loads: lane l reads input
-// jt[tt] = min(16 tt + l%16, k - 1) of RHS tile tt (inputs j >= k only
-// feed accumulator columns that are not stored; the clamp keeps the read
-// inside the column's k values).
+constexpr int kWin = 1024;  // input pointers tabled per window (LDS, per wave)
+constexpr int kPad = 32;    // zero-input entries past a window (one group of columns)
+__device__ double g_zero_inputs[128];  // B source of the columns past a window (never written)
+
+// Per-wave constants of the B loads.  Lane l reads input 16 tt + l%16 of
+// RHS tile tt; with EXACT (k == 16 KT) straight from its column pointer,
+// otherwise at jd[tt] = min(16 tt + l%16, k - 1) - l%16 past it (inputs
+// j >= k only feed accumulator columns that are not stored; the clamp
+// keeps the read inside the column's k values).
 template <int KT>
 struct BLane {
-  int j0;      // l % 16
-  int jt[KT];
+  int j0;  // l % 16
+  int jd[KT];
 };
 
 // One prefetch group: U column quads of A (NT tiles) and B (KT RHS tiles),
@@ -124,36 +133,27 @@ struct Group {
   double b[U][KT];
 };
 
-// Loads the group of part columns [q0, q0 + 4U) (part columns below
-// `lim`): lane l takes column q0 + 4u + l/16.  A: row 16 (tile0 + m) + l%16
-// at byte offset col * cs + r_es[m]; a column past the part lies past the
-// buffer range and loads 0, and rows past the block only feed accumulator
-// rows that are not stored.  B: the column's inputs from x or t (PHASE 1:
-// x row xs + col; PHASE 2: the code table), or zeros past `lim`.
-template <int PHASE, int DT, int KT, int NT, int U>
+// Loads the group of part columns [q0, q0 + 4U): lane l takes column
+// q0 + 4u + l/16.  A: row 16 (tile0 + m) + l%16 at byte offset
+// col * cs + r_es + 16 m es; a column past the part lies past the buffer
+// range and loads 0, and rows past the block only feed accumulator rows
+// that are not stored.  B: from the column's input pointer tbl[col - tbase]
+// (zeros past the window).
+template <bool EXACT, int DT, int KT, int NT, int U>
 __device__ inline void load_group(Group<DT, KT, NT, U> &g, __amdgpu_buffer_rsrc_t rs, uint32_t cs,
-                                 const uint32_t (&r_es)[NT], int q0, int lim, int xs,
-                                 const int *__restrict__ tbl, int tbase,
-                                 const double *__restrict__ x, const double *__restrict__ t,
-                                 int k, const BLane<KT> &bl) {
+                                 uint32_t r_es, int q0, gdouble *const *__restrict__ tbl,
+                                 int tbase, const BLane<KT> &bl) {
   using E = Entry<DT>;
   const int cq = threadIdx.x >> 4;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int col = q0 + 4 * u + cq;
-    const uint32_t cb = (uint32_t)col * cs;
+    const uint32_t cb = (uint32_t)col * cs + r_es;
 #pragma unroll
-    for (int m = 0; m < NT; ++m) g.a[u][m] = E::load_raw(rs, cb + r_es[m]);
-    const double *src;
-    if constexpr (PHASE == 1) {
-      src = x + (size_t)(xs + col) * k;
-    } else {
-      const int code = tbl[min(col, lim - 1) - tbase];
-      src = code >= 0 ? x + (size_t)code * k : t + (size_t)(-1 - code) * k;
-    }
-    if (col >= lim) src = g_zero_inputs;
+    for (int m = 0; m < NT; ++m) g.a[u][m] = E::load_raw(rs, cb + (uint32_t)(m * 16 * E::es));
+    gdouble *p = tbl[col - tbase] + bl.j0;
 #pragma unroll
-    for (int tt = 0; tt < KT; ++tt) g.b[u][tt] = src[bl.jt[tt]];
+    for (int tt = 0; tt < KT; ++tt) g.b[u][tt] = EXACT ? p[16 * tt] : p[bl.jd[tt]];
   }
 }
 
@@ -171,83 +171,79 @@ __device__ inline void mma_group(typename Mfma<Acc>::acc (&acc)[TW][KT],
     }
 }
 
-// Accumulates the product of NT tiles over part columns [lo, hi), U quads
-// at a time, the next group's loads in flight while the current one
-// multiplies.
-template <typename Acc, int PHASE, int DT, int KT, int TW, int NT, int U>
-__device__ inline void window_mma(typename Mfma<Acc>::acc (&acc)[TW][KT],
-                                  __amdgpu_buffer_rsrc_t rs, uint32_t cs,
-                                  const uint32_t (&r_es)[NT], int lo, int hi, int xs,
-                                  const int *__restrict__ tbl, int tbase,
-                                  const double *__restrict__ x, const double *__restrict__ t, int k,
-                                  const BLane<KT> &bl) {
-  constexpr int S = 4 * U;  // columns per group
-  Group<DT, KT, NT, U> g0, g1;
-  load_group<PHASE>(g0, rs, cs, r_es, lo, hi, xs, tbl, tbase, x, t, k, bl);
-  int q0 = lo;
-  for (; q0 + 2 * S < hi; q0 += 2 * S) {  // g0 holds group q0; two more follow
-    load_group<PHASE>(g1, rs, cs, r_es, q0 + S, hi, xs, tbl, tbase, x, t, k, bl);
-    mma_group<Acc>(acc, g0);
-    load_group<PHASE>(g0, rs, cs, r_es, q0 + 2 * S, hi, xs, tbl, tbase, x, t, k, bl);
-    mma_group<Acc>(acc, g1);
+// Tables the input pointers of part columns [w0, w1) (item columns cbase +
+// ...) at tbl[0 ..), and zeros for the kPad entries that follow.
+__device__ inline void fill_window(gdouble **__restrict__ tbl, const Inputs &in,
+                                   const int *__restrict__ codes, int cbase, int w0, int w1,
+                                   const double *__restrict__ x, const double *__restrict__ t,
+                                   int k) {
+  for (int i = (int)threadIdx.x; i < w1 - w0 + kPad; i += 64) {
+    const double *p = g_zero_inputs;
+    if (i < w1 - w0) {
+      const int code = input_code(in, codes, cbase + w0 + i);
+      p = code >= 0 ? x + (size_t)code * k : t + (size_t)(-1 - code) * k;
+    }
+    tbl[i] = (gdouble *)p;
   }
-  const bool last2 = q0 + S < hi;  // one or two groups left
-  if (last2) load_group<PHASE>(g1, rs, cs, r_es, q0 + S, hi, xs, tbl, tbase, x, t, k, bl);
-  mma_group<Acc>(acc, g0);
-  if (last2) mma_group<Acc>(acc, g1);
 }
 
-// One part (ncolp columns stored with DT at rs, item columns cbase + [0,
-// ncolp)) for NT tiles from tile0.  Phase 2 tables the input codes of
-// kWin columns at a time in LDS (one binary search per column), then runs
-// the window's groups.
-template <typename Acc, int PHASE, int DT, int KT, int TW, int NT>
+// Accumulates the product of NT tiles over one part (ncolp columns stored
+// with DT at rs, item columns cbase + [0, ncolp)), kWin columns at a time:
+// the window's input pointers are tabled in LDS, then its column groups
+// run U quads at a time, the next group's loads in flight while the
+// current one multiplies.
+template <typename Acc, bool EXACT, int DT, int KT, int TW, int NT>
 __device__ inline void part_mma(typename Mfma<Acc>::acc (&acc)[TW][KT], __amdgpu_buffer_rsrc_t rs,
                                 int rows_p, int tile0, int ncolp, int cbase, const Inputs &in,
-                                const int *__restrict__ codes, int *__restrict__ tbl,
+                                const int *__restrict__ codes, gdouble **__restrict__ tbl,
                                 const double *__restrict__ x, const double *__restrict__ t, int k,
                                 const BLane<KT> &bl) {
-  if (ncolp <= 0) return;
   constexpr int U = group_quads<KT, NT>();
+  constexpr int S = 4 * U;  // columns per group
+  static_assert(S <= kPad, "a group past the window reads zero entries");
   const uint32_t cs = (uint32_t)rows_p * Entry<DT>::es;
-  uint32_t r_es[NT];
-#pragma unroll
-  for (int m = 0; m < NT; ++m) r_es[m] = (uint32_t)((tile0 + m) * 16 + bl.j0) * Entry<DT>::es;
-  if constexpr (PHASE == 1) {
-    window_mma<Acc, 1, DT, KT, TW, NT, U>(acc, rs, cs, r_es, 0, ncolp, in.z0.src, nullptr, 0, x,
-                                          t, k, bl);
-  } else {
-    for (int w0 = 0; w0 < ncolp; w0 += kWin) {
-      const int w1 = min(ncolp, w0 + kWin);
-      if (w0 > 0) __syncthreads();  // the previous window's codes are read
-      for (int c = w0 + (int)threadIdx.x; c < w1; c += 64)
-        tbl[c - w0] = input_code(in, codes, cbase + c);
-      __syncthreads();
-      window_mma<Acc, 2, DT, KT, TW, NT, U>(acc, rs, cs, r_es, w0, w1, 0, tbl, w0, x, t, k, bl);
+  const uint32_t r_es = (uint32_t)(tile0 * 16 + bl.j0) * Entry<DT>::es;
+  for (int w0 = 0; w0 < ncolp; w0 += kWin) {
+    const int w1 = min(ncolp, w0 + kWin);
+    __syncthreads();  // the table's previous contents are read
+    fill_window(tbl, in, codes, cbase, w0, w1, x, t, k);
+    __syncthreads();
+    Group<DT, KT, NT, U> g0, g1;
+    load_group<EXACT>(g0, rs, cs, r_es, w0, tbl, w0, bl);
+    int q0 = w0;
+    for (; q0 + 2 * S < w1; q0 += 2 * S) {  // g0 holds group q0; two more follow
+      load_group<EXACT>(g1, rs, cs, r_es, q0 + S, tbl, w0, bl);
+      mma_group<Acc>(acc, g0);
+      load_group<EXACT>(g0, rs, cs, r_es, q0 + 2 * S, tbl, w0, bl);
+      mma_group<Acc>(acc, g1);
     }
+    const bool last2 = q0 + S < w1;  // one or two groups left
+    if (last2) load_group<EXACT>(g1, rs, cs, r_es, q0 + S, tbl, w0, bl);
+    mma_group<Acc>(acc, g0);
+    if (last2) mma_group<Acc>(acc, g1);
   }
 }
 
 // Both parts of the item for NT tiles (the unit's tile count, fixed at
 // compile time so that the accumulators stay in place); an item stored
 // with one type has one part.
-template <typename Acc, int PHASE, int DT0, int DT1, int KT, int TW, int NT>
+template <typename Acc, bool EXACT, int DT0, int DT1, int KT, int TW, int NT>
 __device__ inline void unit_mma(typename Mfma<Acc>::acc (&acc)[TW][KT], const HmvWork &w,
                                 __amdgpu_buffer_rsrc_t rs0, __amdgpu_buffer_rsrc_t rs1, int tile0,
                                 const Inputs &in, const int *__restrict__ codes,
-                                int *__restrict__ tbl, const double *__restrict__ x,
+                                gdouble **__restrict__ tbl, const double *__restrict__ x,
                                 const double *__restrict__ t, int k, const BLane<KT> &bl) {
-  part_mma<Acc, PHASE, DT0, KT, TW, NT>(acc, rs0, w.rows_p, tile0, w.ncol0, 0, in, codes, tbl, x,
+  part_mma<Acc, EXACT, DT0, KT, TW, NT>(acc, rs0, w.rows_p, tile0, w.ncol0, 0, in, codes, tbl, x,
                                         t, k, bl);
   if constexpr (DT0 != DT1)
-    part_mma<Acc, PHASE, DT1, KT, TW, NT>(acc, rs1, w.rows_p, tile0, w.ncol1, w.ncol0, in, codes,
+    part_mma<Acc, EXACT, DT1, KT, TW, NT>(acc, rs1, w.rows_p, tile0, w.ncol1, w.ncol0, in, codes,
                                           tbl, x, t, k, bl);
 }
 
 // One wave per unit {item, tile0 << 8 | nt}: tiles tile0 .. tile0+nt-1 of
 // the item.  PHASE 1: t[outidx[out_base + row] * k + j];  PHASE 2:
 // y[(out_base + row) * k + j].
-template <typename Acc, int PHASE, int DT0, int DT1, int KT>
+template <typename Acc, int PHASE, int DT0, int DT1, int KT, bool EXACT>
 __global__ __launch_bounds__(64) void hmv_matmat_kernel(
     const char *__restrict__ arena, const HmvWork *__restrict__ works, const int2 *__restrict__ units,
     const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ codes,
@@ -256,7 +252,7 @@ __global__ __launch_bounds__(64) void hmv_matmat_kernel(
   if (skip && *skip) return;
   constexpr int TW = Shape<KT>::TW;
   __shared__ int runs[2 * HMV_RUN_MAX];
-  __shared__ int tbl[PHASE == 2 ? kWin : 1];
+  __shared__ gdouble *tbl[kWin + kPad];
   const int2 unit = units[blockIdx.x];
   const HmvWork w = works[unit.x];
   const int tile0 = unit.y >> 8, nt = unit.y & 255;
@@ -264,7 +260,7 @@ __global__ __launch_bounds__(64) void hmv_matmat_kernel(
   BLane<KT> bl;
   bl.j0 = lane & 15;
 #pragma unroll
-  for (int tt = 0; tt < KT; ++tt) bl.jt[tt] = min(16 * tt + bl.j0, k - 1);
+  for (int tt = 0; tt < KT; ++tt) bl.jd[tt] = min(16 * tt + bl.j0, k - 1) - bl.j0;
   const Inputs in = load_inputs<64>(w, zsegs, runs);
   const __amdgpu_buffer_rsrc_t rs0 =
       rsrc(arena + w.off0, (uint32_t)((size_t)w.rows_p * w.ncol0 * Entry<DT0>::es));
@@ -277,20 +273,20 @@ __global__ __launch_bounds__(64) void hmv_matmat_kernel(
 #pragma unroll
     for (int tt = 0; tt < KT; ++tt) acc[m][tt] = typename Mfma<Acc>::acc{0, 0, 0, 0};
   if (nt >= TW)
-    unit_mma<Acc, PHASE, DT0, DT1, KT, TW, TW>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in, k,
+    unit_mma<Acc, EXACT, DT0, DT1, KT, TW, TW>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in, k,
                                                bl);
   else if constexpr (TW == 2)
-    unit_mma<Acc, PHASE, DT0, DT1, KT, TW, 1>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in, k,
+    unit_mma<Acc, EXACT, DT0, DT1, KT, TW, 1>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in, k,
                                               bl);
   else if constexpr (TW == 4) {
     if (nt == 1)
-      unit_mma<Acc, PHASE, DT0, DT1, KT, TW, 1>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
+      unit_mma<Acc, EXACT, DT0, DT1, KT, TW, 1>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
                                                 k, bl);
     else if (nt == 2)
-      unit_mma<Acc, PHASE, DT0, DT1, KT, TW, 2>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
+      unit_mma<Acc, EXACT, DT0, DT1, KT, TW, 2>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
                                                 k, bl);
     else
-      unit_mma<Acc, PHASE, DT0, DT1, KT, TW, 3>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
+      unit_mma<Acc, EXACT, DT0, DT1, KT, TW, 3>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
                                                 k, bl);
   }
   // lane holds rows Mfma::row(lane, i), column lane%16 of each 16 x 16 tile
@@ -339,13 +335,19 @@ void launch_units(int kt, const HmvLaunch &L, const HmvUnits &U, const HmvWork *
                   hipStream_t s) {
   const double *t_in = PHASE == 2 ? t : nullptr;
   double *t_out = PHASE == 1 ? t : nullptr;
-#define HMV_MM_LAUNCH(KT)                                                                         \
-  do {                                                                                            \
-    const int w = Shape<KT>::LOG_TW, n = U.n[PHASE - 1][w];                                       \
-    if (n > 0)                                                                                    \
-      hipLaunchKernelGGL((hmv_matmat_kernel<Acc, PHASE, DT0, DT1, KT>), dim3(n), dim3(64), 0, s,  \
-                         L.arena, works, U.u[PHASE - 1][w], L.zseg, x, L.codes, t_in, L.outidx,   \
-                         t_out, y, k, beta, skip);                                                \
+#define HMV_MM_GO(KT, EX)                                                                      \
+  hipLaunchKernelGGL((hmv_matmat_kernel<Acc, PHASE, DT0, DT1, KT, EX>), dim3(n), dim3(64), 0, s, \
+                     L.arena, works, U.u[PHASE - 1][w], L.zseg, x, L.codes, t_in, L.outidx, t_out,  \
+                     y, k, beta, skip)
+#define HMV_MM_LAUNCH(KT)                              \
+  do {                                                 \
+    const int w = Shape<KT>::LOG_TW, n = U.n[PHASE - 1][w]; \
+    if (n > 0) {                                       \
+      if (k == 16 * KT)                                \
+        HMV_MM_GO(KT, true);                           \
+      else                                             \
+        HMV_MM_GO(KT, false);                          \
+    }                                                  \
   } while (0)
   if (kt == 1)
     HMV_MM_LAUNCH(1);
@@ -356,6 +358,7 @@ void launch_units(int kt, const HmvLaunch &L, const HmvUnits &U, const HmvWork *
   else
     HMV_MM_LAUNCH(8);
 #undef HMV_MM_LAUNCH
+#undef HMV_MM_GO
 }
 
 template <typename Acc, int DT0, int DT1>

@@ -275,7 +275,11 @@ int hmv_apply_timed(hmv_arena a, const double *x, double *y, int reps, double *m
 
 /* Bandwidth calibration: `reps` launches of a plain streaming kernel over
  * `bytes` bytes (kind 0: read-only sum of src; kind 1: copy src -> dst);
- * *ms receives the average launch time (HIP events). */
+ * *ms receives the average launch time (HIP events).  Kind 2 calibrates
+ * the FP64 MFMA rate instead: `bytes` (a multiple of 8, at most 2^27) FP64
+ * 16x16x4 MFMAs per wave on register operands, in HMV_MFMA_RATE_WAVES = 4096
+ * waves (4 per SIMD); src is unused, dst receives nothing (8 bytes).
+ * Flops per launch: 2048 * bytes * 4096. */
 int hmv_stream_test(int kind, const void *src, void *dst, size_t bytes, int reps, double *ms);
 
 /* ---- host-side plan inspection (no device needed) ----------------------

@@ -83,3 +83,16 @@ def test_argument_checks_before_any_device_call(H):
     assert L.hmv_plan_load(b"/nonexistent/arena.hmv", C.byref(p)) == -1
     assert b"cannot open" in L.hmv_last_error()
     assert L.hmv_arena_load(b"/nonexistent/arena.hmv", C.byref(p)) == -1
+
+
+def test_calibration_argument_checks(H):
+    """hmv_stream_test rejects unknown kinds and malformed MFMA-rate sizes
+    before any device call."""
+    L = H.hmv_lib()
+    ms = C.c_double()
+    buf = (C.c_char * 64)()
+    assert L.hmv_stream_test(3, buf, buf, 64, 1, C.byref(ms)) == -1
+    assert L.hmv_stream_test(2, None, buf, 20, 1, C.byref(ms)) == -1  # not a multiple of 8
+    assert L.hmv_stream_test(2, None, buf, (1 << 27) + 8, 1, C.byref(ms)) == -1
+    assert L.hmv_stream_test(2, None, None, 64, 1, C.byref(ms)) == -1
+    assert b"hmv_stream_test" in L.hmv_last_error()

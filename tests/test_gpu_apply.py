@@ -237,3 +237,18 @@ def test_multi_arena_file_keeps_instances(gpu, O, sphere3, sphere4, tmp_path):
     assert np.array_equal(b.matvec(x), m.matvec(x))
     b.free()
     m.free()
+
+
+def test_calibration_kernels(gpu):
+    """The plain read/copy and FP64 MFMA-rate calibrations run and give
+    plausible rates (bandwidth below 8 TB/s, MFMA rate below the published
+    78.6 TFLOP/s FP64 peak)."""
+    nbytes = 256 << 20
+    ms_read = gpu.stream_test(0, nbytes, reps=3)
+    ms_copy = gpu.stream_test(1, nbytes, reps=3)
+    tf = gpu.mfma_rate(mfmas_per_wave=8192, reps=2)
+    print("read %.0f GB/s, copy %.0f GB/s, FP64 MFMA %.1f TFLOP/s"
+          % (nbytes / ms_read / 1e6, 2 * nbytes / ms_copy / 1e6, tf), flush=True)
+    assert 0 < nbytes / ms_read / 1e6 < 8000 and 0 < 2 * nbytes / ms_copy / 1e6 < 8000
+    assert 1.0 < tf < 78.6
+

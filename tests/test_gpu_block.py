@@ -16,7 +16,7 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("k", [1, 3, 8, 16, 32, 40, 128])
+@pytest.mark.parametrize("k", [1, 3, 8, 16, 32, 40, 64, 72, 128])
 def test_matmat_f64_columns(gpu, O, two_spheres4, k):
     hm = two_spheres4
     a = gpu.Arena(hm, "f64")
