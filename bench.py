@@ -412,6 +412,17 @@ def hmatmat_only(H, log, reps=5):
         X.free()
         Y.free()
     a.free()
+    # narrow-storage arenas (FP32 MFMA), k = 32
+    for prec in ("f32", "bf16_all"):
+        a = H.Arena(hm, prec)
+        X = H.DeviceBuffer.from_array(rng.uniform(-1, 1, hm.nd * 32))
+        Y = H.DeviceBuffer.from_array(np.zeros(hm.nd * 32))
+        ms = a.apply_block_timed(X.ptr, Y.ptr, 32, reps)
+        out[prec + "_32"] = {"ms": round(ms, 4)}
+        log("H-matmat %s k=32: %.3f ms" % (prec, ms))
+        X.free()
+        Y.free()
+        a.free()
     return out
 
 
