@@ -259,6 +259,9 @@ def hmv_lib():
         L.c_hacapk_adot_body_lfmtx_batch_.restype = None
         L.c_hacapk_adot_body_lfdel_batch_.argtypes = [C.POINTER(RefLeafmtxp)]
         L.c_hacapk_adot_body_lfdel_batch_.restype = None
+        for n in ("c_hacapk_adot_body_lfmtx_", "c_hacapk_adot_body_lfmtx_gpu_"):
+            getattr(L, n).argtypes = [_D, C.POINTER(RefLeafmtxp), _D, _D]
+            getattr(L, n).restype = None
         L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes = [C.POINTER(RefLeafmtxp),
                                                        C.POINTER(RefLcontrol), _D, _D, _D, _I, _I, _I]
         L.c_hacapk_bicgstab_cax_lfmtx_gpu_.restype = None
@@ -1016,6 +1019,17 @@ def adot_body_lfmtx_batch(zau, leafmtxp, zu):
     hmv_lib().c_hacapk_adot_body_lfmtx_batch_(_dptr(zau), C.byref(leafmtxp), _dptr(zu), None,
                                               C.byref(tb), C.byref(ts), C.byref(tc))
     return tb.value
+
+
+def adot_body_lfmtx_host(zau, leafmtxp, zu, variant="lfmtx"):
+    """c_hacapk_adot_body_lfmtx_ (HACApK_MAGMA.c:5) or, with variant
+    "lfmtx_gpu", c_hacapk_adot_body_lfmtx_gpu_ (HACApK_MAGMA.c:79): zau += H zu
+    (in place) through the arena bound to leafmtxp (built on first use)."""
+    assert variant in ("lfmtx", "lfmtx_gpu")
+    assert zau.dtype == np.float64 and zau.flags.c_contiguous
+    zu = np.ascontiguousarray(zu, np.float64)
+    getattr(hmv_lib(), "c_hacapk_adot_body_%s_" % variant)(_dptr(zau), C.byref(leafmtxp),
+                                                           _dptr(zu), None)
 
 
 def adot_body_lfdel_batch(leafmtxp):

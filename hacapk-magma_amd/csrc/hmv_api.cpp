@@ -893,6 +893,26 @@ void report(const char *fn) {
   std::fprintf(stderr, "%s: %s\n", fn, hmv_last_error());
   std::fflush(stderr);
 }
+
+// zau += H*zu on host vectors through the arena bound to H; the first call
+// builds an FP64 arena and binds it.
+void apply_bound(const char *name, double *zau, hmv_ref_leafmtxp *H, double *zu) {
+  if (!zau || !H || !zu) {
+    set_error(std::string(name) + ": null argument");
+    report(name);
+    return;
+  }
+  hmv_arena a = lookup(H);
+  if (!a) {
+    if (hmv_arena_build(H->nd, H->nlf, H->st_lf, H->st_lf_stride, HMV_PREC_F64, &a) != HMV_OK) {
+      report(name);
+      return;
+    }
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    g_tab[H] = a;
+  }
+  if (hmv_apply_host(a, zu, zau) != HMV_OK) report(name);
+}
 }  // namespace
 
 extern "C" {
@@ -924,6 +944,16 @@ void c_hacapk_adot_body_lfmtx_batch_(double *zau, hmv_ref_leafmtxp *H, double *z
   if (hmv_apply_host(a, zu, zau) != HMV_OK) report("c_hacapk_adot_body_lfmtx_batch_");
   auto t1 = std::chrono::steady_clock::now();
   if (time_batch) *time_batch += std::chrono::duration<double>(t1 - t0).count();
+}
+
+void c_hacapk_adot_body_lfmtx_(double *zau, hmv_ref_leafmtxp *H, double *zu, double *zbu) {
+  (void)zbu;
+  apply_bound("c_hacapk_adot_body_lfmtx_", zau, H, zu);
+}
+
+void c_hacapk_adot_body_lfmtx_gpu_(double *zau, hmv_ref_leafmtxp *H, double *zu, double *zbu) {
+  (void)zbu;
+  apply_bound("c_hacapk_adot_body_lfmtx_gpu_", zau, H, zu);
 }
 
 void c_hacapk_adot_body_lfdel_batch_(hmv_ref_leafmtxp *H) {

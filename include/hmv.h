@@ -399,6 +399,18 @@ void c_hacapk_adot_body_lfmtx_batch_(double *zau, hmv_ref_leafmtxp *st_leafmtxp,
                                      double *time_copy);
 /* Replaces c_hacapk_adot_body_lfdel_batch_ (HACApK_BATCH.c:812). */
 void c_hacapk_adot_body_lfdel_batch_(hmv_ref_leafmtxp *st_leafmtxp);
+/* Replace the reference's host applies c_hacapk_adot_body_lfmtx_
+ * (HACApK_MAGMA.c:5-57, scalar loops) and c_hacapk_adot_body_lfmtx_gpu_
+ * (HACApK_MAGMA.c:79-227, per-leaf dgemv on the host despite its name),
+ * which the Fortran driver calls in its non-batch builds
+ * (m_HACApK_solve.f90:100,102) and in its timing loop (:664): zau += H*zu on
+ * host vectors, zbu unused.  The first call builds an FP64 arena and binds
+ * it to st_leafmtxp (reused by later calls and by the solvers; freed by
+ * lfdel_batch_, which a caller must call after changing the leaves). */
+void c_hacapk_adot_body_lfmtx_(double *zau, hmv_ref_leafmtxp *st_leafmtxp, double *zu,
+                               double *zbu);
+void c_hacapk_adot_body_lfmtx_gpu_(double *zau, hmv_ref_leafmtxp *st_leafmtxp, double *zu,
+                                   double *zbu);
 /* Replaces c_hacapk_bicgstab_cax_lfmtx_gpu_ (HACApK_SOLVER.c:421): host u, b;
  * param[82] = max steps, param[90] = tolerance (Fortran param(83), param(91)). */
 void c_hacapk_bicgstab_cax_lfmtx_gpu_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lcontrol *st_ctl,

@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -44,11 +45,20 @@ def test_no_cpu_fallback(H, sphere3):
         H.Arena(sphere3, "f64")
     with pytest.raises(H.HmvError):
         H.DeviceBuffer(64)
+    # the reference-named host applies compute nothing on the CPU: zau is
+    # left as it was and the error names the missing device
+    lp = H.make_leafmtxp(sphere3)
+    for variant in ("lfmtx", "lfmtx_gpu"):
+        zau = np.full(sphere3.nd, 7.0)
+        H.adot_body_lfmtx_host(zau, lp, np.ones(sphere3.nd), variant)
+        assert np.all(zau == 7.0)
+        assert b"no HIP device" in H.hmv_lib().hmv_last_error()
 
 
 def test_new_reference_named_symbols_present(H):
     L = H.hmv_lib()
-    for n in ("c_hacapk_bicgstab_cax_lfmtx_pipe_", "c_hacapk_bicgstab_cax_lfmtx_pipe2_"):
+    for n in ("c_hacapk_bicgstab_cax_lfmtx_pipe_", "c_hacapk_bicgstab_cax_lfmtx_pipe2_",
+              "c_hacapk_adot_body_lfmtx_", "c_hacapk_adot_body_lfmtx_gpu_"):
         assert hasattr(L, n)
 
 

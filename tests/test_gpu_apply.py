@@ -130,6 +130,28 @@ def test_reference_named_batch_apply(gpu, O, sphere3):
     assert _rel(zau, ref) <= TOL64
 
 
+@pytest.mark.parametrize("variant", ["lfmtx", "lfmtx_gpu"])
+def test_reference_named_host_apply(gpu, O, sphere3, variant):
+    """c_hacapk_adot_body_lfmtx_ / lfmtx_gpu_ without a prior lfcpy: the
+    first call binds an arena, a second call reuses it (the timing loop of
+    m_HACApK_solve.f90:655-664 calls it repeatedly with u(:)=1, b(:)=1), and
+    the reference-named solver then finds the bound arena."""
+    hm = sphere3
+    lp = gpu.make_leafmtxp(hm)
+    b = np.ones(hm.nd)
+    ref = O.adot_body_lfmtx(hm, b, np.ones(hm.nd))
+    for _ in range(2):
+        u = np.ones(hm.nd)
+        gpu.adot_body_lfmtx_host(u, lp, b, variant)
+        assert _rel(u, ref) <= TOL64
+    param = np.zeros(100)
+    param[82], param[90] = 200, 1e-8
+    x = np.zeros(hm.nd)
+    nstp, lrtrn = gpu.bicgstab_cax_lfmtx_gpu(lp, x, b, param)
+    assert lrtrn == 0 and nstp > 0
+    gpu.adot_body_lfdel_batch(lp)
+
+
 def test_edge_leaves(gpu, O, H):
     """Hand-made leaf lists: a rank-0 low-rank leaf, a 1x1 dense leaf, rows
     covered by no leaf (y = 0 there), a long low-rank block (> 2048 columns,
