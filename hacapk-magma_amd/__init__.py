@@ -262,6 +262,10 @@ def hmv_lib():
         for n in ("c_hacapk_adot_body_lfmtx_", "c_hacapk_adot_body_lfmtx_gpu_"):
             getattr(L, n).argtypes = [_D, C.POINTER(RefLeafmtxp), _D, _D]
             getattr(L, n).restype = None
+        L.c_hacapk_adot_body_lfcpy_gpu_.argtypes = [_I, C.POINTER(RefLeafmtxp)]
+        L.c_hacapk_adot_body_lfcpy_gpu_.restype = None
+        L.c_hacapk_adot_body_lfdel_gpu_.argtypes = [C.POINTER(RefLeafmtxp)]
+        L.c_hacapk_adot_body_lfdel_gpu_.restype = None
         L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes = [C.POINTER(RefLeafmtxp),
                                                        C.POINTER(RefLcontrol), _D, _D, _D, _I, _I, _I]
         L.c_hacapk_bicgstab_cax_lfmtx_gpu_.restype = None
@@ -269,6 +273,9 @@ def hmv_lib():
         L.c_hacapk_bicgstab_cax_lfmtx_pipe_.restype = None
         L.c_hacapk_bicgstab_cax_lfmtx_pipe2_.argtypes = L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes
         L.c_hacapk_bicgstab_cax_lfmtx_pipe2_.restype = None
+        for n in ("c_hacapk_bicgstab_cax_lfmtx_flat_", "c_hacapk_bicgstab_cax_lfmtx_mgpu1_"):
+            getattr(L, n).argtypes = L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes
+            getattr(L, n).restype = None
         L.hmv_bicgstab_pipe.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int, _D,
                                         C.POINTER(SolveInfo)]
         L.hmv_construct.argtypes = [C.c_int, _D, C.c_int, _I, _D, _D, _I, C.c_int, vp, C.c_int,
@@ -1037,14 +1044,22 @@ def adot_body_lfdel_batch(leafmtxp):
     hmv_lib().c_hacapk_adot_body_lfdel_batch_(C.byref(leafmtxp))
 
 
-def _ref_solver(fn, leafmtxp, u, b, param):
+def _ref_solver(fn, leafmtxp, u, b, param, nrank=1):
+    """Calls a reference-named solver with an lcontrol whose lpmd array (the
+    communicator, ranks, this rank, ...: m_HACApK_use.f90 reads lpmd(1..3))
+    follows the param array, lpmd_offset bytes past it."""
     assert u.dtype == np.float64 and u.flags.c_contiguous
     b = np.ascontiguousarray(b, np.float64)
-    param = np.ascontiguousarray(param, np.float64)
+    param = np.asarray(param, np.float64)
+    buf = np.zeros(len(param) + 8)
+    buf[:len(param)] = param
+    lpmd = buf[len(param):].view(np.int32)
+    lpmd[1] = nrank
     ctl = RefLcontrol()
-    ctl.param = _dptr(param)
+    ctl.param = _dptr(buf)
+    ctl.lpmd_offset = 8 * len(param)
     nd, nstp, lrtrn = C.c_int(len(u)), C.c_int(0), C.c_int(0)
-    fn(C.byref(leafmtxp), C.byref(ctl), _dptr(u), _dptr(b), _dptr(param), C.byref(nd), C.byref(nstp),
+    fn(C.byref(leafmtxp), C.byref(ctl), _dptr(u), _dptr(b), _dptr(buf), C.byref(nd), C.byref(nstp),
        C.byref(lrtrn))
     return nstp.value, lrtrn.value
 
@@ -1054,6 +1069,28 @@ def bicgstab_cax_lfmtx_gpu(leafmtxp, u, b, param):
     param is the 100-slot Fortran param array seen 0-based (param[82] max
     steps, param[90] tolerance).  Returns (nstp, lrtrn)."""
     return _ref_solver(hmv_lib().c_hacapk_bicgstab_cax_lfmtx_gpu_, leafmtxp, u, b, param)
+
+
+def bicgstab_cax_lfmtx_flat(leafmtxp, u, b, param):
+    """c_hacapk_bicgstab_cax_lfmtx_flat_ (HACApK_SOLVER.c:254): as
+    bicgstab_cax_lfmtx_gpu."""
+    return _ref_solver(hmv_lib().c_hacapk_bicgstab_cax_lfmtx_flat_, leafmtxp, u, b, param)
+
+
+def bicgstab_cax_lfmtx_mgpu1(leafmtxp, u, b, param, nrank=1):
+    """c_hacapk_bicgstab_cax_lfmtx_mgpu1_ (HACApK_SOLVER.c:815) with lpmd(2) =
+    nrank; more than one rank returns lrtrn = HMV_ERR_ARG."""
+    return _ref_solver(hmv_lib().c_hacapk_bicgstab_cax_lfmtx_mgpu1_, leafmtxp, u, b, param, nrank)
+
+
+def adot_body_lfcpy_gpu(nd, leafmtxp):
+    """c_hacapk_adot_body_lfcpy_gpu_ (HACApK_MAGMA.c:231): as lfcpy_batch_sorted."""
+    hmv_lib().c_hacapk_adot_body_lfcpy_gpu_(C.byref(C.c_int(nd)), C.byref(leafmtxp))
+
+
+def adot_body_lfdel_gpu(leafmtxp):
+    """c_hacapk_adot_body_lfdel_gpu_ (HACApK_MAGMA.c:369): as lfdel_batch."""
+    hmv_lib().c_hacapk_adot_body_lfdel_gpu_(C.byref(leafmtxp))
 
 
 def bicgstab_cax_lfmtx_pipe(leafmtxp, u, b, param, variant=1):

@@ -956,6 +956,12 @@ void c_hacapk_adot_body_lfmtx_gpu_(double *zau, hmv_ref_leafmtxp *H, double *zu,
   apply_bound("c_hacapk_adot_body_lfmtx_gpu_", zau, H, zu);
 }
 
+void c_hacapk_adot_body_lfcpy_gpu_(int *nd, hmv_ref_leafmtxp *H) {
+  c_hacapk_adot_body_lfcpy_batch_sorted_(nd, H);
+}
+
+void c_hacapk_adot_body_lfdel_gpu_(hmv_ref_leafmtxp *H) { c_hacapk_adot_body_lfdel_batch_(H); }
+
 void c_hacapk_adot_body_lfdel_batch_(hmv_ref_leafmtxp *H) {
   hmv_arena a = nullptr;
   {
@@ -1049,6 +1055,31 @@ void c_hacapk_bicgstab_cax_lfmtx_pipe2_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C
                                         int *lrtrn) {
   ref_solve("c_hacapk_bicgstab_cax_lfmtx_pipe2_", "pipelined BiCGSTAB", hmv_bicgstab_pipe, H, C,
             u_cpu, b_cpu, param, nd, nstp, lrtrn);
+}
+
+void c_hacapk_bicgstab_cax_lfmtx_flat_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C, double *u_cpu,
+                                       double *b_cpu, double *param, int *nd, int *nstp,
+                                       int *lrtrn) {
+  ref_solve("c_hacapk_bicgstab_cax_lfmtx_flat_", "BiCGSTAB", hmv_bicgstab, H, C, u_cpu, b_cpu, param,
+            nd, nstp, lrtrn);
+}
+
+void c_hacapk_bicgstab_cax_lfmtx_mgpu1_(hmv_ref_leafmtxp *H, hmv_ref_lcontrol *C, double *u_cpu,
+                                        double *b_cpu, double *param, int *nd, int *nstp,
+                                        int *lrtrn) {
+  // lpmd(2) = number of ranks (HACApK_SOLVER.c:837 reads lpmd[1])
+  if (C && C->param) {
+    const int *lpmd = (const int *)((const char *)C->param + C->lpmd_offset);
+    if (lpmd[1] > 1) {
+      set_error("c_hacapk_bicgstab_cax_lfmtx_mgpu1_: multi-rank solve not built (one GPU per "
+                "process, one process)");
+      report("c_hacapk_bicgstab_cax_lfmtx_mgpu1_");
+      if (lrtrn) *lrtrn = HMV_ERR_ARG;
+      return;
+    }
+  }
+  ref_solve("c_hacapk_bicgstab_cax_lfmtx_mgpu1_", "BiCGSTAB", hmv_bicgstab, H, C, u_cpu, b_cpu,
+            param, nd, nstp, lrtrn);
 }
 
 }  // extern "C"

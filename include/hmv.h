@@ -411,6 +411,12 @@ void c_hacapk_adot_body_lfmtx_(double *zau, hmv_ref_leafmtxp *st_leafmtxp, doubl
                                double *zbu);
 void c_hacapk_adot_body_lfmtx_gpu_(double *zau, hmv_ref_leafmtxp *st_leafmtxp, double *zu,
                                    double *zbu);
+/* Replace c_hacapk_adot_body_lfcpy_gpu_ / lfdel_gpu_ (HACApK_MAGMA.c:231,369),
+ * the per-leaf device copies of the non-batch MAGMA build
+ * (m_HACApK_solve.f90:379,434,681-683): the same as lfcpy_batch_sorted_ and
+ * lfdel_batch_. */
+void c_hacapk_adot_body_lfcpy_gpu_(int *nd, hmv_ref_leafmtxp *st_leafmtxp);
+void c_hacapk_adot_body_lfdel_gpu_(hmv_ref_leafmtxp *st_leafmtxp);
 /* Replaces c_hacapk_bicgstab_cax_lfmtx_gpu_ (HACApK_SOLVER.c:421): host u, b;
  * param[82] = max steps, param[90] = tolerance (Fortran param(83), param(91)). */
 void c_hacapk_bicgstab_cax_lfmtx_gpu_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lcontrol *st_ctl,
@@ -427,6 +433,18 @@ void c_hacapk_bicgstab_cax_lfmtx_pipe_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lc
  * recurrences with the MPI all-gather overlapped with the dot products,
  * which one GPU does not have; it runs hmv_bicgstab_pipe. */
 void c_hacapk_bicgstab_cax_lfmtx_pipe2_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lcontrol *st_ctl,
+                                        double *u_cpu, double *b_cpu, double *param, int *nd,
+                                        int *nstp, int *lrtrn);
+/* Replaces c_hacapk_bicgstab_cax_lfmtx_flat_ (HACApK_SOLVER.c:254), the
+ * BiCGSTAB on host vectors with one GPU matvec per product
+ * (m_HACApK_use.f90:400): the device BiCGSTAB, same arguments as _gpu_. */
+void c_hacapk_bicgstab_cax_lfmtx_flat_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lcontrol *st_ctl,
+                                       double *u_cpu, double *b_cpu, double *param, int *nd,
+                                       int *nstp, int *lrtrn);
+/* Replaces c_hacapk_bicgstab_cax_lfmtx_mgpu1_ (HACApK_SOLVER.c:815,
+ * m_HACApK_use.f90:349) for one rank (lpmd(2) = 1): the device BiCGSTAB.
+ * With more ranks it prints an error and sets *lrtrn = HMV_ERR_ARG. */
+void c_hacapk_bicgstab_cax_lfmtx_mgpu1_(hmv_ref_leafmtxp *st_leafmtxp, hmv_ref_lcontrol *st_ctl,
                                         double *u_cpu, double *b_cpu, double *param, int *nd,
                                         int *nstp, int *lrtrn);
 

@@ -58,7 +58,9 @@ def test_no_cpu_fallback(H, sphere3):
 def test_new_reference_named_symbols_present(H):
     L = H.hmv_lib()
     for n in ("c_hacapk_bicgstab_cax_lfmtx_pipe_", "c_hacapk_bicgstab_cax_lfmtx_pipe2_",
-              "c_hacapk_adot_body_lfmtx_", "c_hacapk_adot_body_lfmtx_gpu_"):
+              "c_hacapk_adot_body_lfmtx_", "c_hacapk_adot_body_lfmtx_gpu_",
+              "c_hacapk_adot_body_lfcpy_gpu_", "c_hacapk_adot_body_lfdel_gpu_",
+              "c_hacapk_bicgstab_cax_lfmtx_flat_", "c_hacapk_bicgstab_cax_lfmtx_mgpu1_"):
         assert hasattr(L, n)
 
 

@@ -72,6 +72,31 @@ def test_reference_named_bicgstab(gpu, O, sphere3):
     assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-7
 
 
+@pytest.mark.parametrize("name", ["flat", "mgpu1"])
+def test_reference_named_flat_and_single_rank_mgpu(gpu, O, sphere3, name):
+    """c_hacapk_bicgstab_cax_lfmtx_flat_ and _mgpu1_ with one rank solve as
+    the _gpu_ entry point does, on an arena bound by lfcpy_gpu_ (the
+    non-batch build's sequence, m_HACApK_solve.f90:379,434); _mgpu1_ with two
+    ranks refuses and leaves u untouched."""
+    hm = sphere3
+    lp = gpu.make_leafmtxp(hm)
+    gpu.adot_body_lfcpy_gpu(hm.nd, lp)
+    param = np.zeros(100)
+    param[82], param[90] = 200, 1e-8
+    b = np.ones(hm.nd)
+    u = np.zeros(hm.nd)
+    fn = gpu.bicgstab_cax_lfmtx_flat if name == "flat" else gpu.bicgstab_cax_lfmtx_mgpu1
+    nstp, lrtrn = fn(lp, u, b, param)
+    u_ref, nstp_ref, _ = O.bicgstab(hm, b, None, mstep=200, eps=1e-8)
+    assert lrtrn == 0 and abs(nstp - nstp_ref) <= 1
+    assert np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref) < 1e-7
+    if name == "mgpu1":
+        u2 = np.full(hm.nd, 3.0)
+        nstp, lrtrn = gpu.bicgstab_cax_lfmtx_mgpu1(lp, u2, b, param, nrank=2)
+        assert lrtrn == -1 and np.all(u2 == 3.0)
+    gpu.adot_body_lfdel_gpu(lp)
+
+
 def test_maxiter_and_zero_rhs(gpu, sphere3):
     a = gpu.Arena(sphere3, "f64")
     b = gpu.uniform(1, sphere3.nd)
