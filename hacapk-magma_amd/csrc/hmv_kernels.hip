@@ -169,8 +169,13 @@ struct Part {
   }
 };
 
-// Loads per batch (32 bytes in flight per lane for 8-byte lane elements).
-constexpr int kBatch = 4;
+// Loads per batch: 32 bytes in flight per lane (4 loads of 8-byte lane
+// elements, 8 of 4-byte ones), so that narrow lanes keep as many bytes in
+// flight as FP64 ones.
+template <int DT, int VR>
+constexpr int batch_loads() {
+  return sizeof(typename Lane<DT, VR>::raw) >= 8 ? 4 : 8;
+}
 
 // Lane's columns of local range [lo, hi) of a part: local column cg + j*m,
 // j in [ceil((lo-cg)/m), ceil((hi-cg)/m)); zshift maps a local column to its
@@ -181,8 +186,8 @@ __device__ inline void run_cols(const Part<DT, VR> &P, int lo, int hi, int cg, i
   const int j0 = lo > cg ? (lo - cg + m - 1) / m : 0;
   const int j1 = hi > cg ? (hi - cg + m - 1) / m : 0;
   if (j1 > j0)
-    accumulate<Acc, DT, VR, kBatch>(P.rs, P.voff + (uint32_t)(j0 * P.step), P.step, j1 - j0, z,
-                                    cg + j0 * m + zshift, m, zmax, a);
+    accumulate<Acc, DT, VR, batch_loads<DT, VR>()>(P.rs, P.voff + (uint32_t)(j0 * P.step), P.step,
+                                                   j1 - j0, z, cg + j0 * m + zshift, m, zmax, a);
 }
 
 // PHASE 1: t[outidx[out_base + r]] = row sums.
@@ -237,10 +242,13 @@ __device__ inline void stream_item(
       }
       const int hi0 = min(c0 + cw, w.ncol0);
       if (lo0 < hi0) run_cols<Acc, DT0, vr>(P0, lo0, hi0, cg, m, -c0, cw, z, a);
-      if (w.ncol1 > 0 && c0 + cw > w.ncol0) {
-        const Part<DT1, vr> P1(arena + w.off1, w.rows_p, w.ncol1, rq, cg, m);
-        run_cols<Acc, DT1, vr>(P1, max(c0, w.ncol0) - w.ncol0, c0 + cw - w.ncol0, cg, m,
-                               w.ncol0 - c0, cw, z, a);
+      // an item stored with one type has one part (ncol1 == 0)
+      if constexpr (DT0 != DT1) {
+        if (w.ncol1 > 0 && c0 + cw > w.ncol0) {
+          const Part<DT1, vr> P1(arena + w.off1, w.rows_p, w.ncol1, rq, cg, m);
+          run_cols<Acc, DT1, vr>(P1, max(c0, w.ncol0) - w.ncol0, c0 + cw - w.ncol0, cg, m,
+                                 w.ncol0 - c0, cw, z, a);
+        }
       }
     }
   }
