@@ -196,13 +196,22 @@ __device__ inline void run_cols(const Part<DT, VR> &P, int lo, int hi, int cg, i
 // NT: threads per workgroup; inputs are staged NT*8 columns at a time.
 // One work item w; z: LDS of ZCH + 4 accumulation-type slots (inputs of one
 // chunk plus the zero slot; reused for the row reduction).
+// Inputs staged per pass: HMV_ZCHUNK per 256 threads in FP64, twice that
+// (up to HMV_ZCHUNK) for 4-byte accumulation types, so that a workgroup
+// holds the same LDS bytes (at 32 waves per CU the z arrays of one CU take
+// 128 KB either way).
+template <typename Acc, int NT>
+constexpr int zchunk() {
+  return HMV_ZCHUNK * NT / HMV_WG_THREADS * (sizeof(Acc) == 4 && NT < HMV_WG_THREADS ? 2 : 1);
+}
+
 template <typename Acc, int PHASE, int DT0, int DT1, int NT>
 __device__ inline void stream_item(
     const HmvWork &w, Acc *__restrict__ z, int *__restrict__ runs, const char *__restrict__ arena,
     const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ codes,
     const double *__restrict__ t_in, const int *__restrict__ outidx, double *__restrict__ t_out,
     double *__restrict__ y, double beta) {
-  constexpr int ZCH = HMV_ZCHUNK * NT / HMV_WG_THREADS;
+  constexpr int ZCH = zchunk<Acc, NT>();
   const int tid = threadIdx.x;
   constexpr int vr = rows_per_lane<PHASE, DT0>();
   static_assert(vr * NT <= ZCH, "reduction buffer fits in z");
@@ -283,7 +292,7 @@ hmv_stream_kernel(
   if (skip && *skip) return;
   // a finished instance of a multi-instance solve needs no more products
   if (inst_done && inst_done[winst[blockIdx.x]]) return;
-  __shared__ __attribute__((aligned(16))) Acc z[HMV_ZCHUNK * NT / HMV_WG_THREADS + 4];
+  __shared__ __attribute__((aligned(16))) Acc z[zchunk<Acc, NT>() + 4];
   __shared__ int runs[2 * HMV_RUN_MAX];
   stream_item<Acc, PHASE, DT0, DT1, NT>(works[blockIdx.x], z, runs, arena, zsegs, x, codes, t_in,
                                         outidx, t_out, y, beta);
