@@ -29,7 +29,6 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr uint32_t kOff = 0x80000000u;  // out-of-range offset: the load returns 0
 constexpr int kNT = 2;                  // nontemporal cache policy
 
 __device__ inline __amdgpu_buffer_rsrc_t rsrc(const char *base, uint32_t nbytes) {
