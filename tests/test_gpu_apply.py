@@ -231,11 +231,13 @@ def test_invalid_leaves_rejected(gpu, H):
 @pytest.mark.parametrize("prec", ["f64", "bf16"])
 def test_arena_file_round_trip(gpu, O, sphere4, tmp_path, prec):
     """An arena written by hmv_arena_save and one written on the host from
-    the plan both load (hmv_arena_load) into arenas whose apply is bit-equal
-    to the original's."""
+    the plan both load (hmv_arena_load) into arenas whose apply and H-matmat
+    are bit-equal to the original's."""
     a = gpu.Arena(sphere4, prec)
     x = gpu.uniform(12, sphere4.nd)
     y = a.matvec(x)
+    X = np.stack([gpu.uniform(20 + j, sphere4.nd) for j in range(8)], axis=1)
+    Y = a.matmat(X)
     a.save(tmp_path / "dev.hmv")
     gpu.save_plan(sphere4, tmp_path / "host.hmv", prec)
     for name in ("dev.hmv", "host.hmv"):
@@ -243,6 +245,7 @@ def test_arena_file_round_trip(gpu, O, sphere4, tmp_path, prec):
         assert b.nd == sphere4.nd and b.precision == prec
         assert b.stats["arena_bytes"] == a.stats["arena_bytes"]
         assert np.array_equal(b.matvec(x), y)
+        assert np.array_equal(b.matmat(X), Y)
         b.free()
     a.free()
     assert (tmp_path / "dev.hmv").read_bytes() == (tmp_path / "host.hmv").read_bytes()

@@ -1,9 +1,9 @@
 """Many independent instances in one arena (one launch per phase) and their
 per-instance BiCGSTAB solves, through the C ABI.
 
-Bar: the concatenated apply equals each instance's own apply (FP64 <= 1e-13
-relative per instance) and the oracle; each instance's solve reaches the
-true-residual bar and agrees with its single-instance solve.
+Bar: the concatenated apply and H-matmat equal each instance's own apply
+(FP64 <= 1e-13 relative per instance) and the oracle; each instance's solve
+reaches the true-residual bar and agrees with its single-instance solve.
 """
 import numpy as np
 import pytest
@@ -35,6 +35,23 @@ def test_multi_apply_matches_instances(gpu, O, instances):
         a = gpu.Arena(hm, "f64")
         assert _rel(y[lo:hi], a.matvec(x[lo:hi])) <= 1e-13
         assert _rel(y[lo:hi], O.adot_body_lfmtx(hm, x[lo:hi])) <= 1e-12
+        a.free()
+    m.free()
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_multi_matmat_matches_instances(gpu, O, instances, k):
+    """The H-matmat over a multi-instance arena is block diagonal: each
+    instance's rows of every column equal that instance's own matvec."""
+    m = gpu.MultiArena(instances, "f64")
+    X = np.stack([gpu.uniform(50 + j, m.nd) for j in range(k)], axis=1)
+    Y = m.matmat(X)
+    for i, hm in enumerate(instances):
+        lo, hi = m.offsets[i], m.offsets[i + 1]
+        a = gpu.Arena(hm, "f64")
+        for j in (0, k - 1):
+            assert _rel(Y[lo:hi, j], a.matvec(X[lo:hi, j])) <= 1e-13
+        assert _rel(Y[lo:hi, k // 2], O.adot_body_lfmtx(hm, X[lo:hi, k // 2])) <= 1e-12
         a.free()
     m.free()
 
