@@ -232,6 +232,22 @@ def main():
                     "err vs FP64 sol %.2e" % (inf["outer"], inf["inner_total"], inf["seconds"],
                                               inf["seconds"] / i64["seconds"], inf["relres"], err))
             alow.free()
+        # refinement with GCR(32) inner solves on an all-BF16 arena
+        alow = H.Arena(hm, "bf16_all")
+        H.gcr_ir(a64, alow, b, eps=1e-8, inner_eps=5e-3, m=32)
+        u, inf, hist = H.gcr_ir(a64, alow, b, eps=1e-8, inner_eps=5e-3, m=32)
+        err = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+        solves["gcr32_ir_bf16_all"] = {
+            "seconds": round(inf["seconds"], 4), "inner_eps": 5e-3, "outer": inf["outer"],
+            "inner_products": inf["inner_total"], "true_relres": inf["relres"],
+            "status": inf["status_name"], "rel_err_vs_fp64_solution": err,
+            "outer_residuals": [float(v) for v in hist],
+            "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3),
+            "arena_bytes": int(alow.stats["arena_bytes"])}
+        log("GCR(32)-IR bf16_all: outer %d, %d inner products, %.3f s (%.2fx FP64), relres %.2e, "
+            "err vs FP64 sol %.2e" % (inf["outer"], inf["inner_total"], inf["seconds"],
+                                      inf["seconds"] / i64["seconds"], inf["relres"], err))
+        alow.free()
         result["bicgstab"] = solves
 
         # ---- CPU baseline: bounded sample on this host ------------------------
