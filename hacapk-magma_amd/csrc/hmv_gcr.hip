@@ -30,6 +30,7 @@ namespace {
 constexpr int kT = 256;
 constexpr int kMaxB = 512;
 constexpr int kMaxM = HMV_GCR_MAX_M;
+static_assert(kMaxM >= 3, "the partial buffer holds the alpha pairs and the (r, r) partials");
 
 struct GcrState {
   double alpha, bnorm, rnorm, eps;
@@ -89,24 +90,47 @@ __global__ __launch_bounds__(kT) void g_dots_alpha(int n, const double *__restri
   }
 }
 
-// u += alpha p_ik;  r -= alpha ap_ik;  (r, r) partials
-__global__ __launch_bounds__(kT) void g_update_ur(int n, const double *__restrict__ p,
+// alpha = (ap_ik, r) / (ap_ik, ap_ik) from the nb partial pairs of
+// g_dots_alpha (every block forms it alike; block 0 records it, the slot
+// and capap_ik, and flags a breakdown), then u += alpha p_ik;
+// r -= alpha ap_ik;  (r, r) partials into rpart (apart is still being read
+// by other blocks).
+__global__ __launch_bounds__(kT) void g_update_ur(int n, int nb, const double *__restrict__ p,
                                                   const double *__restrict__ ap,
                                                   double *__restrict__ u, double *__restrict__ r,
-                                                  double *__restrict__ part, const GcrState *st) {
+                                                  const double *__restrict__ apart,
+                                                  double *__restrict__ rpart, GcrState *st) {
   if (st->done) return;
   __shared__ double sh[4];
-  const double alpha = st->alpha;
-  const double *pk = p + (size_t)st->ik * n, *q = ap + (size_t)st->ik * n;
-  double s0 = 0.0;
+  double s0 = 0.0, s1 = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kT) {
+    s0 += apart[2 * b];
+    s1 += apart[2 * b + 1];
+  }
+  const double d0 = blk_sum(s0, sh), d1 = blk_sum(s1, sh);
+  const int ik = st->ikn;  // the slot of this step
+  const double alpha = d0 / d1;
+  const bool bad = d1 == 0.0 || !isfinite(alpha);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->ik = ik;
+    st->capap[ik] = d1;
+    st->alpha = alpha;
+    if (bad) {
+      st->status = isfinite(d1) ? HMV_SOLVE_BREAKDOWN : HMV_SOLVE_NONFINITE;
+      st->done = 1;
+    }
+  }
+  if (bad) return;
+  const double *pk = p + (size_t)ik * n, *q = ap + (size_t)ik * n;
+  double s2 = 0.0;
   for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
     u[i] = u[i] + alpha * pk[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
-    s0 = __builtin_fma(ri, ri, s0);
+    s2 = __builtin_fma(ri, ri, s2);
   }
-  const double r0 = blk_sum(s0, sh);
-  if (threadIdx.x == 0) part[2 * blockIdx.x] = r0;
+  const double r0 = blk_sum(s2, sh);
+  if (threadIdx.x == 0) rpart[blockIdx.x] = r0;
 }
 
 // partials of (v_il, ar) for il = 0..ik, v_il = ar for the slot just
@@ -140,25 +164,43 @@ __global__ __launch_bounds__(kT) void g_dots_beta(int n, const double *__restric
   }
 }
 
-// stage 0: setup (d0 = (r,r), d1 = (b,b)); 1: alpha; 2: ||r||, step, stop;
-// 3: betas from the per-slot partials
+// stage 0: setup (d0 = (r,r), d1 = (b,b) from nb partial pairs); 2: ||r||
+// (nb partials), step, stop; 3: betas from the per-slot partials, one wave
+// per slot (lanes sum the blocks in a fixed order, then a fixed shuffle
+// tree: deterministic)
 __global__ __launch_bounds__(kT) void g_final(int nb, const double *__restrict__ part,
                                               GcrState *st, int stage, double *__restrict__ hist) {
   if (st->done && stage != 0) return;
   __shared__ double sh[4];
   if (stage == 3) {
-    for (int l = 0; l <= st->ik; ++l) {
-      double s = 0.0;
-      for (int b = threadIdx.x; b < nb; b += kT) s += part[(size_t)b * kMaxM + l];
-      const double d = blk_sum(s, sh);
-      if (threadIdx.x == 0) st->beta[l] = -d / st->capap[l];
+    // wave w takes slots w, w + 4, ...; all their loads go out together
+    constexpr int kW = kT / 64, kL = (kMaxM + kW - 1) / kW;
+    const int ik = st->ik, lane = threadIdx.x & 63, w0 = threadIdx.x >> 6;
+    double s[kL];
+#pragma unroll
+    for (int j = 0; j < kL; ++j) s[j] = 0.0;
+    for (int b = lane; b < nb; b += 64)
+#pragma unroll
+      for (int j = 0; j < kL; ++j)
+        if (w0 + j * kW <= ik) s[j] += part[(size_t)b * kMaxM + w0 + j * kW];
+#pragma unroll
+    for (int j = 0; j < kL; ++j) {
+      const int l = w0 + j * kW;
+      if (l > ik) break;
+      double v = s[j];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) st->beta[l] = -v / st->capap[l];
     }
     return;
   }
   double s0 = 0.0, s1 = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kT) {
-    s0 += part[2 * b];
-    s1 += part[2 * b + 1];
+  if (stage == 0) {
+    for (int b = threadIdx.x; b < nb; b += kT) {
+      s0 += part[2 * b];
+      s1 += part[2 * b + 1];
+    }
+  } else {
+    for (int b = threadIdx.x; b < nb; b += kT) s0 += part[b];
   }
   const double d0 = blk_sum(s0, sh), d1 = blk_sum(s1, sh);
   if (threadIdx.x != 0) return;
@@ -178,14 +220,6 @@ __global__ __launch_bounds__(kT) void g_final(int nb, const double *__restrict__
       st->status = HMV_SOLVE_CONVERGED;
       st->done = 1;
     } else if (st->maxit <= 0) {
-      st->done = 1;
-    }
-  } else if (stage == 1) {
-    st->ik = st->ikn;
-    st->capap[st->ik] = d1;
-    st->alpha = d0 / d1;
-    if (d1 == 0.0 || !isfinite(st->alpha)) {
-      st->status = isfinite(d1) ? HMV_SOLVE_BREAKDOWN : HMV_SOLVE_NONFINITE;
       st->done = 1;
     }
   } else {
@@ -208,16 +242,22 @@ __global__ __launch_bounds__(kT) void g_final(int nb, const double *__restrict__
 
 // p_ikn = c0 r + sum_{il <= ik, il != ikn} beta_il p_il;  ap_ikn likewise
 // with ar;  c0 = 1 + beta_ikn when ikn <= ik (slot 1 at a restart), else 1
+// ... and the next step's alpha partials (ap_ikn, r), (ap_ikn, ap_ikn) into
+// part[2*blk + {0,1}], in g_dots_alpha's order (same grid, same per-thread
+// sequence), so that the step needs no separate dot pass.
 __global__ __launch_bounds__(kT) void g_update_dir(int n, const double *__restrict__ r,
                                                    const double *__restrict__ ar,
                                                    double *__restrict__ p, double *__restrict__ ap,
-                                                   const GcrState *st) {
+                                                   double *__restrict__ part, const GcrState *st) {
   if (st->done) return;
+  __shared__ double sh[4];
   const int ik = st->ik, ikn = st->ikn;
   const double c0 = ikn <= ik ? 1.0 + st->beta[ikn] : 1.0;
   double *pn = p + (size_t)ikn * n, *apn = ap + (size_t)ikn * n;
+  double s0 = 0.0, s1 = 0.0;
   for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
-    double pv = c0 * r[i], av = c0 * ar[i];
+    const double ri = r[i];
+    double pv = c0 * ri, av = c0 * ar[i];
     for (int l = 0; l <= ik; ++l) {
       if (l == ikn) continue;
       const double bl = st->beta[l];
@@ -226,6 +266,13 @@ __global__ __launch_bounds__(kT) void g_update_dir(int n, const double *__restri
     }
     pn[i] = pv;
     apn[i] = av;
+    s0 = __builtin_fma(av, ri, s0);
+    s1 = __builtin_fma(av, av, s1);
+  }
+  const double r0 = blk_sum(s0, sh), r1 = blk_sum(s1, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r0;
+    part[2 * blockIdx.x + 1] = r1;
   }
 }
 
@@ -315,19 +362,21 @@ int gcr_work(hmv_arena_s *a, int m, int maxit, GcrWork *&out) {
 hipError_t gcr_iteration(hmv_arena_s *a, GcrWork &w, int n, int m, double *u) {
   const int g = grid_of(n);
   const int *skip = &w.st->done;
-  hipLaunchKernelGGL(g_dots_alpha, dim3(g), dim3(kT), 0, w.s, n, w.ap, w.r, w.part,
-                     (const GcrState *)w.st);
-  hipLaunchKernelGGL(g_final, dim3(1), dim3(kT), 0, w.s, g, w.part, w.st, 1, w.hist);
-  hipLaunchKernelGGL(g_update_ur, dim3(g), dim3(kT), 0, w.s, n, w.p, w.ap, u, w.r, w.part,
-                     (const GcrState *)w.st);
-  hipLaunchKernelGGL(g_final, dim3(1), dim3(kT), 0, w.s, g, w.part, w.st, 2, w.hist);
+  // (r, r) partials at [2g, 3g) of the g * kMaxM partials, apart from the
+  // alpha pairs at [0, 2g) that g_update_ur reads while writing them
+  double *rpart = w.part + 2 * (size_t)g;
+  // the alpha partials of this step come from the previous step's
+  // g_update_dir (or, for the first step, from the setup's g_dots_alpha)
+  hipLaunchKernelGGL(g_update_ur, dim3(g), dim3(kT), 0, w.s, n, g, w.p, w.ap, u, w.r,
+                     (const double *)w.part, rpart, w.st);
+  hipLaunchKernelGGL(g_final, dim3(1), dim3(kT), 0, w.s, g, (const double *)rpart, w.st, 2, w.hist);
   hipError_t e = hmv::enqueue_apply(a, w.r, w.ar, 0.0, skip, w.s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(g_dots_beta, dim3(g, (m + kGroup - 1) / kGroup), dim3(kT), 0, w.s, n, w.ap,
                      w.ar, w.part,
                      (const GcrState *)w.st);
   hipLaunchKernelGGL(g_final, dim3(1), dim3(kT), 0, w.s, g, w.part, w.st, 3, w.hist);
-  hipLaunchKernelGGL(g_update_dir, dim3(g), dim3(kT), 0, w.s, n, w.r, w.ar, w.p, w.ap,
+  hipLaunchKernelGGL(g_update_dir, dim3(g), dim3(kT), 0, w.s, n, w.r, w.ar, w.p, w.ap, w.part,
                      (const GcrState *)w.st);
   return hipGetLastError();
 }
@@ -370,9 +419,13 @@ int gcrm_run(hmv_arena a, const double *b, double *u, double eps, int maxit, int
   hipLaunchKernelGGL(g_init, dim3(g), dim3(kT), 0, w.s, n, b, w.ar, w.r, w.p);
   hipLaunchKernelGGL(g_dots2, dim3(g), dim3(kT), 0, w.s, n, w.r, w.r, b, b, w.part);
   hipLaunchKernelGGL(g_final, dim3(1), dim3(kT), 0, w.s, g, w.part, w.st, 0, w.hist);
-  // ap_1 = A p_1 (skipped when already converged)
-  if ((e = hmv::enqueue_apply(a, w.p, w.ap, 0.0, &w.st->done, w.s)) != hipSuccess ||
-      (e = hipStreamSynchronize(w.s)) != hipSuccess)
+  // ap_1 = A p_1 (skipped when already converged) and the first step's
+  // alpha partials
+  if ((e = hmv::enqueue_apply(a, w.p, w.ap, 0.0, &w.st->done, w.s)) != hipSuccess)
+    return hmv::hip_fail(e, "hmv_gcrm: setup");
+  hipLaunchKernelGGL(g_dots_alpha, dim3(g), dim3(kT), 0, w.s, n, w.ap, w.r, w.part,
+                     (const GcrState *)w.st);
+  if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(w.s)) != hipSuccess)
     return hmv::hip_fail(e, "hmv_gcrm: setup");
   if (!w.exec || w.g_u != u) {
     w.drop_graph();
