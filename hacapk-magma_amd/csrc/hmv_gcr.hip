@@ -276,12 +276,12 @@ __global__ __launch_bounds__(kT) void g_update_dir(int n, const double *__restri
   }
 }
 
-// r = b - ar;  p_1 = r
+// r = b - ar (r = b when ar is null: a zero initial guess);  p_1 = r
 __global__ __launch_bounds__(kT) void g_init(int n, const double *__restrict__ b,
                                              const double *__restrict__ ar,
                                              double *__restrict__ r, double *__restrict__ p) {
   for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
-    const double ri = b[i] - ar[i];
+    const double ri = ar ? b[i] - ar[i] : b[i];
     r[i] = ri;
     p[i] = ri;
   }
@@ -385,7 +385,8 @@ hipError_t gcr_iteration(hmv_arena_s *a, GcrWork &w, int n, int m, double *u) {
 
 namespace {
 int gcrm_run(hmv_arena a, const double *b, double *u, double eps, int maxit, int m,
-             int check_every, double *hist, hmv_solve_info *info, bool true_res);
+             int check_every, double *hist, hmv_solve_info *info, bool true_res,
+             bool zero_guess = false);
 }
 
 extern "C" int hmv_gcrm(hmv_arena a, const double *b, double *u, double eps, int maxit, int m,
@@ -395,8 +396,10 @@ extern "C" int hmv_gcrm(hmv_arena a, const double *b, double *u, double eps, int
 
 namespace {
 // true_res: recompute ||b - A u|| / ||b|| at the end (one more product).
+// zero_guess: u is known to be zero, so r = b without the product A u.
 int gcrm_run(hmv_arena a, const double *b, double *u, double eps, int maxit, int m,
-             int check_every, double *hist, hmv_solve_info *info, bool true_res) {
+             int check_every, double *hist, hmv_solve_info *info, bool true_res,
+             bool zero_guess) {
   if (!a || !b || !u || maxit < 0 || m < 1 || m > kMaxM) {
     hmv::set_error("hmv_gcrm: invalid argument (1 <= m <= 32)");
     return HMV_ERR_ARG;
@@ -414,9 +417,10 @@ int gcrm_run(hmv_arena a, const double *b, double *u, double eps, int maxit, int
   init.m = m;
   init.maxit = maxit;
   if ((e = hipMemcpyAsync(w.st, &init, sizeof(init), hipMemcpyHostToDevice, w.s)) != hipSuccess ||
-      (e = hmv::enqueue_apply(a, u, w.ar, 0.0, nullptr, w.s)) != hipSuccess)
+      (!zero_guess && (e = hmv::enqueue_apply(a, u, w.ar, 0.0, nullptr, w.s)) != hipSuccess))
     return hmv::hip_fail(e, "hmv_gcrm: setup");
-  hipLaunchKernelGGL(g_init, dim3(g), dim3(kT), 0, w.s, n, b, w.ar, w.r, w.p);
+  hipLaunchKernelGGL(g_init, dim3(g), dim3(kT), 0, w.s, n, b, zero_guess ? nullptr : w.ar, w.r,
+                     w.p);
   hipLaunchKernelGGL(g_dots2, dim3(g), dim3(kT), 0, w.s, n, w.r, w.r, b, b, w.part);
   hipLaunchKernelGGL(g_final, dim3(1), dim3(kT), 0, w.s, g, w.part, w.st, 0, w.hist);
   // ap_1 = A p_1 (skipped when already converged) and the first step's
@@ -570,7 +574,7 @@ extern "C" int hmv_gcr_ir(hmv_arena a64, hmv_arena alow, const double *b, double
   while (status == HMV_SOLVE_MAXITER && rel >= eps && outer < max_outer) {
     // solve Alow d = r to inner_eps with GCR(m), d starting at zero
     hmv_solve_info in{};
-    rc = gcrm_run(alow, w.r, w.d, inner_eps, max_inner, m, check_every, nullptr, &in, false);
+    rc = gcrm_run(alow, w.r, w.d, inner_eps, max_inner, m, check_every, nullptr, &in, false, true);
     if (rc) return rc;
     inner_total += in.nstp;
     if (in.status == HMV_SOLVE_NONFINITE) status = HMV_SOLVE_NONFINITE;
