@@ -228,9 +228,9 @@ __global__ __launch_bounds__(kThreads) void k_init(int n, const double *__restri
                                                     double *__restrict__ r,
                                                     double *__restrict__ rhat,
                                                     double *__restrict__ p,
-                                                    double *__restrict__ v) {
+                                                    double *__restrict__ v, int zero_guess) {
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-    const double ri = b[i] - r[i];
+    const double ri = zero_guess ? b[i] : b[i] - r[i];  // r holds A u unless u = 0
     r[i] = ri;
     rhat[i] = ri;
     p[i] = 0.0;
@@ -369,8 +369,10 @@ hipError_t enqueue_iteration(hmv_arena_s *a, HmvSolveWork *w, double *u) {
 }
 
 // Core BiCGSTAB; b, u device vectors.  hist_host may be null.
+// zero_guess: u is known to be zero, so r = b without the product A u.
 int run_bicgstab(hmv_arena_s *a, const double *b, double *u, double eps, int maxit,
-                 int check_every, double *hist_host, hmv_solve_info *info, bool use_graph) {
+                 int check_every, double *hist_host, hmv_solve_info *info, bool use_graph,
+                 bool zero_guess) {
   int rc = ensure_work(a, maxit + 1);
   if (rc) return rc;
   HmvSolveWork *w = a->solve;
@@ -387,9 +389,10 @@ int run_bicgstab(hmv_arena_s *a, const double *b, double *u, double eps, int max
   SolveState host{};
   if ((e = hipMemcpyAsync(w->st, &init, sizeof(init), hipMemcpyHostToDevice, s)) != hipSuccess)
     return hmv::hip_fail(e, "bicgstab: state upload");
-  if ((e = hmv::enqueue_apply(a, u, w->r, 0.0, nullptr, s)) != hipSuccess)
+  if (!zero_guess && (e = hmv::enqueue_apply(a, u, w->r, 0.0, nullptr, s)) != hipSuccess)
     return hmv::hip_fail(e, "bicgstab: initial apply");
-  hipLaunchKernelGGL(k_init, dim3(g), dim3(kThreads), 0, s, n, b, w->r, w->rhat, w->p, w->v);
+  hipLaunchKernelGGL(k_init, dim3(g), dim3(kThreads), 0, s, n, b, w->r, w->rhat, w->p, w->v,
+                     zero_guess ? 1 : 0);
   hipLaunchKernelGGL(k_dots, dim3(g), dim3(kThreads), 0, s, n, w->r, w->r, b, b, w->part,
                      (const SolveState *)nullptr);
   hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, s, g, w->part, w->st, 0, w->hist);
@@ -497,7 +500,7 @@ int hmv_bicgstab(hmv_arena a, const double *b, double *u, double eps, int maxit,
     return HMV_ERR_ARG;
   }
   hmv_solve_info loc{};
-  int rc = run_bicgstab(a, b, u, eps, maxit, check_every, hist, &loc, true);
+  int rc = run_bicgstab(a, b, u, eps, maxit, check_every, hist, &loc, true, false);
   if (rc) return rc;
   rc = true_relres(a, b, u, &loc.true_relres);
   if (rc) return rc;
@@ -534,7 +537,8 @@ int hmv_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, d
       return hmv::hip_fail(e, "ir: sync");
     hipLaunchKernelGGL(k_fill, dim3(g), dim3(kThreads), 0, wl->stream, n, wl->x2, 0.0);
     hmv_solve_info in{};
-    rc = run_bicgstab(alow, w64->r2, wl->x2, inner_eps, max_inner, check_every, nullptr, &in, true);
+    rc = run_bicgstab(alow, w64->r2, wl->x2, inner_eps, max_inner, check_every, nullptr, &in, true,
+                      true);
     if (rc) return rc;
     inner_total += in.nstp;
     if ((e = hipStreamSynchronize(wl->stream)) != hipSuccess) return hmv::hip_fail(e, "ir: sync");
@@ -592,7 +596,7 @@ int hmv_bicgstab_ru(hmv_arena a64, hmv_arena alow, const double *b, double *u, d
   if ((e = hipMemcpyAsync(w->st, &init, sizeof(init), hipMemcpyHostToDevice, s)) != hipSuccess ||
       (e = hmv::enqueue_apply(a64, u, w->r, 0.0, nullptr, s)) != hipSuccess)
     return hmv::hip_fail(e, "bicgstab_ru: setup");
-  hipLaunchKernelGGL(k_init, dim3(g), dim3(kThreads), 0, s, n, b, w->r, w->rhat, w->p, w->v);
+  hipLaunchKernelGGL(k_init, dim3(g), dim3(kThreads), 0, s, n, b, w->r, w->rhat, w->p, w->v, 0);
   hipLaunchKernelGGL(k_dots, dim3(g), dim3(kThreads), 0, s, n, w->r, w->r, b, b, w->part,
                      (const SolveState *)nullptr);
   hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, s, g, w->part, w->st, 0, w->hist);
