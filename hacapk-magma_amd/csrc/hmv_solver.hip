@@ -52,6 +52,17 @@ __device__ inline double block_sum(double v, double *sh) {
   return r;
 }
 
+// block_sum's total (same order) on every thread of the block.
+__device__ inline double block_sum_all(double v, double *sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return r;
+}
+
 // Partial sums of (a,b) and, when c != null, (c,d); part[2*block + {0,1}].
 __global__ __launch_bounds__(kThreads) void k_dots(int n, const double *__restrict__ a,
                                                     const double *__restrict__ b,
@@ -188,25 +199,65 @@ __global__ __launch_bounds__(kThreads) void k_update_p(int n, const double *__re
     p[i] = r[i] + beta * (p[i] - zeta * v[i]);
 }
 
-// t = r - alpha v
-__global__ __launch_bounds__(kThreads) void k_update_t(int n, const double *__restrict__ r,
+// alpha = rr / (r^, v) from the nb partial pairs of k_dots, summed in
+// k_final's order (every block forms it alike; block 0 records it and flags
+// a breakdown), then t = r - alpha v.
+__global__ __launch_bounds__(kThreads) void k_update_t(int n, int nb, const double *__restrict__ r,
                                                         const double *__restrict__ v,
                                                         double *__restrict__ t,
-                                                        const SolveState *st) {
+                                                        const double *__restrict__ part,
+                                                        SolveState *st) {
   if (st->done) return;
-  const double alpha = st->alpha;
+  __shared__ double sh[4];
+  double s0 = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) s0 += part[2 * i];
+  const double zden = block_sum_all(s0, sh);
+  const double alpha = st->rr / zden;
+  const bool bad = zden == 0.0 || !finite(alpha);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->d0 = zden;
+    st->alpha = alpha;
+    if (bad) {
+      st->status = finite(zden) ? HMV_SOLVE_BREAKDOWN : HMV_SOLVE_NONFINITE;
+      st->done = 1;
+    }
+  }
+  if (bad) return;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads)
     t[i] = r[i] - alpha * v[i];
 }
 
-// u += alpha p + zeta t;  r = t - zeta s;  partials of (r^, r) and (r, r)
+// zeta = (s, t) / (s, s) from the nb partial pairs zpart of k_dots, summed
+// in k_final's order (every block alike; block 0 records it and flags a
+// breakdown), then u += alpha p + zeta t;  r = t - zeta s;  partials of
+// (r^, r) and (r, r) into part (apart from zpart, which other blocks are
+// still reading).
 __global__ __launch_bounds__(kThreads) void k_update_ur(
-    int n, const double *__restrict__ p, const double *__restrict__ t,
+    int n, int nb, const double *__restrict__ p, const double *__restrict__ t,
     const double *__restrict__ s, const double *__restrict__ rhat, double *__restrict__ u,
-    double *__restrict__ r, double *__restrict__ part, const SolveState *st) {
+    double *__restrict__ r, const double *__restrict__ zpart, double *__restrict__ part,
+    SolveState *st) {
   if (st->done) return;
   __shared__ double sh[4];
-  const double alpha = st->alpha, zeta = st->zeta;
+  double z0 = 0.0, z1 = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) {
+    z0 += zpart[2 * i];
+    z1 += zpart[2 * i + 1];
+  }
+  const double znom = block_sum_all(z0, sh), zden = block_sum_all(z1, sh);
+  const double zeta = znom / zden;
+  const bool bad = zden == 0.0 || !finite(zeta);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->d0 = znom;
+    st->d1 = zden;
+    st->zeta = zeta;
+    if (bad) {
+      st->status = finite(zden) ? HMV_SOLVE_BREAKDOWN : HMV_SOLVE_NONFINITE;
+      st->done = 1;
+    }
+  }
+  if (bad) return;
+  const double alpha = st->alpha;
   double s0 = 0.0, s1 = 0.0;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     u[i] = (u[i] + alpha * p[i]) + zeta * t[i];
@@ -323,7 +374,7 @@ int ensure_work(hmv_arena_s *a, int hist_need) {
         (e = hipMalloc(&w->p, nb)) != hipSuccess || (e = hipMalloc(&w->v, nb)) != hipSuccess ||
         (e = hipMalloc(&w->t, nb)) != hipSuccess || (e = hipMalloc(&w->s, nb)) != hipSuccess ||
         (e = hipMalloc(&w->x2, nb)) != hipSuccess || (e = hipMalloc(&w->r2, nb)) != hipSuccess ||
-        (e = hipMalloc(&w->part, sizeof(double) * 2 * kMaxBlocks)) != hipSuccess ||
+        (e = hipMalloc(&w->part, sizeof(double) * 4 * kMaxBlocks)) != hipSuccess ||
         (e = hipMalloc(&w->st, sizeof(SolveState))) != hipSuccess) {
       hmv::free_solve_work(a);
       return hmv::hip_fail(e, "solver workspace");
@@ -356,15 +407,18 @@ hipError_t enqueue_iteration(hmv_arena_s *a, HmvSolveWork *w, double *u) {
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_dots, dim3(g), dim3(kThreads), 0, s, n, w->rhat, w->v,
                      (const double *)nullptr, (const double *)nullptr, w->part, w->st);
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, s, g, w->part, w->st, 1, w->hist);
-  hipLaunchKernelGGL(k_update_t, dim3(g), dim3(kThreads), 0, s, n, w->r, w->v, w->t, w->st);
+  // alpha and zeta are formed inside k_update_t / k_update_ur; the (r^, r),
+  // (r, r) partials go to the second half of the partial buffer
+  double *part2 = w->part + 2 * (size_t)g;
+  hipLaunchKernelGGL(k_update_t, dim3(g), dim3(kThreads), 0, s, n, g, w->r, w->v, w->t,
+                     (const double *)w->part, w->st);
   if ((e = hmv::enqueue_apply(a, w->t, w->s, 0.0, skip, s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_dots, dim3(g), dim3(kThreads), 0, s, n, w->s, w->t, w->s, w->s, w->part,
                      w->st);
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, s, g, w->part, w->st, 2, w->hist);
-  hipLaunchKernelGGL(k_update_ur, dim3(g), dim3(kThreads), 0, s, n, w->p, w->t, w->s, w->rhat, u,
-                     w->r, w->part, w->st);
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, s, g, w->part, w->st, 3, w->hist);
+  hipLaunchKernelGGL(k_update_ur, dim3(g), dim3(kThreads), 0, s, n, g, w->p, w->t, w->s, w->rhat,
+                     u, w->r, (const double *)w->part, part2, w->st);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(kThreads), 0, s, g, (const double *)part2, w->st, 3,
+                     w->hist);
   return hipGetLastError();
 }
 
