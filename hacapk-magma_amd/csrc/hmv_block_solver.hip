@@ -24,7 +24,7 @@
 namespace {
 
 constexpr int kT = 256;
-constexpr int kMaxB = 256;
+constexpr int kMaxB = 2048;  // 8 blocks (32 waves) per CU for the element-wise passes
 
 struct ColState {
   double alpha, beta, zeta, rr, rnorm, bnorm;
@@ -129,8 +129,8 @@ __device__ inline void col_step(ColState &c, double d0, double d1, const BlockSt
   }
 }
 
-// One thread per column: fixed-order sum of the block partials, then the
-// column's scalar step.
+// Fixed-order sum of each column's block partials, then the column's
+// scalar step.
 __global__ __launch_bounds__(kT) void kb_final(int nb, int k, const double *__restrict__ part,
                                                ColState *__restrict__ cs, BlockState *bs,
                                                int stage) {
@@ -138,17 +138,26 @@ __global__ __launch_bounds__(kT) void kb_final(int nb, int k, const double *__re
   __shared__ int sdone;
   if (threadIdx.x == 0) sdone = 1;
   __syncthreads();
-  const int j = threadIdx.x;
-  if (j < k) {
+  // wave w reduces columns w, w + 4, ...: its lanes sum the blocks in a
+  // fixed order, then a fixed shuffle tree (deterministic); lane 0 steps
+  // the column
+  const int lane = threadIdx.x & 63;
+  for (int j = threadIdx.x >> 6; j < k; j += kT / 64) {
     double d0 = 0.0, d1 = 0.0;
-    for (int b = 0; b < nb; ++b) {
+    for (int b = lane; b < nb; b += 64) {
       d0 += part[2 * ((size_t)b * k + j)];
       d1 += part[2 * ((size_t)b * k + j) + 1];
     }
-    ColState c = cs[j];
-    col_step(c, d0, d1, bs, stage);
-    cs[j] = c;
-    if (!c.done) atomicAnd(&sdone, 0);
+    for (int off = 32; off > 0; off >>= 1) {
+      d0 += __shfl_xor(d0, off, 64);
+      d1 += __shfl_xor(d1, off, 64);
+    }
+    if (lane == 0) {
+      ColState c = cs[j];
+      col_step(c, d0, d1, bs, stage);
+      cs[j] = c;
+      if (!c.done) atomicAnd(&sdone, 0);
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
