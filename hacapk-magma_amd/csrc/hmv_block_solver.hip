@@ -130,37 +130,37 @@ __device__ inline void col_step(ColState &c, double d0, double d1, const BlockSt
 }
 
 // Fixed-order sum of each column's block partials, then the column's
-// scalar step.
+// scalar step.  A block's 2k partial values are contiguous: thread t sums
+// value t % 2k over the blocks b = t / 2k + G i (G = 256 / 2k groups), so
+// every load is coalesced, then the G group sums are added in order.
 __global__ __launch_bounds__(kT) void kb_final(int nb, int k, const double *__restrict__ part,
                                                ColState *__restrict__ cs, BlockState *bs,
                                                int stage) {
   if (bs->all_done && stage != 0) return;
+  __shared__ double red[kT];
   __shared__ int sdone;
-  if (threadIdx.x == 0) sdone = 1;
+  const int tid = threadIdx.x, nv = 2 * k, G = kT / nv;  // k is a power of two <= 128
+  if (tid == 0) sdone = 1;
+  const int v = tid % nv, q = tid / nv;
+  double acc = 0.0;
+#pragma unroll 8
+  for (int b = q; b < nb; b += G) acc += part[(size_t)b * nv + v];
+  red[tid] = acc;
   __syncthreads();
-  // wave w reduces columns w, w + 4, ...: its lanes sum the blocks in a
-  // fixed order, then a fixed shuffle tree (deterministic); lane 0 steps
-  // the column
-  const int lane = threadIdx.x & 63;
-  for (int j = threadIdx.x >> 6; j < k; j += kT / 64) {
-    double d0 = 0.0, d1 = 0.0;
-    for (int b = lane; b < nb; b += 64) {
-      d0 += part[2 * ((size_t)b * k + j)];
-      d1 += part[2 * ((size_t)b * k + j) + 1];
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      d0 += __shfl_xor(d0, off, 64);
-      d1 += __shfl_xor(d1, off, 64);
-    }
-    if (lane == 0) {
-      ColState c = cs[j];
-      col_step(c, d0, d1, bs, stage);
-      cs[j] = c;
-      if (!c.done) atomicAnd(&sdone, 0);
-    }
+  if (tid < nv) {
+    double sum = red[tid];
+    for (int g = 1; g < G; ++g) sum += red[g * nv + tid];
+    red[tid] = sum;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid < k) {
+    ColState c = cs[tid];
+    col_step(c, red[2 * tid], red[2 * tid + 1], bs, stage);
+    cs[tid] = c;
+    if (!c.done) atomicAnd(&sdone, 0);
+  }
+  __syncthreads();
+  if (tid == 0) {
     if (stage == 0) bs->iter = 0;
     if (stage == 3) bs->iter += 1;
     bs->all_done = sdone;
