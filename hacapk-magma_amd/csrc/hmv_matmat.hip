@@ -138,10 +138,20 @@ struct Group {
 // range and loads 0, and rows past the block only feed accumulator rows
 // that are not stored.  B: from the column's input pointer tbl[col - tbase]
 // (zeros past the window).
-template <bool EXACT, int DT, int KT, int NT, int U>
+// Where the B loads of a column come from: the window's pointer table
+// (tbl[col - tbase]), or, for a part whose inputs are one x run (phase 1),
+// straight from that run (xr + col k) with zeros past column hi.
+struct BSrc {
+  gdouble *const *tbl;
+  int tbase;
+  gdouble *xr, *zr;
+  int hi;
+};
+
+template <bool EXACT, bool RUN, int DT, int KT, int NT, int U>
 __device__ inline void load_group(Group<DT, KT, NT, U> &g, __amdgpu_buffer_rsrc_t rs, uint32_t cs,
-                                 uint32_t r_es, int q0, gdouble *const *__restrict__ tbl,
-                                 int tbase, const BLane<KT> &bl) {
+                                 uint32_t r_es, int q0, const BSrc &src, int k,
+                                 const BLane<KT> &bl) {
   using E = Entry<DT>;
   const int cq = threadIdx.x >> 4;
 #pragma unroll
@@ -150,7 +160,11 @@ __device__ inline void load_group(Group<DT, KT, NT, U> &g, __amdgpu_buffer_rsrc_
     const uint32_t cb = (uint32_t)col * cs + r_es;
 #pragma unroll
     for (int m = 0; m < NT; ++m) g.a[u][m] = E::load_raw(rs, cb + (uint32_t)(m * 16 * E::es));
-    gdouble *p = tbl[col - tbase] + bl.j0;
+    gdouble *p;
+    if constexpr (RUN)
+      p = col < src.hi ? src.xr + (size_t)col * k : src.zr;
+    else
+      p = src.tbl[col - src.tbase] + bl.j0;
 #pragma unroll
     for (int tt = 0; tt < KT; ++tt) g.b[u][tt] = EXACT ? p[16 * tt] : p[bl.jd[tt]];
   }
@@ -186,57 +200,73 @@ __device__ inline void fill_window(gdouble **__restrict__ tbl, const Inputs &in,
   }
 }
 
+// Multiplies part columns [lo, hi) into the accumulators, U quads at a
+// time, the next group's loads in flight while the current one multiplies.
+template <typename Acc, bool EXACT, bool RUN, int DT, int KT, int TW, int NT>
+__device__ inline void run_columns(typename Mfma<Acc>::acc (&acc)[TW][KT],
+                                   __amdgpu_buffer_rsrc_t rs, uint32_t cs, uint32_t r_es, int lo,
+                                   int hi, const BSrc &src, int k, const BLane<KT> &bl) {
+  constexpr int U = group_quads<KT, NT>();
+  constexpr int S = 4 * U;  // columns per group
+  static_assert(S <= kPad, "a group past the window reads zero entries");
+  Group<DT, KT, NT, U> g0, g1;
+  load_group<EXACT, RUN>(g0, rs, cs, r_es, lo, src, k, bl);
+  int q0 = lo;
+  for (; q0 + 2 * S < hi; q0 += 2 * S) {  // g0 holds group q0; two more follow
+    load_group<EXACT, RUN>(g1, rs, cs, r_es, q0 + S, src, k, bl);
+    mma_group<Acc>(acc, g0);
+    load_group<EXACT, RUN>(g0, rs, cs, r_es, q0 + 2 * S, src, k, bl);
+    mma_group<Acc>(acc, g1);
+  }
+  const bool last2 = q0 + S < hi;  // one or two groups left
+  if (last2) load_group<EXACT, RUN>(g1, rs, cs, r_es, q0 + S, src, k, bl);
+  mma_group<Acc>(acc, g0);
+  if (last2) mma_group<Acc>(acc, g1);
+}
+
 // Accumulates the product of NT tiles over one part (ncolp columns stored
-// with DT at rs, item columns cbase + [0, ncolp)), kWin columns at a time:
-// the window's input pointers are tabled in LDS, then its column groups
-// run U quads at a time, the next group's loads in flight while the
-// current one multiplies.
-template <typename Acc, bool EXACT, int DT, int KT, int TW, int NT>
+// with DT at rs, item columns cbase + [0, ncolp)).  Inputs of one x run
+// (phase 1) are addressed directly; otherwise kWin columns at a time, the
+// window's input pointers tabled in LDS first.
+template <typename Acc, bool EXACT, bool RUN, int DT, int KT, int TW, int NT>
 __device__ inline void part_mma(typename Mfma<Acc>::acc (&acc)[TW][KT], __amdgpu_buffer_rsrc_t rs,
                                 int rows_p, int tile0, int ncolp, int cbase, const Inputs &in,
                                 const int *__restrict__ codes, gdouble **__restrict__ tbl,
                                 const double *__restrict__ x, const double *__restrict__ t, int k,
                                 const BLane<KT> &bl) {
-  constexpr int U = group_quads<KT, NT>();
-  constexpr int S = 4 * U;  // columns per group
-  static_assert(S <= kPad, "a group past the window reads zero entries");
+  if (ncolp <= 0) return;
   const uint32_t cs = (uint32_t)rows_p * Entry<DT>::es;
   const uint32_t r_es = (uint32_t)(tile0 * 16 + bl.j0) * Entry<DT>::es;
-  for (int w0 = 0; w0 < ncolp; w0 += kWin) {
-    const int w1 = min(ncolp, w0 + kWin);
-    __syncthreads();  // the table's previous contents are read
-    fill_window(tbl, in, codes, cbase, w0, w1, x, t, k);
-    __syncthreads();
-    Group<DT, KT, NT, U> g0, g1;
-    load_group<EXACT>(g0, rs, cs, r_es, w0, tbl, w0, bl);
-    int q0 = w0;
-    for (; q0 + 2 * S < w1; q0 += 2 * S) {  // g0 holds group q0; two more follow
-      load_group<EXACT>(g1, rs, cs, r_es, q0 + S, tbl, w0, bl);
-      mma_group<Acc>(acc, g0);
-      load_group<EXACT>(g0, rs, cs, r_es, q0 + 2 * S, tbl, w0, bl);
-      mma_group<Acc>(acc, g1);
+  if constexpr (RUN) {
+    const BSrc src{nullptr, 0, (gdouble *)(x + (size_t)(in.z0.src + cbase) * k) + bl.j0,
+                   (gdouble *)g_zero_inputs + bl.j0, ncolp};
+    run_columns<Acc, EXACT, true, DT, KT, TW, NT>(acc, rs, cs, r_es, 0, ncolp, src, k, bl);
+  } else {
+    for (int w0 = 0; w0 < ncolp; w0 += kWin) {
+      const int w1 = min(ncolp, w0 + kWin);
+      __syncthreads();  // the table's previous contents are read
+      fill_window(tbl, in, codes, cbase, w0, w1, x, t, k);
+      __syncthreads();
+      const BSrc src{tbl, w0, nullptr, nullptr, w1};
+      run_columns<Acc, EXACT, false, DT, KT, TW, NT>(acc, rs, cs, r_es, w0, w1, src, k, bl);
     }
-    const bool last2 = q0 + S < w1;  // one or two groups left
-    if (last2) load_group<EXACT>(g1, rs, cs, r_es, q0 + S, tbl, w0, bl);
-    mma_group<Acc>(acc, g0);
-    if (last2) mma_group<Acc>(acc, g1);
   }
 }
 
 // Both parts of the item for NT tiles (the unit's tile count, fixed at
 // compile time so that the accumulators stay in place); an item stored
 // with one type has one part.
-template <typename Acc, bool EXACT, int DT0, int DT1, int KT, int TW, int NT>
+template <typename Acc, bool EXACT, bool RUN, int DT0, int DT1, int KT, int TW, int NT>
 __device__ inline void unit_mma(typename Mfma<Acc>::acc (&acc)[TW][KT], const HmvWork &w,
                                 __amdgpu_buffer_rsrc_t rs0, __amdgpu_buffer_rsrc_t rs1, int tile0,
                                 const Inputs &in, const int *__restrict__ codes,
                                 gdouble **__restrict__ tbl, const double *__restrict__ x,
                                 const double *__restrict__ t, int k, const BLane<KT> &bl) {
-  part_mma<Acc, EXACT, DT0, KT, TW, NT>(acc, rs0, w.rows_p, tile0, w.ncol0, 0, in, codes, tbl, x,
-                                        t, k, bl);
+  part_mma<Acc, EXACT, RUN, DT0, KT, TW, NT>(acc, rs0, w.rows_p, tile0, w.ncol0, 0, in, codes,
+                                             tbl, x, t, k, bl);
   if constexpr (DT0 != DT1)
-    part_mma<Acc, EXACT, DT1, KT, TW, NT>(acc, rs1, w.rows_p, tile0, w.ncol1, w.ncol0, in, codes,
-                                          tbl, x, t, k, bl);
+    part_mma<Acc, EXACT, RUN, DT1, KT, TW, NT>(acc, rs1, w.rows_p, tile0, w.ncol1, w.ncol0, in,
+                                               codes, tbl, x, t, k, bl);
 }
 
 // One wave per unit {item, tile0 << 8 | nt}: tiles tile0 .. tile0+nt-1 of
@@ -251,7 +281,7 @@ __global__ __launch_bounds__(64) void hmv_matmat_kernel(
   if (skip && *skip) return;
   constexpr int TW = Shape<KT>::TW;
   __shared__ int runs[2 * HMV_RUN_MAX];
-  __shared__ gdouble *tbl[kWin + kPad];
+  __shared__ gdouble *tbl[PHASE == 1 ? 1 : kWin + kPad];  // phase 1 reads its x run directly
   const int2 unit = units[blockIdx.x];
   const HmvWork w = works[unit.x];
   const int tile0 = unit.y >> 8, nt = unit.y & 255;
@@ -272,20 +302,20 @@ __global__ __launch_bounds__(64) void hmv_matmat_kernel(
 #pragma unroll
     for (int tt = 0; tt < KT; ++tt) acc[m][tt] = typename Mfma<Acc>::acc{0, 0, 0, 0};
   if (nt >= TW)
-    unit_mma<Acc, EXACT, DT0, DT1, KT, TW, TW>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in, k,
+    unit_mma<Acc, EXACT, PHASE == 1, DT0, DT1, KT, TW, TW>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in, k,
                                                bl);
   else if constexpr (TW == 2)
-    unit_mma<Acc, EXACT, DT0, DT1, KT, TW, 1>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in, k,
+    unit_mma<Acc, EXACT, PHASE == 1, DT0, DT1, KT, TW, 1>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in, k,
                                               bl);
   else if constexpr (TW == 4) {
     if (nt == 1)
-      unit_mma<Acc, EXACT, DT0, DT1, KT, TW, 1>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
+      unit_mma<Acc, EXACT, PHASE == 1, DT0, DT1, KT, TW, 1>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
                                                 k, bl);
     else if (nt == 2)
-      unit_mma<Acc, EXACT, DT0, DT1, KT, TW, 2>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
+      unit_mma<Acc, EXACT, PHASE == 1, DT0, DT1, KT, TW, 2>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
                                                 k, bl);
     else
-      unit_mma<Acc, EXACT, DT0, DT1, KT, TW, 3>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
+      unit_mma<Acc, EXACT, PHASE == 1, DT0, DT1, KT, TW, 3>(acc, w, rs0, rs1, tile0, in, codes, tbl, x, t_in,
                                                 k, bl);
   }
   // lane holds rows Mfma::row(lane, i), column lane%16 of each 16 x 16 tile
