@@ -10,9 +10,10 @@
 // 8-byte (or 4/2-byte) load per lane and tile gives the 16 x 4 A fragment
 // (lane l: row l%16 of column l/16 of the column-major block), and KT loads
 // per lane give the 4 x 16 B fragments x[column][16 tt + l%16] straight
-// from the interleaved inputs, through a per-wave LDS table of the columns'
-// input addresses (L2-resident inputs: the units of an XCD are neighbouring
-// items, so they share columns).  Each (tile, RHS tile) pair then takes one
+// from the interleaved inputs: phase 1 from its item's x run, phase 2
+// through a per-wave LDS table of the columns' input addresses (L2-resident
+// inputs: the units of an XCD are neighbouring items, so they share
+// columns).  Each (tile, RHS tile) pair then takes one
 // MFMA (16x16x4; FP64, or FP32 for FP32/BF16 arenas) into its 16 x 16
 // accumulator; the B fragment is shared by the group's tiles.  Loads run
 // one group of U quads ahead of the MFMAs.  Columns past the block load
