@@ -112,7 +112,7 @@ typedef const double __attribute__((address_space(1))) gdouble;
 
 constexpr int kWin = 1024;  // input pointers tabled per window (LDS, per wave)
 constexpr int kPad = 32;    // zero-input entries past a window (one group of columns)
-__device__ double g_zero_inputs[128];  // B source of the columns past a window (never written)
+__device__ double g_zero_inputs[128];  // B source of columns past a window or part (never written)
 
 // Per-wave constants of the B loads.  Lane l reads input 16 tt + l%16 of
 // RHS tile tt; with EXACT (k == 16 KT) straight from its column pointer,
@@ -133,12 +133,6 @@ struct Group {
   double b[U][KT];
 };
 
-// Loads the group of part columns [q0, q0 + 4U): lane l takes column
-// q0 + 4u + l/16.  A: row 16 (tile0 + m) + l%16 at byte offset
-// col * cs + r_es + 16 m es; a column past the part lies past the buffer
-// range and loads 0, and rows past the block only feed accumulator rows
-// that are not stored.  B: from the column's input pointer tbl[col - tbase]
-// (zeros past the window).
 // Where the B loads of a column come from: the window's pointer table
 // (tbl[col - tbase]), or, for a part whose inputs are one x run (phase 1),
 // straight from that run (xr + col k) with zeros past column hi.
@@ -148,6 +142,13 @@ struct BSrc {
   gdouble *xr, *zr;
   int hi;
 };
+
+// Loads the group of part columns [q0, q0 + 4U): lane l takes column
+// q0 + 4u + l/16.  A: row 16 (tile0 + m) + l%16 at byte offset
+// col * cs + r_es + 16 m es; a column past the part lies past the buffer
+// range and loads 0, and rows past the block only feed accumulator rows
+// that are not stored.  B: from the column's inputs as src gives them
+// (zeros past the window or the part).
 
 template <bool EXACT, bool RUN, int DT, int KT, int NT, int U>
 __device__ inline void load_group(Group<DT, KT, NT, U> &g, __amdgpu_buffer_rsrc_t rs, uint32_t cs,
