@@ -70,8 +70,8 @@ __global__ __launch_bounds__(kT) void g_dots2(int n, const double *__restrict__ 
   }
 }
 
-// (ap_ik, r), (ap_ik, ap_ik) for the slot ik = step mod m, which the
-// previous step (or the setup) left in st->ikn
+// (ap_ik, r), (ap_ik, ap_ik) for the slot ik = st->ikn of the first step
+// (at setup; g_update_dir produces them for the later steps)
 __global__ __launch_bounds__(kT) void g_dots_alpha(int n, const double *__restrict__ ap,
                                                    const double *__restrict__ r,
                                                    double *__restrict__ part, const GcrState *st) {
