@@ -40,6 +40,16 @@ class HmvError(RuntimeError):
 # ---------------------------------------------------------------------------
 # structs
 # ---------------------------------------------------------------------------
+def _leaf_ints(ptr, nlf, stride):
+    """(ltmtx, kt, nstrtl, ndl, nstrtt, ndt) of nlf leaf records of `stride`
+    bytes at address ptr, as an (nlf, 6) int64 table (the records' first six
+    int32 fields, read through a strided view)."""
+    if nlf == 0:
+        return np.zeros((0, 6), np.int64)
+    raw = np.ctypeslib.as_array((C.c_int32 * (nlf * stride // 4)).from_address(ptr))
+    return raw.reshape(nlf, stride // 4)[:, :6].astype(np.int64)
+
+
 class LeafRec(C.Structure):
     """stc_HACApK_leafmtx (HACApK_MAGMA.h:12-20)."""
     _fields_ = [("ltmtx", C.c_int), ("kt", C.c_int), ("nstrtl", C.c_int), ("ndl", C.c_int),
@@ -451,8 +461,8 @@ class DeviceHMatrix:
                                            specs, stride.value, C.byref(cp), C.byref(d)),
                    "hmv_construct")
             self._d = d
-            self.table = np.array([(r.ltmtx, 0, r.nstrtl, r.ndl, r.nstrtt, r.ndt)
-                                   for r in (LeafRec * self.nlf).from_address(specs)], np.int64)
+            self.table = _leaf_ints(specs, self.nlf, stride.value)
+            self.table[:, 1] = 0
         finally:
             L.hmvgen_free(h)
         kt = np.zeros(self.nlf, np.int32)
@@ -568,8 +578,7 @@ class HMatrix:
 
     def leaf_table(self):
         """numpy table of (ltmtx, kt, nstrtl, ndl, nstrtt, ndt) per leaf."""
-        arr = self.leaves()
-        return np.array([(l.ltmtx, l.kt, l.nstrtl, l.ndl, l.nstrtt, l.ndt) for l in arr], np.int64)
+        return _leaf_ints(self.leaves_ptr, self.nlf, self.stride)
 
     def to_cluster_order(self, v):
         """Original order -> cluster order (b(:nd)=rhs(lod(:nd)), m_HACApK_use.f90:285)."""

@@ -512,7 +512,7 @@ hipError_t upload(T **dst, const T *src, size_t n) {
 namespace hmv {
 
 // Runs the arena fill records on the device (the image already zeroed).
-hipError_t launch_fill(const std::vector<HmvFillOp> &ops, char *arena) {
+hipError_t launch_fill(const FillOps &ops, char *arena) {
   constexpr size_t kBatch = 1 << 20;  // records per launch (grid size)
   HmvFillOp *d = nullptr;
   const size_t nb = std::min(ops.size(), kBatch);

@@ -70,7 +70,7 @@ void free_gcr_work(hmv_arena_s *a);
 hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
                                const int *skip, hipStream_t s);
 // Runs an arena's fill records on the device (image zeroed beforehand).
-hipError_t launch_fill(const std::vector<HmvFillOp> &ops, char *arena);
+hipError_t launch_fill(const FillOps &ops, char *arena);
 // Arena from leaves whose factors are in device memory (hmv_construct).
 int build_from_device_leaves(int nd, int nlf, const void *leaves, int stride, int prec,
                              const hmv_tuning *tuning, hmv_arena *out);

@@ -14,9 +14,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <map>
 #include <numeric>
 #include <thread>
+#include <unordered_map>
 #include <utility>
 
 namespace hmv {
@@ -213,12 +213,17 @@ bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const 
     int r = 0;
     int np = 1;  // column chunks (t partials per rank vector)
   };
-  std::map<std::pair<int, int>, int> rkey, ckey;
+  // group index by (first row or column, size), numbered in order of first
+  // appearance
+  auto gkey = [](int s, int n) { return (uint64_t)(uint32_t)s << 32 | (uint32_t)n; };
+  std::unordered_map<uint64_t, int> rkey, ckey;
+  rkey.reserve((size_t)nlf);
+  ckey.reserve((size_t)nlf);
   std::vector<RowGroup> rg;
   std::vector<ColGroup> cg;
   for (int i = 0; i < nlf; ++i) {
     const RefLeaf *l = L[i];
-    auto rk = std::make_pair(l->nstrtl - 1, l->ndl);
+    const uint64_t rk = gkey(l->nstrtl - 1, l->ndl);
     auto it = rkey.find(rk);
     int r;
     if (it == rkey.end()) {
@@ -234,7 +239,7 @@ bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const 
     } else if (l->kt > 0) {
       rg[r].lr.push_back(i);
       rg[r].k += l->kt;
-      auto ck = std::make_pair(l->nstrtt - 1, l->ndt);
+      const uint64_t ck = gkey(l->nstrtt - 1, l->ndt);
       auto jt = ckey.find(ck);
       int c;
       if (jt == ckey.end()) {
@@ -328,11 +333,17 @@ bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const 
     err = "too many t entries for 32-bit input codes";
     return false;
   }
-  // rank vectors of each column group, in group order: (leaf, l)
-  std::vector<std::vector<std::pair<int, int>>> cg_rows(cg.size());
-  for (int c = 0; c < (int)cg.size(); ++c)
+  // rank vectors of each column group, in group order: (leaf, l), in one
+  // flat array where group c's rows start at cg_first[c]
+  std::vector<int64_t> cg_first(cg.size() + 1, 0);
+  for (size_t c = 0; c < cg.size(); ++c) cg_first[c + 1] = cg_first[c] + cg[c].r;
+  std::vector<std::pair<int, int>> cg_flat((size_t)cg_first[cg.size()]);
+  parallel_for((int64_t)cg.size(), [&](int64_t c) {
+    std::pair<int, int> *o = cg_flat.data() + cg_first[c];
     for (int i : cg[c].lr)
-      for (int k = 0; k < L[i]->kt; ++k) cg_rows[c].push_back({i, k});
+      for (int k = 0; k < L[i]->kt; ++k) *o++ = {i, k};
+  });
+  auto cg_rows = [&](int c) { return cg_flat.data() + cg_first[c]; };
 
   // ---- phase-2 atoms ------------------------------------------------------
   std::vector<int> bnd;
@@ -455,7 +466,7 @@ bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const 
     w.zseg_end = (int)P.zseg.size();
     w.out_base = (int)P.outidx.size();
     for (int r = 0; r < p.nr; ++r) {
-      const auto &rv = cg_rows[p.c][p.r0 + r];
+      const auto &rv = cg_rows(p.c)[p.r0 + r];
       const int64_t s = leaf_tbase[rv.first] + rv.second;
       P.outidx.push_back((int)(tout[s] + p.k));
     }
@@ -525,35 +536,63 @@ bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const 
 
   // ---- fill ---------------------------------------------------------------
   if (device) {
-    // the same copies as the host fill below, as records for the device
-    // (the image is zeroed there first)
-    for (const Piece &p : pieces) {
+    // The same copies as the host fill below, as records for the device (the
+    // image is zeroed there first).  A piece's consecutive rank vectors of
+    // one leaf are one record (V's rows lie ndt apart).  The records are
+    // counted per piece and per slice, then written in parallel at their
+    // offsets.
+    const size_t npc = pieces.size(), nsl = slices.size();
+    std::vector<int64_t> at(npc + nsl + 1, 0);
+    parallel_for((int64_t)npc, [&](int64_t pi) {
+      const Piece &p = pieces[pi];
+      const std::pair<int, int> *rows = cg_rows(p.c);
+      int64_t n = 0;
+      for (int r = 0; r < p.nr; ++r)
+        if (r == 0 || rows[p.r0 + r].first != rows[p.r0 + r - 1].first) ++n;
+      at[pi + 1] = n;
+    });
+    parallel_for((int64_t)nsl, [&](int64_t si) {
+      int64_t n = 0;
+      for (int gi : slices[si].cov) n += (int64_t)(rg[gi].dense.size() + rg[gi].lr.size());
+      at[npc + si + 1] = n;
+    });
+    for (size_t q = 0; q < npc + nsl; ++q) at[q + 1] += at[q];
+    P.fill.resize((size_t)at[npc + nsl]);
+    HmvFillOp *F = P.fill.data();
+    parallel_for((int64_t)npc, [&](int64_t pi) {
+      const Piece &p = pieces[pi];
+      const std::pair<int, int> *rows = cg_rows(p.c);
       const int rows_p = (int)round_up(p.nr, vr);
-      for (int r = 0; r < p.nr; ++r) {
-        const auto &rv = cg_rows[p.c][p.r0 + r];
+      HmvFillOp *o = F + at[pi];
+      for (int r = 0; r < p.nr;) {
+        const auto &rv = rows[p.r0 + r];
+        int m = 1;  // rank vectors rv.second .. rv.second + m - 1 of one leaf
+        while (r + m < p.nr && rows[p.r0 + r + m].first == rv.first) ++m;
         const RefLeaf *l = L[rv.first];
-        P.fill.push_back(HmvFillOp{p.off, l->a1 + (size_t)rv.second * l->ndt + p.c0, 0, 1, rows_p, r,
-                                   0, 1, p.cw, dt_lr});
+        *o++ = HmvFillOp{p.off, l->a1 + (size_t)rv.second * l->ndt + p.c0, l->ndt, 1, rows_p, r,
+                         0, m, p.cw, dt_lr};
+        r += m;
       }
-    }
-    for (const Slice &S : slices) {
+    });
+    parallel_for((int64_t)nsl, [&](int64_t si) {
+      const Slice &S = slices[si];
       const int rows = S.b - S.a;
       const int rows_p = (int)round_up(rows, vr2);
+      HmvFillOp *o = F + at[npc + si];
       int c0 = 0, c1 = 0;
       auto put_dense = [&](const RowGroup &g, int64_t off, int dt, int &col) {
         for (int i : g.dense) {
           const RefLeaf *l = L[i];
-          P.fill.push_back(HmvFillOp{off, l->a1 + (size_t)(S.a - g.s) * l->ndt, l->ndt, 1, rows_p, 0,
-                                     col, rows, l->ndt, dt});
+          *o++ = HmvFillOp{off, l->a1 + (size_t)(S.a - g.s) * l->ndt, l->ndt, 1, rows_p, 0, col,
+                           rows, l->ndt, dt};
           col += l->ndt;
         }
       };
       auto put_u = [&](const RowGroup &g, int64_t off, int dt, int &col) {
-        for (int i : g.lr) {
+        for (int i : g.lr) {  // rank > 0 only
           const RefLeaf *l = L[i];
-          if (l->kt > 0)
-            P.fill.push_back(HmvFillOp{off, u_factor(l) + (S.a - g.s), 1, l->ndl, rows_p, 0, col,
-                                       rows, l->kt, dt});
+          *o++ = HmvFillOp{off, u_factor(l) + (S.a - g.s), 1, l->ndl, rows_p, 0, col, rows, l->kt,
+                           dt};
           col += l->kt;
         }
       };
@@ -567,7 +606,7 @@ bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const 
           put_u(g, S.off1, dt_lr, c1);
         }
       }
-    }
+    });
   } else {
   parallel_for((int64_t)pieces.size(), [&](int64_t pi) {
     const Piece &p = pieces[pi];
@@ -575,7 +614,7 @@ bool plan_impl(int nd, int nlf, const void *leaves, int stride, int prec, const 
     const int rows_p = (int)round_up(p.nr, vr);
     std::memset(dst, 0, (size_t)(round_up(p.off + p.bytes, HMV_ARENA_ALIGN) - p.off));
     for (int r = 0; r < p.nr; ++r) {
-      const auto &rv = cg_rows[p.c][p.r0 + r];
+      const auto &rv = cg_rows(p.c)[p.r0 + r];
       const RefLeaf *l = L[rv.first];
       const double *v = l->a1 + (size_t)rv.second * l->ndt + p.c0;  // V(:, l) from column c0
       for (int j = 0; j < p.cw; ++j) store(dst, dt_lr, r + (int64_t)rows_p * j, v[j]);

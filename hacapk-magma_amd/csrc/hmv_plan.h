@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "hmv_internal.h"
@@ -43,6 +44,28 @@ struct PlanStats {
   int max_ndt = 0;
 };
 
+// Allocator whose resize leaves plain records uninitialised (they are all
+// written afterwards, in parallel).
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+  template <typename U>
+  void construct(U *p) noexcept {
+    ::new (static_cast<void *>(p)) U;
+  }
+  template <typename U, typename... A>
+  void construct(U *p, A &&...a) {
+    ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+  }
+};
+using FillOps = std::vector<HmvFillOp, NoInitAlloc<HmvFillOp>>;
+
 struct Plan {
   int nd = 0;
   int prec = kF64;
@@ -65,7 +88,7 @@ struct Plan {
   std::vector<int> seg;            // instance row offsets (ninst + 1): {0, nd} for one H-matrix
   // device-fill plans (leaf factors in device memory): no host image; the
   // copies that make it, run on the device at upload
-  std::vector<HmvFillOp> fill;
+  FillOps fill;
 };
 
 // Tunables of the repack (chosen by the autotuner or left at defaults).

@@ -94,7 +94,7 @@ def test_high_rank_blocks_take_the_second_pass(gpu, O, H):
     dev.free()
 
 
-@pytest.mark.parametrize("prec", ["f64", "bf16"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16", "bf16_all"])
 def test_arena_from_device_factors(gpu, O, H, tmp_path, prec):
     """The arena repacked on the device from device factors is byte-equal to
     the host repack of the same factors (downloaded), and applies bit-equal."""
@@ -111,7 +111,8 @@ def test_arena_from_device_factors(gpu, O, H, tmp_path, prec):
     assert (tmp_path / "dev.hmv").read_bytes() == (tmp_path / "host.hmv").read_bytes()
     y = a.matvec(x)
     y_ref = O.adot_body_lfmtx(got, x)
-    assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < (1e-12 if prec == "f64" else 2e-2)
+    tol = {"f64": 1e-12, "f32": 1e-6, "bf16": 2e-2, "bf16_all": 2e-2}[prec]
+    assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < tol
     a.free()
     b.free()
     dev.free()
