@@ -886,14 +886,16 @@ int build_impl(int nnode, const double *nodes, int nface, const int *faces,
   nthr = 1;
 #endif
   int err = 0;
-  // Largest blocks first so that the dynamic schedule ends balanced.
+  // Largest blocks first so that the dynamic schedule of the fill ends
+  // balanced (the block structure alone keeps leaf order).
   std::vector<int> order(nlf);
   for (int i = 0; i < nlf; ++i) order[i] = i;
-  std::sort(order.begin(), order.end(), [&](int a, int b) {
-    int64_t sa = (int64_t)bg.leaves[a].ndl * bg.leaves[a].ndt;
-    int64_t sb = (int64_t)bg.leaves[b].ndl * bg.leaves[b].ndt;
-    return sa != sb ? sa > sb : a < b;
-  });
+  if (fill)
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+      int64_t sa = (int64_t)bg.leaves[a].ndl * bg.leaves[a].ndt;
+      int64_t sb = (int64_t)bg.leaves[b].ndl * bg.leaves[b].ndt;
+      return sa != sb ? sa > sb : a < b;
+    });
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthr)
   for (int q = 0; q < nlf; ++q) {
     const int ip = order[q];
