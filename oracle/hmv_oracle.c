@@ -29,7 +29,9 @@
  *   - gcrm: within one restart cycle GCR minimises ||r|| over the Krylov space,
  *     so its residual norms equal GMRES's (scipy) to rounding; convergence to the
  *     dense solution.  The restart step is the reference's own (see the
- *     function); no reference run covers it: parity of that step unpinned.
+ *     function): the step after a restart equals its closed form built from a
+ *     QR factorisation of the Krylov matrix (GMRES(m) residual, slot-1 scaling
+ *     1 + beta1 with the old capap(1), projection on slots 2..m).
  *   - bicgstab_pipe: in exact arithmetic the pipelined recurrences produce the
  *     iterates of BiCGSTAB with the same shadow residual, so its residual norms
  *     equal oracle_bicgstab's (itself pinned above) while far above rounding;

@@ -7,7 +7,9 @@
 * setcutthread ranges tile the leaf list (m_HACApK_base.f90:584-611);
 * BiCGSTAB (m_HACApK_solve.f90:342-449) == scipy's BiCGSTAB iterates on the
   same dense matrix, terminates on tiny systems, and reproduces the
-  reference's recorded run (ivb/job_cpu.sh.o78632) on input_10ts.pbs.
+  reference's recorded run (ivb/job_cpu.sh.o78632) on input_10ts.pbs;
+* GCR(m) (m_HACApK_solve.f90:506-571) == GMRES within a cycle, and its restart
+  step == the closed form from a QR factorisation of the Krylov matrix.
 """
 import ctypes as C
 import json
@@ -99,6 +101,46 @@ def test_gcrm_first_cycle_matches_gmres(O, H, sphere3):
     k = min(20, len(res), nstp)
     assert k >= 15
     assert np.allclose(np.log10(hist[1:k + 1]), np.log10(res[:k]), atol=1e-8)
+
+
+@pytest.mark.parametrize("m", [2, 3, 4])
+def test_gcrm_restart_step_closed_form(O, H, sphere3, m):
+    """The restart of GCR(m) (m_HACApK_solve.f90:553-566) in closed form.  The
+    new direction is written into slot 1 before the projections, so the
+    slot-1 term uses the new A r with the old capap(1) = ||A r0||^2 and scales
+    r by 1 + beta1, beta1 = -||A r_m||^2 / ||A r0||^2, while slots 2..m (an
+    orthogonal basis of A K_m(r0) without A r0) are projected out of the
+    unscaled A r_m.  Here r_m (the GMRES(m) residual) and that basis come from
+    a QR factorisation of the Krylov matrix, not from the recurrence; step
+    m + 1 is the minimal-residual step along the resulting direction.  A clean
+    restart, a full orthogonalisation or a missing 1 + beta1 each miss it."""
+    hm = sphere3
+    A = O.dense(hm)
+    b = H.uniform(9, hm.nd)
+    _, nstp, hist = O.gcrm(hm, b, None, mstep=m + 1, mreset=m, eps=1e-14)
+    assert nstp == m + 1
+    bn = np.linalg.norm(b)
+    K = np.empty((hm.nd, m))
+    v = b.copy()
+    for j in range(m):  # normalised Krylov vectors A^j r0 (u0 = 0, r0 = b)
+        v = A @ v
+        v /= np.linalg.norm(v)
+        K[:, j] = v
+    Q, _ = np.linalg.qr(K)
+    rm = b - Q @ (Q.T @ b)
+    w = A @ rm
+    ar0 = A @ b
+    beta1 = -(w @ w) / (ar0 @ ar0)
+    Q2 = Q[:, 1:]
+
+    def after_step(apn):
+        return np.linalg.norm(rm - (apn @ rm) / (apn @ apn) * apn) / bn
+
+    want = after_step((1 + beta1) * w - Q2 @ (Q2.T @ w))
+    assert abs(np.log10(hist[m]) - np.log10(np.linalg.norm(rm) / bn)) < 1e-9
+    assert abs(np.log10(hist[m + 1]) - np.log10(want)) < 1e-9
+    for other in (w, w - Q @ (Q.T @ w), w - Q2 @ (Q2.T @ w)):
+        assert abs(np.log10(after_step(other)) - np.log10(want)) > 1e-4
 
 
 def test_gcrm_converges_to_dense_solution(O, H, sphere3):
