@@ -139,9 +139,28 @@ TFIN_DTYPE = np.dtype([("slot", "<i4"), ("base", "<i4"), ("np", "<i4"), ("pad", 
 
 
 class RefLeafmtxp(C.Structure):
-    """C view of st_HACApK_leafmtxp built without HAVE_MAGMA (m_HACApK_base.f90:64-76)."""
+    """hmv_ref_leafmtxp: C view of st_HACApK_leafmtxp in a HAVE_MAGMA /
+    HAVE_MAGMA_BATCH build (m_HACApK_base.f90:64-235, HACApK_MAGMA.h:28-106),
+    the layout every GPU call site of the reference passes (544 bytes)."""
+    _fields_ = [("nd", C.c_int), ("nlf", C.c_int), ("nlfkt", C.c_int), ("ktmax", C.c_int),
+                ("st_lf_stride", C.c_int),
+                ("m", C.c_int), ("n", C.c_int), ("gn", C.c_int), ("max_block", C.c_int),
+                ("gflops", C.c_double), ("gpu_vectors", C.c_void_p * 6),
+                ("num_batch", C.c_int), ("total_size_y", C.c_int), ("transA", C.c_int),
+                ("batch_tables", C.c_void_p * 20),
+                ("num_streamed", C.c_int), ("num_streamed_t", C.c_int),
+                ("streamed_tables", C.c_void_p * 6), ("mgpu_tables", C.c_void_p * 25),
+                ("mpi_rank", C.c_int), ("st_lf", C.c_void_p)]
+
+
+class RefLeafmtxpPlain(C.Structure):
+    """hmv_ref_leafmtxp_plain: st_HACApK_leafmtxp of a build without
+    HAVE_MAGMA / HAVE_MAGMA_BATCH (m_HACApK_base.f90:64-76, 234)."""
     _fields_ = [("nd", C.c_int), ("nlf", C.c_int), ("nlfkt", C.c_int), ("ktmax", C.c_int),
                 ("st_lf_stride", C.c_int), ("st_lf", C.c_void_p)]
+
+
+REF_LAYOUT = {"magma": 0, "plain": 1}
 
 
 class RefLcontrol(C.Structure):
@@ -263,21 +282,28 @@ def hmv_lib():
         L.hmv_arena_from_plan.argtypes = [vp, C.POINTER(vp)]
         L.hmv_arena_save.argtypes = [vp, C.c_char_p]
         L.hmv_arena_load.argtypes = [C.c_char_p, C.POINTER(vp)]
-        L.c_hacapk_adot_body_lfcpy_batch_sorted_.argtypes = [_I, C.POINTER(RefLeafmtxp)]
+        L.hmv_ref_set_layout.argtypes = [C.c_int]
+        L.hmv_ref_get_layout.argtypes = []
+        L.magma_init_.argtypes = []
+        L.magma_init_.restype = None
+        L.magma_finalize_.argtypes = []
+        L.magma_finalize_.restype = None
+        # the caller's st_HACApK_leafmtxp goes as void* (either layout)
+        L.c_hacapk_adot_body_lfcpy_batch_sorted_.argtypes = [_I, vp]
         L.c_hacapk_adot_body_lfcpy_batch_sorted_.restype = None
-        L.c_hacapk_adot_body_lfmtx_batch_.argtypes = [_D, C.POINTER(RefLeafmtxp), _D, _D, _D, _D, _D]
+        L.c_hacapk_adot_body_lfmtx_batch_.argtypes = [_D, vp, _D, _D, _D, _D, _D]
         L.c_hacapk_adot_body_lfmtx_batch_.restype = None
-        L.c_hacapk_adot_body_lfdel_batch_.argtypes = [C.POINTER(RefLeafmtxp)]
+        L.c_hacapk_adot_body_lfdel_batch_.argtypes = [vp]
         L.c_hacapk_adot_body_lfdel_batch_.restype = None
         for n in ("c_hacapk_adot_body_lfmtx_", "c_hacapk_adot_body_lfmtx_gpu_"):
-            getattr(L, n).argtypes = [_D, C.POINTER(RefLeafmtxp), _D, _D]
+            getattr(L, n).argtypes = [_D, vp, _D, _D]
             getattr(L, n).restype = None
-        L.c_hacapk_adot_body_lfcpy_gpu_.argtypes = [_I, C.POINTER(RefLeafmtxp)]
+        L.c_hacapk_adot_body_lfcpy_gpu_.argtypes = [_I, vp]
         L.c_hacapk_adot_body_lfcpy_gpu_.restype = None
-        L.c_hacapk_adot_body_lfdel_gpu_.argtypes = [C.POINTER(RefLeafmtxp)]
+        L.c_hacapk_adot_body_lfdel_gpu_.argtypes = [vp]
         L.c_hacapk_adot_body_lfdel_gpu_.restype = None
-        L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes = [C.POINTER(RefLeafmtxp),
-                                                       C.POINTER(RefLcontrol), _D, _D, _D, _I, _I, _I]
+        L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes = [vp, C.POINTER(RefLcontrol), _D, _D, _D, _I,
+                                                       _I, _I]
         L.c_hacapk_bicgstab_cax_lfmtx_gpu_.restype = None
         L.c_hacapk_bicgstab_cax_lfmtx_pipe_.argtypes = L.c_hacapk_bicgstab_cax_lfmtx_gpu_.argtypes
         L.c_hacapk_bicgstab_cax_lfmtx_pipe_.restype = None
@@ -1008,10 +1034,16 @@ def bicgstab_ru(a64, alow, b, u0=None, eps=1e-8, maxit=3000, check_every=4, delt
 # ---------------------------------------------------------------------------
 # reference-named entry points
 # ---------------------------------------------------------------------------
-def make_leafmtxp(hm):
-    """st_HACApK_leafmtxp view of an HMatrix for the reference-named calls."""
+def make_leafmtxp(hm, layout="magma", fill=None):
+    """st_HACApK_leafmtxp view of an HMatrix for the reference-named calls, in
+    the layout of a HAVE_MAGMA build ("magma") or of a build without it
+    ("plain"); fill (magma layout): a byte value written over the whole MAGMA
+    block first, standing for whatever the caller's build keeps there."""
     t = hm.leaf_table()
-    lp = RefLeafmtxp()
+    lp = RefLeafmtxp() if layout == "magma" else RefLeafmtxpPlain()
+    if fill is not None:
+        assert layout == "magma"
+        C.memset(C.addressof(lp), fill, C.sizeof(lp))
     lp.nd = hm.nd
     lp.nlf = hm.nlf
     lp.nlfkt = int((t[:, 0] == 1).sum()) if len(t) else 0
@@ -1020,6 +1052,15 @@ def make_leafmtxp(hm):
     lp.st_lf = hm.leaves_ptr
     lp._owner = hm
     return lp
+
+
+def set_ref_layout(layout):
+    """hmv_ref_set_layout: the st_HACApK_leafmtxp layout the reference-named
+    entry points read ("magma" or "plain"); returns the previous one."""
+    prev = hmv_lib().hmv_ref_set_layout(REF_LAYOUT[layout])
+    if prev < 0:
+        _check(prev, "hmv_ref_set_layout")
+    return {v: k for k, v in REF_LAYOUT.items()}[prev]
 
 
 def adot_body_lfcpy_batch_sorted(nd, leafmtxp):

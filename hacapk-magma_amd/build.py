@@ -6,6 +6,8 @@ Products (all in-tree, git-ignored):
   oracle/liboracle.so                CPU oracle (test infrastructure)
   oracle/_ref/libhacapk_ref.so       reference entry-function sources compiled with
                                      flang (only where /root/reference exists)
+  oracle/_ref/ref_layout_*.txt       flang's layout of the reference's container types
+  oracle/_ref/ref_driver             Fortran caller of libhmv.so with those types
 
 Usage: python hacapk-magma_amd/build.py [--only hmv|hmvgen|oracle|ref]
 """
@@ -100,7 +102,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", choices=["hmv", "hmvgen", "oracle", "ref"], default=None)
     args = ap.parse_args(argv)
-    steps = {"hmvgen": build_hmvgen, "oracle": build_oracle, "ref": build_ref, "hmv": build_hmv}
+    # the reference step links oracle/_ref/ref_driver against libhmv.so: engine first
+    steps = {"hmvgen": build_hmvgen, "oracle": build_oracle, "hmv": build_hmv, "ref": build_ref}
     for name, fn in steps.items():
         if args.only and args.only != name:
             continue

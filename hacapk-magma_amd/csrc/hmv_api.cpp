@@ -3,7 +3,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cstddef>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -879,9 +882,67 @@ int hmv_plan_free(hmv_plan p) {
 // ---------------------------------------------------------------------------
 // Reference-named entry points.
 // ---------------------------------------------------------------------------
+static_assert(sizeof(hmv_leaf) == 48 && offsetof(hmv_leaf, a1size) == 24 &&
+                  offsetof(hmv_leaf, a1) == 32,
+              "stc_HACApK_leafmtx layout (HACApK_MAGMA.h:12-20)");
+static_assert(offsetof(hmv_ref_leafmtxp, st_lf_stride) == 16 &&
+                  offsetof(hmv_ref_leafmtxp, gflops) == 40 &&
+                  offsetof(hmv_ref_leafmtxp, gpu_vectors) == 48 &&
+                  offsetof(hmv_ref_leafmtxp, num_batch) == 96 &&
+                  offsetof(hmv_ref_leafmtxp, batch_tables) == 112 &&
+                  offsetof(hmv_ref_leafmtxp, num_streamed) == 272 &&
+                  offsetof(hmv_ref_leafmtxp, streamed_tables) == 280 &&
+                  offsetof(hmv_ref_leafmtxp, mgpu_tables) == 328 &&
+                  offsetof(hmv_ref_leafmtxp, mpi_rank) == 528 &&
+                  offsetof(hmv_ref_leafmtxp, st_lf) == 536 && sizeof(hmv_ref_leafmtxp) == 544,
+              "stc_HACApK_leafmtxp of a HAVE_MAGMA build (HACApK_MAGMA.h:28-106)");
+static_assert(offsetof(hmv_ref_leafmtxp_plain, st_lf) == 24 &&
+                  sizeof(hmv_ref_leafmtxp_plain) == 32,
+              "stc_HACApK_leafmtxp without HAVE_MAGMA (m_HACApK_base.f90:64-76,234)");
+
 namespace {
 std::mutex g_tab_mu;
 std::map<const void *, hmv_arena> g_tab;
+
+// Struct layout of the caller's build: -1 = not read yet (HMV_REF_LAYOUT
+// from the environment at the first call), else HMV_REF_LAYOUT_*.
+std::atomic<int> g_layout{-1};
+
+int ref_layout() {
+  int l = g_layout.load();
+  if (l < 0) {
+    const char *env = std::getenv("HMV_REF_LAYOUT");
+    int want = (env && std::strcmp(env, "plain") == 0) ? HMV_REF_LAYOUT_PLAIN : HMV_REF_LAYOUT_MAGMA;
+    g_layout.compare_exchange_strong(l, want);
+    l = g_layout.load();
+  }
+  return l;
+}
+
+// What the entry points read from the caller's st_HACApK_leafmtxp.
+struct RefView {
+  int nd, nlf, stride;
+  const void *st_lf;
+  int *mpi_rank;  // null in the plain layout
+};
+
+RefView ref_view(hmv_ref_leafmtxp *H) {
+  if (ref_layout() == HMV_REF_LAYOUT_PLAIN) {
+    auto *P = reinterpret_cast<hmv_ref_leafmtxp_plain *>(H);
+    return {P->nd, P->nlf, P->st_lf_stride, P->st_lf, nullptr};
+  }
+  return {H->nd, H->nlf, H->st_lf_stride, H->st_lf, &H->mpi_rank};
+}
+
+// FP64 arena of the caller's leaves (nd from the caller's argument where
+// the reference takes one); mpi_rank = 0, as the reference's lfcpy stores
+// the rank of its one process.
+int build_bound(hmv_ref_leafmtxp *H, int nd, hmv_arena *a) {
+  const RefView v = ref_view(H);
+  int rc = hmv_arena_build(nd, v.nlf, v.st_lf, v.stride, HMV_PREC_F64, a);
+  if (rc == HMV_OK && v.mpi_rank) *v.mpi_rank = 0;
+  return rc;
+}
 
 hmv_arena lookup(const void *key) {
   std::lock_guard<std::mutex> lk(g_tab_mu);
@@ -904,7 +965,7 @@ void apply_bound(const char *name, double *zau, hmv_ref_leafmtxp *H, double *zu)
   }
   hmv_arena a = lookup(H);
   if (!a) {
-    if (hmv_arena_build(H->nd, H->nlf, H->st_lf, H->st_lf_stride, HMV_PREC_F64, &a) != HMV_OK) {
+    if (build_bound(H, ref_view(H).nd, &a) != HMV_OK) {
       report(name);
       return;
     }
@@ -917,11 +978,29 @@ void apply_bound(const char *name, double *zau, hmv_ref_leafmtxp *H, double *zu)
 
 extern "C" {
 
+int hmv_ref_set_layout(int layout) {
+  if (layout != HMV_REF_LAYOUT_MAGMA && layout != HMV_REF_LAYOUT_PLAIN) {
+    set_error("hmv_ref_set_layout: unknown layout " + std::to_string(layout));
+    return HMV_ERR_ARG;
+  }
+  const int prev = ref_layout();
+  g_layout.store(layout);
+  return prev;
+}
+
+int hmv_ref_get_layout(void) { return ref_layout(); }
+
+void magma_init_(void) {
+  if (hmv_init(0) != HMV_OK) report("magma_init_");
+}
+
+void magma_finalize_(void) {}
+
 void c_hacapk_adot_body_lfcpy_batch_sorted_(int *nd, hmv_ref_leafmtxp *H) {
   if (!nd || !H) return;
   c_hacapk_adot_body_lfdel_batch_(H);
   hmv_arena a = nullptr;
-  if (hmv_arena_build(*nd, H->nlf, H->st_lf, H->st_lf_stride, HMV_PREC_F64, &a) != HMV_OK) {
+  if (build_bound(H, *nd, &a) != HMV_OK) {
     report("c_hacapk_adot_body_lfcpy_batch_sorted_");
     return;
   }
@@ -998,7 +1077,7 @@ void ref_solve(const char *name, const char *label, SolveFn solve, hmv_ref_leafm
   hmv_arena a = lookup(H);
   bool own = false;
   if (!a) {
-    if (hmv_arena_build(*nd, H->nlf, H->st_lf, H->st_lf_stride, HMV_PREC_F64, &a) != HMV_OK) {
+    if (build_bound(H, *nd, &a) != HMV_OK) {
       report(name);
       return;
     }

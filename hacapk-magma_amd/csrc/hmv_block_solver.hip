@@ -101,8 +101,11 @@ __device__ inline void col_step(ColState &c, double d0, double d1, const BlockSt
         c.alpha = 0.0;
       }
     } else if (stage == 2) {
-      c.zeta = d0 / d1;
-      if (d1 == 0.0 || !isfinite(c.zeta)) {
+      // s = A t = 0 (both dots zero): the half step u += alpha p, r = t is
+      // taken with zeta = 0; stage 3 reports convergence or breakdown
+      const bool half = d1 == 0.0 && d0 == 0.0;
+      c.zeta = half ? 0.0 : d0 / d1;
+      if (!half && (d1 == 0.0 || !isfinite(c.zeta))) {
         c.status = isfinite(d1) ? HMV_SOLVE_BREAKDOWN : HMV_SOLVE_NONFINITE;
         c.done = 1;
         c.zeta = 0.0;
@@ -121,7 +124,7 @@ __device__ inline void col_step(ColState &c, double d0, double d1, const BlockSt
       } else if (c.step >= bs->maxit) {
         c.status = HMV_SOLVE_MAXITER;
         c.done = 1;
-      } else if (c.rr == 0.0) {
+      } else if (c.rr == 0.0 || c.zeta == 0.0) {
         c.status = HMV_SOLVE_BREAKDOWN;
         c.done = 1;
       }

@@ -185,8 +185,11 @@ __global__ __launch_bounds__(kT) void p_final(int nb, const double *__restrict__
       st->done = 1;
     }
   } else if (stage == 1) {
-    st->zeta = d[0] / d[1];
-    if (!isfinite(st->zeta) || d[1] == 0.0) {
+    // both dots zero: the half step (zeta = 0) is taken and stage 2 reports
+    // convergence, or breakdown through beta = alpha / zeta
+    const bool half = d[0] == 0.0 && d[1] == 0.0;
+    st->zeta = half ? 0.0 : d[0] / d[1];
+    if (!half && (!isfinite(st->zeta) || d[1] == 0.0)) {
       st->status = isfinite(d[1]) ? HMV_SOLVE_BREAKDOWN : HMV_SOLVE_NONFINITE;
       st->done = 1;
     }

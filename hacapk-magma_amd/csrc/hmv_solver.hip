@@ -141,8 +141,11 @@ __global__ __launch_bounds__(kThreads) void k_final(int nb, const double *__rest
     }
   } else if (stage == 2) {
     const double znom = st->d0, zden = st->d1;
-    st->zeta = znom / zden;
-    if (zden == 0.0 || !finite(st->zeta)) {
+    // s = A t = 0 (both dots zero): t is the residual of u + alpha p; zeta = 0
+    // takes that half step and stage 3 decides (converged, else breakdown)
+    const bool half = zden == 0.0 && znom == 0.0;
+    st->zeta = half ? 0.0 : znom / zden;
+    if (!half && (zden == 0.0 || !finite(st->zeta))) {
       st->status = finite(zden) ? HMV_SOLVE_BREAKDOWN : HMV_SOLVE_NONFINITE;
       st->done = 1;
     }
@@ -181,7 +184,8 @@ __global__ __launch_bounds__(kThreads) void k_final(int nb, const double *__rest
     } else if (st->step >= st->maxit) {
       st->status = HMV_SOLVE_MAXITER;
       st->done = 1;
-    } else if (st->rr == 0.0) {
+    } else if (st->rr == 0.0 || st->zeta == 0.0) {
+      // zeta = 0: a half step that did not converge (beta = alpha / zeta fails)
       st->status = HMV_SOLVE_BREAKDOWN;
       st->done = 1;
     }
@@ -245,8 +249,11 @@ __global__ __launch_bounds__(kThreads) void k_update_ur(
     z1 += zpart[2 * i + 1];
   }
   const double znom = block_sum_all(z0, sh), zden = block_sum_all(z1, sh);
-  const double zeta = znom / zden;
-  const bool bad = zden == 0.0 || !finite(zeta);
+  // s = A t = 0 (both dots zero): the half step u += alpha p, r = t is taken
+  // with zeta = 0, and k_final's beta stage reports convergence or breakdown
+  const bool half = zden == 0.0 && znom == 0.0;
+  const double zeta = half ? 0.0 : znom / zden;
+  const bool bad = !half && (zden == 0.0 || !finite(zeta));
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->d0 = znom;
     st->d1 = zden;

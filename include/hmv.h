@@ -367,10 +367,45 @@ int hmv_arena_save(hmv_arena a, const char *path);      /* copies the device tab
 int hmv_arena_load(const char *path, hmv_arena *out);   /* hmv_plan_load + hmv_arena_from_plan */
 
 /* ---- reference-named entry points --------------------------------------
- * For a Fortran driver built WITHOUT HAVE_MAGMA: the C view of its
- * st_HACApK_leafmtxp is then the five integers followed by the st_lf
- * pointer (m_HACApK_base.f90:64-76 with the MAGMA block compiled out), and
- * the device state lives in a side table keyed by the struct's address. */
+ * The C view of the caller's st_HACApK_leafmtxp.  Every reference call
+ * site of these entry points but one is compiled only under HAVE_MAGMA or
+ * HAVE_MAGMA_BATCH (m_HACApK_use.f90:317-455, m_HACApK_solve.f90:97-103,
+ * 372-435, 652-741, 850-966), and those flags put the MAGMA block between
+ * st_lf_stride and st_lf (m_HACApK_base.f90:64-235, C mirror
+ * HACApK_MAGMA.h:28-106).  hmv_ref_leafmtxp is that layout (544 bytes,
+ * st_lf at byte 536, magma_int_t = int), the default.  A driver built
+ * without either flag reaches only c_hacapk_adot_body_lfmtx_
+ * (m_HACApK_solve.f90:102) and passes hmv_ref_leafmtxp_plain (st_lf at byte
+ * 24); select it with hmv_ref_set_layout(HMV_REF_LAYOUT_PLAIN) or the
+ * environment variable HMV_REF_LAYOUT=plain, read at the first call.
+ *
+ * The engine reads nd, nlf, st_lf_stride and st_lf (and each leaf through
+ * st_lf_stride and a1size, as HACApK_MAGMA.c:27 does), writes mpi_rank = 0
+ * when it builds an arena (the reference's lfcpy stores the MPI rank there,
+ * HACApK_BATCH_SORT.c:98, and m_HACApK_solve.f90:739 reads it), and leaves
+ * every other MAGMA field untouched.  Its device state lives in a side table
+ * keyed by the struct's address. */
+typedef struct {
+  int nd;
+  int nlf;
+  int nlfkt;
+  int ktmax;
+  int st_lf_stride;
+  /* MAGMA block, HACApK_MAGMA.h:34-102 (byte offsets in comments) */
+  int m, n, gn, max_block;        /* 20 .. 32                                   */
+  double gflops;                  /* 40                                         */
+  void *gpu_vectors[6];           /* 48: mtx1_gpu mtx2_gpu zu_gpu zau_gpu zau_pin zbu_gpu */
+  int num_batch, total_size_y, transA; /* 96 .. 104                             */
+  void *batch_tables[20];         /* 112: d_A_array .. d_inc, batch_order .. batch_size */
+  int num_streamed, num_streamed_t;    /* 272, 276                              */
+  void *streamed_tables[6];       /* 280: h_A_array_streamed .. h_lda_streamed  */
+  void *mgpu_tables[25];          /* 328: d_A_mgpu .. total_size_y_mgpu, iwork  */
+  int mpi_rank;                   /* 528                                        */
+  hmv_leaf *st_lf;                /* 536: first word of the Fortran descriptor  */
+} hmv_ref_leafmtxp;
+
+/* st_HACApK_leafmtxp of a build without HAVE_MAGMA / HAVE_MAGMA_BATCH
+ * (m_HACApK_base.f90:64-76 and :234). */
 typedef struct {
   int nd;
   int nlf;
@@ -378,7 +413,15 @@ typedef struct {
   int ktmax;
   int st_lf_stride;
   hmv_leaf *st_lf;
-} hmv_ref_leafmtxp;
+} hmv_ref_leafmtxp_plain;
+
+#define HMV_REF_LAYOUT_MAGMA 0  /* hmv_ref_leafmtxp (default)   */
+#define HMV_REF_LAYOUT_PLAIN 1  /* hmv_ref_leafmtxp_plain       */
+/* Selects the struct layout the reference-named entry points read; returns
+ * the previous layout, or HMV_ERR_ARG for an unknown value.  No reference
+ * counterpart (the reference fixes it at compile time). */
+int hmv_ref_set_layout(int layout);
+int hmv_ref_get_layout(void);
 
 typedef struct {
   int lf_umpi;
@@ -389,6 +432,14 @@ typedef struct {
   int lthr_offset;
   double *param;
 } hmv_ref_lcontrol;
+
+/* Replace magma_init_ / magma_finalize_ (HACApK_MAGMA.c:69-75), which a
+ * HAVE_MAGMA driver calls once around its run (m_HACApK_use.f90:61,74):
+ * magma_init_ selects device 0 of the process (hmv_init) and prints the
+ * error if there is none; magma_finalize_ does nothing (an arena bound by
+ * lfcpy lives until lfdel). */
+void magma_init_(void);
+void magma_finalize_(void);
 
 /* Replaces c_hacapk_adot_body_lfcpy_batch_sorted_ (HACApK_BATCH_SORT.c:86). */
 void c_hacapk_adot_body_lfcpy_batch_sorted_(int *nd, hmv_ref_leafmtxp *st_leafmtxp);

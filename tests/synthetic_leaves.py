@@ -62,3 +62,62 @@ def many_runs(H, seed=0):
     lv += [("dense", 48, 64, 256, 64, 0)]
     lv += [("lowrank", 112, 400, 256, 256, 3)]
     return LeafSet(H, nd, lv, seed)
+
+
+class DenseBlocks:
+    """nd unknowns covered by dense leaves of given values: blocks is a list
+    of (row0, col0, M), 0-based, M an (ndl, ndt) array; a1 holds M row-major
+    (ndt x ndl column-major, as the reference stores a dense leaf)."""
+
+    def __init__(self, H, nd, blocks):
+        self.nd = nd
+        self.nlf = len(blocks)
+        self._rec = (H.LeafRec * self.nlf)()
+        self._bufs = []
+        self._table = []
+        for i, (r0, c0, M) in enumerate(blocks):
+            M = np.asarray(M, np.float64)
+            nr, nc = M.shape
+            assert 0 <= r0 and r0 + nr <= nd and 0 <= c0 and c0 + nc <= nd
+            buf = np.ascontiguousarray(M.reshape(-1))
+            rec = self._rec[i]
+            rec.ltmtx, rec.kt = 2, 0
+            rec.nstrtl, rec.ndl, rec.nstrtt, rec.ndt = r0 + 1, nr, c0 + 1, nc
+            rec.a1size = buf.nbytes
+            rec.a1 = buf.ctypes.data_as(C.POINTER(C.c_double))
+            self._bufs.append(buf)
+            self._table.append([2, 0, r0 + 1, nr, c0 + 1, nc])
+        self.leaves_ptr = C.addressof(self._rec)
+        self.stride = C.sizeof(H.LeafRec)
+        self.dense = np.zeros((nd, nd))
+        for r0, c0, M in blocks:
+            M = np.asarray(M, np.float64)
+            self.dense[r0:r0 + M.shape[0], c0:c0 + M.shape[1]] += M
+
+    def leaf_table(self):
+        return np.array(self._table, dtype=np.int64).reshape(-1, 6)
+
+
+SKEW = [[0.0, 1.0], [-1.0, 0.0]]  # (b, A b) = 0 for every b: r^.v = 0 at step 1
+
+
+def skew_system(H):
+    """2 x 2 skew-symmetric: BiCGSTAB's alpha = (r^,r)/(r^,A p) divides by 0."""
+    return DenseBlocks(H, 2, [(0, 0, SKEW)])
+
+
+def skew_and_scaled_identity(H):
+    """4 x 4 block diagonal [SKEW, 2 I]: right-hand sides on the first two
+    unknowns break down, on the last two the first half step solves
+    exactly (t = r - alpha A p = 0)."""
+    return DenseBlocks(H, 4, [(0, 0, SKEW), (2, 2, [[2.0, 0.0], [0.0, 2.0]])])
+
+
+def scaled_identity(H):
+    """2 I on two unknowns: the first half step of BiCGSTAB solves exactly."""
+    return DenseBlocks(H, 2, [(0, 0, [[2.0, 0.0], [0.0, 2.0]])])
+
+
+def with_nan(H, value=float("nan")):
+    """[[2, value], [0, 3]]: a leaf holding a non-finite entry."""
+    return DenseBlocks(H, 2, [(0, 0, [[2.0, value], [0.0, 3.0]])])
