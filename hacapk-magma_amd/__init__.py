@@ -236,6 +236,7 @@ def hmv_lib():
                                             C.POINTER(Tuning), C.POINTER(vp)]
         L.hmv_default_tuning.argtypes = [C.c_int, C.POINTER(Tuning)]
         L.hmv_apply_block.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp]
+        L.hmv_apply_block_static.argtypes = [vp, C.c_int, vp, vp, C.c_double, vp]
         L.hmv_bicgstab_ru.argtypes = [vp, vp, vp, vp, C.c_double, C.c_int, C.c_int, C.c_double,
                                       C.POINTER(SolveInfo)]
         L.hmv_arena_build_multi.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),

@@ -350,7 +350,10 @@ int hmv_init(int device) {
   if (rc) return rc;
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return hip_fail(e, "hmv_init");
-  return HMV_OK;
+  // the static workspace's device address, looked up once per device here
+  // (hmv_apply_block_static looks it up itself if hmv_init was not called)
+  double *scratch = nullptr;
+  return hmv::static_scratch(&scratch);
 }
 
 int hmv_arena_build(int nd, int nlf, const void *leaves, int leaf_stride, int precision,

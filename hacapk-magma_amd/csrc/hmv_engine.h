@@ -78,6 +78,8 @@ int build_from_device_leaves(int nd, int nlf, const void *leaves, int stride, in
 bool download_plan(const hmv_arena_s *a, Plan &P, std::string &err);
 // Makes the H-matmat t buffer hold k right-hand sides.
 int ensure_tk(hmv_arena_s *a, int k);
+// Device address of the current device's static workspace (hmv_matmat.hip).
+int static_scratch(double **out);
 }  // namespace hmv
 
 #define HMV_MFMA_RATE_WAVES (256 * 4 * 4)  // waves of the MFMA-rate calibration

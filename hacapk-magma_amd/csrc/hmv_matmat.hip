@@ -21,6 +21,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <mutex>
+#include <string>
+
 #include "hmv_engine.h"
 #include "hmv_inputs.h"
 #include "hmv_internal.h"
@@ -421,4 +425,86 @@ extern "C" hipError_t hmv_launch_block(const HmvLaunch *L, const HmvUnits *units
   else
     launch_block<float, HMV_DT_BF16, HMV_DT_BF16>(*L, *units, k, x, t, y, beta, skip, stream);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Static workspace of hmv_apply_block_static: the H-matmat's t buffer
+// (tpart_size * k doubles) in a file-scope device array instead of memory
+// allocated on first use.  Every device the library runs on holds its own
+// copy until the process ends; host code never passes `scratch` itself (on
+// the host it names a stand-in, not device memory): the device address of
+// each device's copy comes from hipGetSymbolAddress, taken once per device
+// with that device current, and only that address reaches the kernels.
+// Nothing in it needs clearing between calls: phase 1 writes every t slot
+// and chunk partial that the finalize and phase 2 read.  Calls that overlap
+// on one device share the array and race.
+static __device__ __align__(256) double scratch[HMV_STATIC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+namespace hmv {
+// Device address of the current device's copy of `scratch`, looked up on
+// the first call per device.
+int static_scratch(double **out) {
+  *out = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "static scratch: hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("static scratch: device index " + std::to_string(dev) + " out of range");
+    return HMV_ERR_DEVICE;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "static scratch: hipGetSymbolAddress");
+    if (!p) {
+      set_error("static scratch: no device address");
+      return HMV_ERR_DEVICE;
+    }
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch[dev];
+  return HMV_OK;
+}
+}  // namespace hmv
+
+extern "C" int hmv_apply_block_static(hmv_arena a, int k, const double *x, double *y, double beta,
+                                      void *stream) {
+  if (!a || !x || !y || k < 1 || k > HMV_MAX_RHS) {
+    hmv::set_error("hmv_apply_block_static: invalid argument (k must be 1..128)");
+    return HMV_ERR_ARG;
+  }
+  const int64_t need = std::max<int64_t>(1, a->tpart_size) * k * (int64_t)sizeof(double);
+  if (need > (int64_t)HMV_STATIC_SCRATCH_BYTES) {
+    hmv::set_error("hmv_apply_block_static: the product needs " + std::to_string(need) +
+                   " bytes of t buffer (k = " + std::to_string(k) +
+                   "), the static scratch holds " + std::to_string(HMV_STATIC_SCRATCH_BYTES));
+    return HMV_ERR_ARG;
+  }
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hmv::hip_fail(e, "hmv_apply_block_static");
+  if (dev != a->device) {
+    hmv::set_error("hmv_apply_block_static: the arena lives on device " +
+                   std::to_string(a->device) + ", the current device is " + std::to_string(dev));
+    return HMV_ERR_ARG;
+  }
+  double *t = nullptr;
+  int rc = hmv::static_scratch(&t);
+  if (rc) return rc;
+  HmvUnits u;
+  for (int p = 0; p < 2; ++p)
+    for (int w = 0; w < HMV_MM_WIDTHS; ++w) {
+      u.u[p][w] = a->d_units[p][w];
+      u.n[p][w] = a->nunits[p][w];
+    }
+  e = hmv_launch_block(&a->launch, &u, k, x, t, y, beta, nullptr, (hipStream_t)stream);
+  if (e != hipSuccess) return hmv::hip_fail(e, "hmv_apply_block_static: launch");
+  return HMV_OK;
 }

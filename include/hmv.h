@@ -144,6 +144,19 @@ int hmv_apply(hmv_arena a, const double *x, double *y, double beta, void *stream
  * counterpart (the reference applies one vector per call). */
 #define HMV_MAX_RHS 128
 int hmv_apply_block(hmv_arena a, int k, const double *x, double *y, double beta, void *stream);
+/* H-matmat that allocates nothing: as hmv_apply_block (same kernels and
+ * result), with the product's t buffer in a static device workspace of
+ * HMV_STATIC_SCRATCH_BYTES per device instead of memory hmv_apply_block
+ * allocates on its first call with a larger k.  No reference counterpart:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops).  Needs max(1, tparts) * k * 8 bytes of t buffer
+ * (tparts from hmv_arena_stats_get); a larger product fails with
+ * HMV_ERR_ARG.  The arena's device must be current.  Each device holds its
+ * workspace until the process ends; every call on a device shares it, so
+ * calls that overlap on one device (other streams or threads) race. */
+#define HMV_STATIC_SCRATCH_BYTES ((size_t)256 << 20)
+int hmv_apply_block_static(hmv_arena a, int k, const double *x, double *y, double beta,
+                           void *stream);
 /* Average time of `reps` H-matmats (HIP events on an internal stream). */
 int hmv_apply_block_timed(hmv_arena a, int k, const double *x, double *y, int reps, double *ms);
 
