@@ -13,12 +13,16 @@ the arena resident in HBM.  value = algorithmic bytes per matvec
 
 Also reported: per-phase HIP-event timings and the roofline of the dominant
 kernel, BiCGSTAB time-to-1e-8 in FP64 and with FP32 / BF16 storage under
-FP64 iterative refinement, the CPU baseline (the oracle's OpenMP per-leaf
-GEMV restatement of the reference CPU path, timed on this host), and the
-leaf-rank / leaf-size histograms.  Unless --no-extras: on one icosphere of
-327,680 unknowns the per-precision apply rates, FP64 vs mixed-precision
-refined solves, the multi-RHS H-matmat (k = 8, 32, 128) and a 32-RHS block
-BiCGSTAB; and a batch of 48 uneven instances solved in one arena.
+FP64 iterative refinement (each solution's error against the oracle's FP64
+solution), the CPU baseline (the oracle's restatement of the reference CPU
+path, per-leaf GEMV, timed on this host's cores at the thread count of its
+CPU share and at 1 thread, and its BiCGSTAB to 1e-8), and the leaf-rank /
+leaf-size histograms.  Unless --no-extras: on one icosphere of 327,680
+unknowns the per-precision apply rates and errors, FP64 vs mixed-precision
+refined solves, the multi-RHS H-matmat (k = 8, 32, 128) and block BiCGSTAB
+(FP64, and FP32 factors refined in FP64) at k = 8, 32, 128; and a batch of 48
+uneven instances in one arena, solved in FP64 and with autotuned BF16
+factors under per-instance FP64 refinement.
 """
 import argparse
 import json
@@ -52,6 +56,86 @@ def load_traffic(kernel_phase):
     return None
 
 
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+
+
+def cpu_threads():
+    """Threads of this process's CPU share: OMP_NUM_THREADS when set (the
+    share a GPU box gives one GPU), else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cond_bound(nd):
+    """cond * 2e-8, cond = 2 sqrt(N) + 1: the sphere single-layer operator's
+    eigenvalues are R/(2l+1) over (L+1)^2 ~ N modes, so two solutions whose
+    relative residuals are below 1e-8 may differ by up to this much."""
+    return (2.0 * np.sqrt(nd) + 1.0) * 2e-8
+
+
+def rel_err(u, u_ref):
+    return float(np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref))
+
+
+def timed_median(fn, runs=20, warm=3):
+    """Median wall time (s) of `runs` calls after `warm` untimed ones."""
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline_leg(O, hm, x, b, alg_bytes, log):
+    """The reference CPU path restated by the oracle, on this host: the
+    per-leaf GEMV apply (m_HACApK_solve.f90:203-253 with HACApK_MAGMA.c:114-199)
+    at the CPU share's thread count and at 1 thread (the reference's own
+    setting: OpenMP stubbed to one thread, sequential MKL), median of 20
+    after 3 warm-ups; and its BiCGSTAB (m_HACApK_solve.f90:342-449) from 0
+    to 1e-8 at the full thread count.  Returns (record, FP64 oracle
+    solution)."""
+    nthr = cpu_threads()
+    t_all = timed_median(lambda: O.adot_lfmtx_hyp(hm, x, nthr))
+    t_one = timed_median(lambda: O.adot_lfmtx_hyp(hm, x, 1))
+    t0 = time.perf_counter()
+    u_ref, nstp, hist = O.bicgstab(hm, b, None, mstep=3000, eps=1e-8, nthr=nthr)
+    t_solve = time.perf_counter() - t0
+    tr = rel_err(O.adot_lfmtx_hyp(hm, u_ref, nthr), b)
+    rec = {"value": round(alg_bytes / t_all / 1e9, 2), "unit": "GB/s", "cores": nthr, "kind": "port",
+           "sample": "median of 20 FP64 H-matvecs after 3 warm-ups of the same %d-unknown H-matrix "
+                     "(oracle per-leaf GEMV over storage-balanced leaf ranges, %d threads); the same "
+                     "at 1 thread; one BiCGSTAB to 1e-8" % (hm.nd, nthr),
+           "ms_per_matvec": round(1e3 * t_all, 3),
+           "one_thread": {"GBs": round(alg_bytes / t_one / 1e9, 2), "ms_per_matvec": round(1e3 * t_one, 3)},
+           "bicgstab": {"seconds": round(t_solve, 3), "iterations": int(nstp),
+                        "relres": float(hist[-1]), "true_relres": tr, "threads": nthr},
+           "cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
+           "threads_note": "threads = this process's CPU share (OMP_NUM_THREADS or CPU affinity)"}
+    log("CPU baseline (%s): %d threads %.1f GB/s (%.2f ms/matvec), 1 thread %.2f GB/s (%.1f ms); "
+        "BiCGSTAB %d it in %.2f s, true relres %.2e"
+        % (rec["cpu_model"], nthr, alg_bytes / t_all / 1e9, 1e3 * t_all, alg_bytes / t_one / 1e9,
+           1e3 * t_one, nstp, t_solve, tr))
+    return rec, u_ref
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,7 +143,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--subdiv", type=int, default=6, help="icosphere subdivisions per sphere")
     ap.add_argument("--quick", action="store_true", help="matvec only (for profiler runs)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline sample budget")
     ap.add_argument("--hmatmat", action="store_true",
                     help="only the FP64 H-matmat (k = 8, 32, 128) on 327,680 unknowns (for profiler runs)")
     ap.add_argument("--no-extras", action="store_true",
@@ -141,6 +224,9 @@ def main():
     traffic = load_traffic(dom)
     roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/pmc_traffic_two_spheres_163840.txt (rocprofv3 --pmc "
+                                  "FETCH_SIZE and WRITE_SIZE of this kernel on this workload, scaled "
+                                  "by the calibration kernels' counts; not measured in this run)",
                 "alg_bytes_per_launch": dbytes, "avg_launch_ms": round(dms, 5),
                 "phases_ms": {"phase1": round(ms1, 5), "phase2": round(ms2, 5), "apply": round(msa, 5)},
                 "apply_event_GBs": round(alg_bytes / (msa * 1e-3) / 1e9, 1)}
@@ -151,7 +237,7 @@ def main():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     y = dy.download()
-    nthr = max(1, min(16, os.cpu_count() or 1))
+    nthr = cpu_threads()
     y_ref = O.adot_lfmtx_hyp(hm, x, nthr)
     parity = float(np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref))
     log("parity vs oracle: %.3e" % parity)
@@ -194,13 +280,18 @@ def main():
     }
 
     if not args.quick:
-        # ---- BiCGSTAB time to 1e-8: FP64 vs mixed precision -------------------
+        # ---- CPU baseline: bounded sample on this host (and the oracle's
+        # FP64 solution, against which every GPU solve's error is reported)
         b = H.uniform(1, nd)
-        solves = {}
+        result["cpu_baseline"], u_oracle = cpu_baseline_leg(O, hm, x, b, alg_bytes, log)
+
+        # ---- BiCGSTAB time to 1e-8: FP64 vs mixed precision -------------------
+        solves = {"oracle_solution_bound": cond_bound(nd)}
         u64, i64, h64 = a64.bicgstab(b, eps=1e-8, maxit=2000, check_every=8)   # warm (graph capture)
         u64, i64, h64 = a64.bicgstab(b, eps=1e-8, maxit=2000, check_every=8)
         solves["fp64"] = {"seconds": round(i64["seconds"], 4), "iterations": i64["nstp"],
-                          "true_relres": i64["true_relres"], "status": i64["status_name"]}
+                          "true_relres": i64["true_relres"], "status": i64["status_name"],
+                          "rel_err_vs_oracle_solution": rel_err(u64, u_oracle)}
         log("BiCGSTAB FP64: %d it, %.3f s, true relres %.2e" % (i64["nstp"], i64["seconds"], i64["true_relres"]))
         for prec, ie in (("f32", 1e-4), ("bf16", 1e-2)):
             alow = H.Arena(hm, prec)
@@ -213,6 +304,7 @@ def main():
                 "true_relres": inf["relres"], "status": inf["status_name"],
                 "rel_err_vs_fp64_solution": err, "outer_residuals": [float(v) for v in hist],
                 "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3),
+                "rel_err_vs_oracle_solution": rel_err(u, u_oracle),
                 "arena_bytes": int(alow.stats["arena_bytes"])}
             log("BiCGSTAB-IR %s: outer %d inner %d, %.3f s (%.2fx FP64), relres %.2e, err vs FP64 sol %.2e"
                 % (prec, inf["outer"], inf["inner_total"], inf["seconds"], inf["seconds"] / i64["seconds"],
@@ -227,7 +319,8 @@ def main():
                     "inner_products": inf["inner_total"], "true_relres": inf["relres"],
                     "status": inf["status_name"], "rel_err_vs_fp64_solution": err,
                     "outer_residuals": [float(v) for v in hist],
-                    "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3)}
+                    "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3),
+                    "rel_err_vs_oracle_solution": rel_err(u, u_oracle)}
                 log("GCR(32)-IR bf16: outer %d, %d inner products, %.3f s (%.2fx FP64), relres %.2e, "
                     "err vs FP64 sol %.2e" % (inf["outer"], inf["inner_total"], inf["seconds"],
                                               inf["seconds"] / i64["seconds"], inf["relres"], err))
@@ -243,28 +336,15 @@ def main():
             "status": inf["status_name"], "rel_err_vs_fp64_solution": err,
             "outer_residuals": [float(v) for v in hist],
             "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3),
+            "rel_err_vs_oracle_solution": rel_err(u, u_oracle),
             "arena_bytes": int(alow.stats["arena_bytes"])}
         log("GCR(32)-IR bf16_all: outer %d, %d inner products, %.3f s (%.2fx FP64), relres %.2e, "
             "err vs FP64 sol %.2e" % (inf["outer"], inf["inner_total"], inf["seconds"],
                                       inf["seconds"] / i64["seconds"], inf["relres"], err))
         alow.free()
+        solves["cpu_time_ratio_vs_fp64_gpu"] = round(
+            result["cpu_baseline"]["bicgstab"]["seconds"] / i64["seconds"], 1)
         result["bicgstab"] = solves
-
-        # ---- CPU baseline: bounded sample on this host ------------------------
-        O.adot_lfmtx_hyp(hm, x, nthr)
-        n, t0 = 0, time.perf_counter()
-        while True:
-            O.adot_lfmtx_hyp(hm, x, nthr)
-            n += 1
-            el = time.perf_counter() - t0
-            if el > args.cpu_seconds or n >= 400:
-                break
-        cpu_gbs = alg_bytes * n / el / 1e9
-        result["cpu_baseline"] = {"value": round(cpu_gbs, 2), "unit": "GB/s", "cores": nthr, "kind": "port",
-                                  "sample": "%d FP64 H-matvecs of the same %d-unknown H-matrix "
-                                            "(oracle OpenMP per-leaf GEMV, %.1f s)" % (n, nd, el),
-                                  "ms_per_matvec": round(1e3 * el / n, 3)}
-        log("CPU baseline: %d threads, %.1f GB/s (%.2f ms/matvec)" % (nthr, cpu_gbs, 1e3 * el / n))
 
         # ---- leaf histograms ---------------------------------------------------
         result["histograms"] = {
@@ -276,52 +356,67 @@ def main():
         result["cpu_baseline"] = None
 
     if not args.quick and not args.no_extras:
-        result["sphere_327680"] = study_327680(H, log)
-        result["instances_48"] = study_instances(H, log)
+        result["sphere_327680"] = study_327680(H, O, log)
+        result["instances_48"] = study_instances(H, O, log)
 
     print(json.dumps(result), flush=True)
 
 
-def study_327680(H, log):
-    """One icosphere s=7 (327,680 unknowns), ACA+ 1e-5: H-matvec rate per
-    storage precision, BiCGSTAB to 1e-8 in FP64 vs FP64 iterative refinement
-    with FP32 / BF16 inner solves (RHS = 1 and U[-1,1] seed 2), and the
-    multi-RHS H-matmat and block BiCGSTAB (k RHS U[-1,1] seed 3)."""
+def study_327680(H, O, log):
+    """One icosphere s=7 (327,680 unknowns), ACA+ 1e-5: H-matvec rate and
+    error per storage precision, BiCGSTAB to 1e-8 in FP64 vs FP64 iterative
+    refinement with FP32 / BF16 inner solves (RHS = 1 and U[-1,1] seed 2,
+    errors against the oracle's FP64 solution), the multi-RHS H-matmat and
+    block BiCGSTAB in FP64 and with FP32 factors refined in FP64 (k RHS
+    U[-1,1] seed 3)."""
     out = {}
+    nthr = cpu_threads()
     t0 = time.perf_counter()
     hm = H.HMatrix(*H.icosphere(7), H.baseline_params(aca_eps=1e-5))
     nd = hm.nd
     out["construction_s"] = round(time.perf_counter() - t0, 2)
     arenas = {p: H.Arena(hm, p) for p in ("f64", "f32", "bf16", "bf16_all")}
-    x = H.DeviceBuffer.from_array(H.uniform(1, nd))
+    xh = H.uniform(1, nd)
+    x = H.DeviceBuffer.from_array(xh)
     y = H.DeviceBuffer.from_array(np.zeros(nd))
+    y_ref = O.adot_lfmtx_hyp(hm, xh, nthr)
     rates = {}
     for p, a in arenas.items():
         m1, m2, ma = a.apply_timed(x.ptr, y.ptr, 30)
+        err = rel_err(y.download(), y_ref)
         rates[p] = {"ms": round(ma, 4), "storage_alg_GBs": round(a.stats["alg_bytes"] / ma / 1e6, 1),
-                    "arena_GB": round(a.stats["arena_bytes"] / 1e9, 3), "tuning": a.tuning}
-        log("s=7 %s apply %.4f ms, %.1f GB/s at storage precision" % (p, ma, rates[p]["storage_alg_GBs"]))
+                    "phase1_ms": round(m1, 4), "phase2_ms": round(m2, 4),
+                    "arena_GB": round(a.stats["arena_bytes"] / 1e9, 3), "tuning": a.tuning,
+                    "rel_err_vs_fp64_oracle": err}
+        log("s=7 %s apply %.4f ms (p1 %.4f, p2 %.4f), %.1f GB/s at storage precision, rel err vs FP64 "
+            "oracle %.2e" % (p, ma, m1, m2, rates[p]["storage_alg_GBs"], err))
     out["apply"] = rates
-    solves = {}
+    solves = {"oracle_solution_bound": cond_bound(nd)}
     inner = {"f32": 1e-4, "bf16": 1e-2, "bf16_all": 1e-2}
     for rname, b in (("ones", np.ones(nd)), ("uniform_seed2", H.uniform(2, nd))):
+        t_or = time.perf_counter()
+        u_oracle, n_or, _ = O.bicgstab(hm, b, None, mstep=3000, eps=1e-8, nthr=nthr)
+        t_or = time.perf_counter() - t_or
         a64 = arenas["f64"]
         a64.bicgstab(b, eps=1e-8, maxit=3000, check_every=8)
         u64, i64, _ = a64.bicgstab(b, eps=1e-8, maxit=3000, check_every=8)
-        row = {"fp64": {"seconds": round(i64["seconds"], 4), "iterations": i64["nstp"],
-                        "true_relres": i64["true_relres"]}}
+        row = {"oracle_cpu": {"seconds": round(t_or, 3), "iterations": int(n_or), "threads": nthr},
+               "fp64": {"seconds": round(i64["seconds"], 4), "iterations": i64["nstp"],
+                        "true_relres": i64["true_relres"],
+                        "rel_err_vs_oracle_solution": rel_err(u64, u_oracle)}}
         a64.gcrm(b, eps=1e-8, maxit=3000, mreset=8, check_every=8, history=False)
-        _, ig, _ = a64.gcrm(b, eps=1e-8, maxit=3000, mreset=8, check_every=8, history=False)
+        ug, ig, _ = a64.gcrm(b, eps=1e-8, maxit=3000, mreset=8, check_every=8, history=False)
         row["gcrm8_fp64"] = {"seconds": round(ig["seconds"], 4), "iterations": ig["nstp"],
-                             "true_relres": ig["true_relres"], "status": ig["status_name"]}
-        log("s=7 %s: GCR(8) FP64 %.4f s (%d it), relres %.2e"
-            % (rname, ig["seconds"], ig["nstp"], ig["true_relres"]))
+                             "true_relres": ig["true_relres"], "status": ig["status_name"],
+                             "rel_err_vs_oracle_solution": rel_err(ug, u_oracle)}
+        log("s=7 %s: oracle BiCGSTAB %d it in %.2f s (%d threads); FP64 BiCGSTAB %d it %.4f s; GCR(8) "
+            "FP64 %.4f s (%d it)" % (rname, n_or, t_or, nthr, i64["nstp"], i64["seconds"],
+                                     ig["seconds"], ig["nstp"]))
         a64.bicgstab_pipe(b, eps=1e-8, maxit=3000, check_every=8, history=False)
-        _, ip, _ = a64.bicgstab_pipe(b, eps=1e-8, maxit=3000, check_every=8, history=False)
+        up, ip, _ = a64.bicgstab_pipe(b, eps=1e-8, maxit=3000, check_every=8, history=False)
         row["pipelined_bicgstab_fp64"] = {"seconds": round(ip["seconds"], 4), "iterations": ip["nstp"],
-                                          "true_relres": ip["true_relres"], "status": ip["status_name"]}
-        log("s=7 %s: pipelined BiCGSTAB FP64 %.4f s (%d it), relres %.2e"
-            % (rname, ip["seconds"], ip["nstp"], ip["true_relres"]))
+                                          "true_relres": ip["true_relres"], "status": ip["status_name"],
+                                          "rel_err_vs_oracle_solution": rel_err(up, u_oracle)}
         for p in ("f32", "bf16", "bf16_all"):
             H.bicgstab_ir(a64, arenas[p], b, eps=1e-8, inner_eps=inner[p], max_outer=60, max_inner=3000)
             u, inf, hist = H.bicgstab_ir(a64, arenas[p], b, eps=1e-8, inner_eps=inner[p], max_outer=60,
@@ -330,11 +425,12 @@ def study_327680(H, log):
                 "seconds": round(inf["seconds"], 4), "inner_eps": inner[p], "outer": inf["outer"],
                 "inner_iterations": inf["inner_total"], "true_relres": inf["relres"],
                 "status": inf["status_name"], "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3),
-                "rel_err_vs_fp64_solution": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                "rel_err_vs_oracle_solution": rel_err(u, u_oracle),
+                "rel_err_vs_fp64_gpu_solution": rel_err(u, u64),
                 "outer_residuals": [float(v) for v in hist]}
-            log("s=7 %s: FP64 %.4f s (%d it) vs %s-IR %.4f s (%.2fx), relres %.2e"
-                % (rname, i64["seconds"], i64["nstp"], p, inf["seconds"], inf["seconds"] / i64["seconds"],
-                   inf["relres"]))
+            log("s=7 %s: FP64 %.4f s vs %s BiCGSTAB-IR (inner %.0e) %.4f s (%.2fx), relres %.2e, err vs "
+                "oracle %.1e" % (rname, i64["seconds"], p, inner[p], inf["seconds"],
+                                 inf["seconds"] / i64["seconds"], inf["relres"], rel_err(u, u_oracle)))
             if p == "f32":
                 continue
             H.gcr_ir(a64, arenas[p], b, eps=1e-8, inner_eps=5e-3, m=32)
@@ -343,44 +439,69 @@ def study_327680(H, log):
                 "seconds": round(inf["seconds"], 4), "inner_eps": 5e-3, "outer": inf["outer"],
                 "inner_products": inf["inner_total"], "true_relres": inf["relres"],
                 "status": inf["status_name"], "time_ratio_vs_fp64": round(inf["seconds"] / i64["seconds"], 3),
-                "rel_err_vs_fp64_solution": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                "rel_err_vs_oracle_solution": rel_err(u, u_oracle),
+                "rel_err_vs_fp64_gpu_solution": rel_err(u, u64),
                 "outer_residuals": [float(v) for v in hist]}
             log("s=7 %s: FP64 %.4f s vs %s GCR(32)-IR %.4f s (%.2fx), %d outer, %d inner products, "
-                "relres %.2e" % (rname, i64["seconds"], p, inf["seconds"], inf["seconds"] / i64["seconds"],
-                                 inf["outer"], inf["inner_total"], inf["relres"]))
+                "relres %.2e, err vs oracle %.1e"
+                % (rname, i64["seconds"], p, inf["seconds"], inf["seconds"] / i64["seconds"],
+                   inf["outer"], inf["inner_total"], inf["relres"], rel_err(u, u_oracle)))
         solves[rname] = row
     out["bicgstab"] = solves
-    # multi-RHS
+    # multi-RHS H-matmat
     a64 = arenas["f64"]
     st = a64.stats
     E = st["entries_lr"] + st["entries_dense"]
     single_ms = rates["f64"]["ms"]
-    mfma_tf = H.mfma_rate()  # measured FP64 MFMA ceiling
-    mm = {"mfma_f64_TFLOPs_measured": round(mfma_tf, 1)}
+    mfma_tf = H.mfma_rate()  # measured FP64 MFMA ceiling on register operands
+    mm = {"mfma_f64_TFLOPs_measured": round(mfma_tf, 1), "fp64_peak_TFLOPs": FP64_PEAK_TFLOPS}
     rng = np.random.default_rng(3)
     for k in (8, 32, 128):
         X = H.DeviceBuffer.from_array(rng.uniform(-1, 1, nd * k))
         Y = H.DeviceBuffer.from_array(np.zeros(nd * k))
         ms = a64.apply_block_timed(X.ptr, Y.ptr, k, 5)
         tf = 2.0 * E * k / ms / 1e9
+        hbm_ms = (8.0 * E + 16.0 * nd * k) / (HBM_PEAK_GBS * 1e6)
+        mfma_ms = 2.0 * E * k / (FP64_PEAK_TFLOPS * 1e9)
+        floor = max(hbm_ms, mfma_ms)
         mm[str(k)] = {"ms": round(ms, 4), "ms_per_rhs": round(ms / k, 4),
                       "speedup_per_rhs_vs_single": round(single_ms * k / ms, 2),
-                      "TFLOPs": round(tf, 2), "frac_of_measured_mfma": round(tf / mfma_tf, 3)}
-        log("s=7 H-matmat k=%d: %.3f ms (%.2f ms/RHS, %.1fx single, %.1f TFLOP/s = %.0f%% of the "
-            "measured FP64 MFMA rate %.1f)" % (k, ms, ms / k, single_ms * k / ms, tf,
-                                               100 * tf / mfma_tf, mfma_tf))
+                      "TFLOPs": round(tf, 2), "frac_of_fp64_peak": round(tf / FP64_PEAK_TFLOPS, 3),
+                      "frac_of_measured_mfma": round(tf / mfma_tf, 3),
+                      "floor_ms": round(floor, 4), "bound": "hbm" if hbm_ms >= mfma_ms else "mfma",
+                      "roofline_frac": round(floor / ms, 3)}
+        log("s=7 H-matmat k=%d: %.3f ms (%.3f ms/RHS, %.1fx single), %.1f TFLOP/s = %.0f%% of the FP64 "
+            "peak; floor %.3f ms (%s) -> %.0f%% of roofline"
+            % (k, ms, ms / k, single_ms * k / ms, tf, 100 * tf / FP64_PEAK_TFLOPS, floor,
+               mm[str(k)]["bound"], 100 * floor / ms))
         X.free()
         Y.free()
     out["hmatmat_fp64"] = mm
-    B = rng.uniform(-1, 1, (nd, 32))
-    a64.block_bicgstab(B, eps=1e-8, maxit=3000)
-    U, cols, inf = a64.block_bicgstab(B, eps=1e-8, maxit=3000, check_every=8)
-    out["block_bicgstab_k32"] = {
-        "seconds": round(inf["seconds"], 4), "iterations": inf["nstp"], "status": inf["status_name"],
-        "worst_true_relres": inf["true_relres"], "rhs_solves_per_s": round(32 / inf["seconds"], 2),
-        "single_rhs_solves_per_s": round(1.0 / solves["uniform_seed2"]["fp64"]["seconds"], 2)}
-    log("s=7 block BiCGSTAB k=32: %.3f s, %.1f RHS-solves/s (single: %.1f)"
-        % (inf["seconds"], 32 / inf["seconds"], 1.0 / solves["uniform_seed2"]["fp64"]["seconds"]))
+    # block BiCGSTAB: FP64, and FP32 factors refined in FP64, k = 8, 32, 128
+    single_rate = 1.0 / solves["uniform_seed2"]["fp64"]["seconds"]
+    blk = {"single_rhs_solves_per_s": round(single_rate, 2)}
+    for k in (8, 32, 128):
+        B = rng.uniform(-1, 1, (nd, k))
+        a64.block_bicgstab(B, eps=1e-8, maxit=3000, check_every=8)
+        U, cols, inf = a64.block_bicgstab(B, eps=1e-8, maxit=3000, check_every=8)
+        blk["fp64_k%d" % k] = {
+            "seconds": round(inf["seconds"], 4), "iterations": inf["nstp"], "status": inf["status_name"],
+            "worst_true_relres": inf["true_relres"], "rhs_solves_per_s": round(k / inf["seconds"], 2)}
+        log("s=7 block BiCGSTAB FP64 k=%d: %.3f s, %.1f RHS-solves/s (single: %.1f)"
+            % (k, inf["seconds"], k / inf["seconds"], single_rate))
+        H.block_bicgstab_ir(a64, arenas["f32"], B, eps=1e-8, inner_eps=1e-4, max_outer=40, max_inner=3000)
+        U2, cols2, inf2 = H.block_bicgstab_ir(a64, arenas["f32"], B, eps=1e-8, inner_eps=1e-4,
+                                              max_outer=40, max_inner=3000)
+        errs = [rel_err(U2[:, j], U[:, j]) for j in (0, k - 1)]
+        blk["f32_ir_k%d" % k] = {
+            "seconds": round(inf2["seconds"], 4), "outer": inf2["outer"],
+            "inner_iterations": inf2["inner_total"], "status": inf2["status_name"],
+            "worst_true_relres": inf2["true_relres"], "rhs_solves_per_s": round(k / inf2["seconds"], 2),
+            "rel_err_vs_fp64_block_solution_sampled": max(errs)}
+        log("s=7 block BiCGSTAB FP32 factors + FP64 refinement k=%d: %.3f s (%d outer), %.1f RHS-solves/s,"
+            " worst relres %.2e" % (k, inf2["seconds"], inf2["outer"], k / inf2["seconds"],
+                                    inf2["true_relres"]))
+    out["block_bicgstab"] = blk
     # the same H-matrix filled on the device, and its arena repacked there
     mesh = H.icosphere(7)
     t0 = time.perf_counter()
@@ -388,7 +509,6 @@ def study_327680(H, log):
     t1 = time.perf_counter()
     ad = dh.arena("f64")
     t2 = time.perf_counter()
-    xh = H.uniform(1, nd)
     yd, yh = ad.matvec(xh), a64.matvec(xh)
     out["device_construction"] = {
         "blocks_host_s": round(dh.block_seconds, 3), "fill_device_s": round(dh.fill_seconds, 3),
@@ -464,9 +584,14 @@ def construct_only(H, log, reps=2):
     return out
 
 
-def study_instances(H, log):
+def study_instances(H, O, log):
     """48 icospheres with s in {4,5,6} and radii U[0.5,2] (seed 4), ACA+ 1e-4,
-    FP64: one arena, one BiCGSTAB per instance to 1e-8 (RHS = 1)."""
+    RHS = 1: one FP64 arena with one BiCGSTAB per instance to 1e-8; and the
+    mixed-precision configuration: a BF16-factor arena whose tuning the
+    autotuner picks over the whole batch, with per-instance FP64 refinement
+    (hmv_multi_bicgstab_ir).  Errors of sampled instances (one of each size)
+    against the oracle's FP64 solutions."""
+    nthr = cpu_threads()
     rng = np.random.default_rng(4)
     subs = rng.choice([4, 5, 6], size=48)
     radii = rng.uniform(0.5, 2.0, size=48)
@@ -482,19 +607,53 @@ def study_instances(H, log):
     m.bicgstab_multi(b, eps=1e-8, maxit=3000, check_every=4)
     u, inst, secs, inf = m.bicgstab_multi(b, eps=1e-8, maxit=3000, check_every=4)
     steps = [d["nstp"] for d in inst]
+    sample = sorted({int(np.flatnonzero(subs == s)[0]) for s in (4, 5, 6) if (subs == s).any()})
+    oracle = {}
+    for i in sample:
+        lo, hi = m.offsets[i], m.offsets[i + 1]
+        oracle[i] = O.bicgstab(hms[i], b[lo:hi], None, mstep=3000, eps=1e-8, nthr=nthr)[0]
+
+    def errs(uu):
+        return {str(i): rel_err(uu[m.offsets[i]:m.offsets[i + 1]], oracle[i]) for i in sample}
+
     out = {"instances": 48, "unknowns": int(m.nd),
            "sizes": {"5120": int((subs == 4).sum()), "20480": int((subs == 5).sum()),
                      "81920": int((subs == 6).sum())},
            "construction_s": round(tcons, 2), "arena_GB": round(m.stats["arena_bytes"] / 1e9, 2),
            "apply_all_ms": round(ma, 4), "apply_alg_GBs": round(m.stats["alg_bytes"] / ma / 1e6, 1),
-           "solve_seconds": round(inf["seconds"], 4), "status": inf["status_name"],
-           "worst_true_relres": inf["true_relres"], "steps_min": min(steps), "steps_max": max(steps),
-           "unknowns_per_s": round(m.nd / inf["seconds"], 1),
-           "instance_seconds_p50": round(float(np.percentile(secs, 50)), 4),
-           "instance_seconds_p99": round(float(np.percentile(secs, 99)), 4)}
-    log("48 instances (%d unknowns): one apply %.3f ms; solve %.4f s, %.3e unknowns/s, p50 %.4f s p99 %.4f s"
-        % (m.nd, ma, inf["seconds"], m.nd / inf["seconds"], out["instance_seconds_p50"],
-           out["instance_seconds_p99"]))
+           "fp64": {"solve_seconds": round(inf["seconds"], 4), "status": inf["status_name"],
+                    "worst_true_relres": inf["true_relres"], "steps_min": min(steps),
+                    "steps_max": max(steps), "unknowns_per_s": round(m.nd / inf["seconds"], 1),
+                    "instance_seconds_p50": round(float(np.percentile(secs, 50)), 4),
+                    "instance_seconds_p99": round(float(np.percentile(secs, 99)), 4),
+                    "rel_err_vs_oracle_solution_sampled": errs(u)}}
+    log("48 instances (%d unknowns): one apply %.3f ms; FP64 solve %.4f s, %.3e unknowns/s, p50 %.4f s "
+        "p99 %.4f s" % (m.nd, ma, inf["seconds"], m.nd / inf["seconds"],
+                        out["fp64"]["instance_seconds_p50"], out["fp64"]["instance_seconds_p99"]))
+    # mixed precision: autotuned BF16 factors, per-instance FP64 refinement
+    t0 = time.perf_counter()
+    best, best_ms, dflt_ms = H.autotune_multi(hms, "bf16", reps=5)
+    t_tune = time.perf_counter() - t0
+    mlow = H.MultiArena(hms, "bf16", tuning=best)
+    H.multi_bicgstab_ir(m, mlow, b, eps=1e-8, inner_eps=1e-2, max_outer=40, max_inner=3000)
+    u2, inst2, secs2, inf2 = H.multi_bicgstab_ir(m, mlow, b, eps=1e-8, inner_eps=1e-2, max_outer=40,
+                                                 max_inner=3000)
+    out["mixed_bf16"] = {
+        "tuning": best, "autotune_s": round(t_tune, 1), "apply_ms_tuned": round(best_ms, 4),
+        "apply_ms_default": round(dflt_ms, 4), "arena_GB": round(mlow.stats["arena_bytes"] / 1e9, 2),
+        "inner_eps": 1e-2, "outer": inf2["outer"], "inner_iterations": inf2["inner_total"],
+        "solve_seconds": round(inf2["seconds"], 4), "status": inf2["status_name"],
+        "worst_true_relres": inf2["true_relres"], "unknowns_per_s": round(m.nd / inf2["seconds"], 1),
+        "instance_seconds_p50": round(float(np.percentile(secs2, 50)), 4),
+        "instance_seconds_p99": round(float(np.percentile(secs2, 99)), 4),
+        "time_ratio_vs_fp64": round(inf2["seconds"] / inf["seconds"], 3),
+        "rel_err_vs_oracle_solution_sampled": errs(u2)}
+    log("48 instances BF16 (autotuned %s, %.4f ms vs %.4f default, %.0f s) + FP64 refinement: %.4f s "
+        "(%.2fx FP64), %d outer, %.3e unknowns/s, p50 %.4f s p99 %.4f s, worst relres %.2e"
+        % (best, best_ms, dflt_ms, t_tune, inf2["seconds"], inf2["seconds"] / inf["seconds"],
+           inf2["outer"], m.nd / inf2["seconds"], out["mixed_bf16"]["instance_seconds_p50"],
+           out["mixed_bf16"]["instance_seconds_p99"], inf2["true_relres"]))
+    mlow.free()
     m.free()
     return out
 

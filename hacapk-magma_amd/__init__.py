@@ -243,6 +243,15 @@ def hmv_lib():
                                             C.POINTER(vp), C.c_int, C.c_int, C.POINTER(Tuning),
                                             C.POINTER(vp)]
         L.hmv_arena_segments.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
+        L.hmv_autotune_multi.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(vp), C.c_int, C.c_int, C.c_int,
+                                         C.POINTER(Tuning), _D, _D]
+        L.hmv_block_bicgstab_ir.argtypes = [vp, vp, C.c_int, vp, vp, C.c_double, C.c_double,
+                                            C.c_int, C.c_int, C.c_int, C.POINTER(ColumnInfo),
+                                            C.POINTER(SolveInfo)]
+        L.hmv_multi_bicgstab_ir.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_int,
+                                            C.c_int, C.c_int, C.POINTER(ColumnInfo), _D,
+                                            C.POINTER(SolveInfo)]
         L.hmv_multi_bicgstab.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int,
                                          C.POINTER(ColumnInfo), _D, C.POINTER(SolveInfo)]
         L.hmv_apply_block_timed.argtypes = [vp, C.c_int, vp, vp, C.c_int, _D]
@@ -963,6 +972,70 @@ class MultiArena(Arena):
                                             int(check_every), inst, _dptr(secs), C.byref(info)),
                "hmv_multi_bicgstab")
         return du.download(), [c.as_dict() for c in inst], secs, info.as_dict()
+
+
+def block_bicgstab_ir(a64, alow, B, U0=None, eps=1e-8, inner_eps=1e-4, max_outer=30,
+                      max_inner=2000, check_every=8):
+    """Mixed-precision refinement of a block solve (hmv_block_bicgstab_ir): B
+    is (N, k), k a power of two <= 128.  Returns (U, per-column info list,
+    info)."""
+    B = np.ascontiguousarray(B, np.float64)
+    k = B.shape[1]
+    U0 = np.zeros_like(B) if U0 is None else np.ascontiguousarray(U0, np.float64)
+    db, du = DeviceBuffer.from_array(B.reshape(-1)), DeviceBuffer.from_array(U0.reshape(-1))
+    cols = (ColumnInfo * k)()
+    info = SolveInfo()
+    try:
+        _check(hmv_lib().hmv_block_bicgstab_ir(a64.handle, alow.handle, k, db.ptr, du.ptr,
+                                               float(eps), float(inner_eps), int(max_outer),
+                                               int(max_inner), int(check_every), cols,
+                                               C.byref(info)), "hmv_block_bicgstab_ir")
+        U = du.download().reshape(a64.nd, k)
+    finally:
+        db.free()
+        du.free()
+    return U, [c.as_dict() for c in cols], info.as_dict()
+
+
+def multi_bicgstab_ir(m64, mlow, b, u0=None, eps=1e-8, inner_eps=1e-4, max_outer=30,
+                      max_inner=2000, check_every=4):
+    """Mixed-precision refinement per instance over two MultiArenas of the
+    same instances (hmv_multi_bicgstab_ir).  Returns (u, per-instance info
+    list, per-instance seconds, info)."""
+    b = np.ascontiguousarray(b, np.float64)
+    u0 = np.zeros_like(b) if u0 is None else np.ascontiguousarray(u0, np.float64)
+    n = len(m64.offsets) - 1
+    db, du = DeviceBuffer.from_array(b), DeviceBuffer.from_array(u0)
+    inst = (ColumnInfo * n)()
+    secs = np.zeros(n)
+    info = SolveInfo()
+    try:
+        _check(hmv_lib().hmv_multi_bicgstab_ir(m64.handle, mlow.handle, db.ptr, du.ptr, float(eps),
+                                               float(inner_eps), int(max_outer), int(max_inner),
+                                               int(check_every), inst, _dptr(secs), C.byref(info)),
+               "hmv_multi_bicgstab_ir")
+        u = du.download()
+    finally:
+        db.free()
+        du.free()
+    return u, [c.as_dict() for c in inst], secs, info.as_dict()
+
+
+def autotune_multi(hms, precision="f64", reps=10):
+    """hmv_autotune_multi over the instances hms: returns (best tuning dict,
+    ms per apply of all instances, ms with the defaults)."""
+    n = len(hms)
+    strides = {hm.stride for hm in hms}
+    assert len(strides) == 1, "leaf records of one layout"
+    nd = (C.c_int * n)(*[hm.nd for hm in hms])
+    nlf = (C.c_int * n)(*[hm.nlf for hm in hms])
+    lv = (C.c_void_p * n)(*[hm.leaves_ptr for hm in hms])
+    t = Tuning()
+    best, dflt = C.c_double(), C.c_double()
+    _check(hmv_lib().hmv_autotune_multi(n, nd, nlf, lv, strides.pop(), PREC[precision], int(reps),
+                                        C.byref(t), C.byref(best), C.byref(dflt)),
+           "hmv_autotune_multi")
+    return t.as_dict(), best.value, dflt.value
 
 
 def default_tuning(precision="f64"):

@@ -339,6 +339,134 @@ int upload_plan(hmv::Plan &P, hmv_arena *out, std::chrono::steady_clock::time_po
   return HMV_OK;
 }
 
+// One block-diagonal leaf list of ninst instances: instance i's rows and
+// columns shifted by the sizes of the instances before it; seg receives the
+// ninst + 1 row offsets.  False (with the error set) on invalid input.
+bool concat_instances(const char *who, int ninst, const int *nd, const int *nlf,
+                      const void *const *leaves, int leaf_stride, std::vector<int> &seg,
+                      std::vector<hmv::RefLeaf> &all) {
+  if (ninst < 1 || !nd || !nlf || !leaves || leaf_stride < (int)sizeof(hmv::RefLeaf)) {
+    set_error(std::string(who) + ": invalid argument");
+    return false;
+  }
+  seg.assign(ninst + 1, 0);
+  int64_t total_lf = 0;
+  for (int i = 0; i < ninst; ++i) {
+    if (nd[i] < 1 || nlf[i] < 1 || !leaves[i]) {
+      set_error(std::string(who) + ": empty instance " + std::to_string(i));
+      return false;
+    }
+    if ((int64_t)seg[i] + nd[i] > INT32_MAX) {
+      set_error(std::string(who) + ": more than 2^31 unknowns");
+      return false;
+    }
+    seg[i + 1] = seg[i] + nd[i];
+    total_lf += nlf[i];
+  }
+  if (total_lf > INT32_MAX) {
+    set_error(std::string(who) + ": too many leaves");
+    return false;
+  }
+  all.clear();
+  all.reserve((size_t)total_lf);
+  for (int i = 0; i < ninst; ++i) {
+    const char *p = static_cast<const char *>(leaves[i]);
+    for (int l = 0; l < nlf[i]; ++l) {
+      hmv::RefLeaf r;
+      std::memcpy(&r, p + (size_t)l * leaf_stride, sizeof(r));
+      if (r.nstrtl < 1 || r.nstrtt < 1 || r.nstrtl - 1 + r.ndl > nd[i] ||
+          r.nstrtt - 1 + r.ndt > nd[i]) {
+        set_error(std::string(who) + ": leaf outside its instance " + std::to_string(i));
+        return false;
+      }
+      r.nstrtl += seg[i];
+      r.nstrtt += seg[i];
+      all.push_back(r);
+    }
+  }
+  return true;
+}
+
+// Coordinate search of the repack tunables (small-workgroup threads of each
+// phase, phase-1 bytes per workgroup, phase-2 rows, phase-1 row tile), one
+// at a time, each candidate an arena built by make(tuning, &arena) and timed
+// over `reps` applies (HIP events); a candidate must beat the current best
+// by more than 1% (timing noise) to replace it.
+template <typename Make>
+int autotune_search(const char *who, int nd, int precision, int reps, Make make, hmv_tuning *best,
+                    double *best_ms, double *default_ms) {
+  int rc = check_device();
+  if (rc) return rc;
+  if (reps < 3) reps = 3;
+  const size_t nb = sizeof(double) * (size_t)nd;
+  double *dx = nullptr, *dy = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e;
+  if ((e = hipMalloc(&dx, nb)) != hipSuccess || (e = hipMalloc(&dy, nb)) != hipSuccess ||
+      (e = hipMemset(dx, 0, nb)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+    rc = hip_fail(e, (std::string(who) + ": setup").c_str());
+  }
+  // time per apply of an arena built with t (the arena lives only for the measurement)
+  auto measure = [&](const hmv::PlanTuning &t, double *ms) -> int {
+    hmv_arena a = nullptr;
+    int r = make(t, &a);
+    if (r) return r;
+    hipError_t he = hipSuccess;
+    for (int i = 0; i < 2 && he == hipSuccess; ++i) he = hmv::enqueue_apply(a, dx, dy, 0.0, nullptr, s);
+    if (he == hipSuccess) he = hipEventRecord(e0, s);
+    for (int i = 0; i < reps && he == hipSuccess; ++i)
+      he = hmv::enqueue_apply(a, dx, dy, 0.0, nullptr, s);
+    if (he == hipSuccess) he = hipEventRecord(e1, s);
+    if (he == hipSuccess) he = hipEventSynchronize(e1);
+    float m = 0.f;
+    if (he == hipSuccess) he = hipEventElapsedTime(&m, e0, e1);
+    release(a);
+    if (he != hipSuccess) return hip_fail(he, (std::string(who) + ": timing").c_str());
+    *ms = (double)m / reps;
+    return HMV_OK;
+  };
+  auto same = [](const hmv::PlanTuning &a, const hmv::PlanTuning &b) {
+    return a.p1_target_bytes == b.p1_target_bytes && a.p1_row_tile == b.p1_row_tile &&
+           a.p2_max_rows == b.p2_max_rows && a.p1_small_threads == b.p1_small_threads &&
+           a.p2_small_threads == b.p2_small_threads;
+  };
+  hmv::PlanTuning cur = hmv::default_tuning(precision);
+  double cur_ms = 0.0;
+  if (!rc) rc = measure(cur, &cur_ms);
+  if (!rc && default_ms) *default_ms = cur_ms;
+  auto search = [&](auto set, std::initializer_list<int64_t> values) {
+    for (int64_t v : values) {
+      if (rc) return;
+      hmv::PlanTuning t = cur;
+      set(t, v);
+      if (same(t, cur)) continue;
+      double ms = 0.0;
+      rc = measure(t, &ms);
+      if (!rc && ms < 0.99 * cur_ms) {
+        cur = t;
+        cur_ms = ms;
+      }
+    }
+  };
+  search([](hmv::PlanTuning &t, int64_t v) { t.p1_small_threads = (int)v; }, {-1, 64, 128});
+  search([](hmv::PlanTuning &t, int64_t v) { t.p2_small_threads = (int)v; }, {-1, 64, 128});
+  search([](hmv::PlanTuning &t, int64_t v) { t.p1_target_bytes = v; }, {48 << 10, 96 << 10, 192 << 10});
+  search([](hmv::PlanTuning &t, int64_t v) { t.p2_max_rows = (int)v; }, {128, 256});
+  search([](hmv::PlanTuning &t, int64_t v) { t.p1_row_tile = (int)v; }, {128, 256});
+  if (e1) (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (s) (void)hipStreamDestroy(s);
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  if (rc) return rc;
+  *best = to_public(cur);
+  if (best_ms) *best_ms = cur_ms;
+  return HMV_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -366,50 +494,16 @@ int hmv_arena_build(int nd, int nlf, const void *leaves, int leaf_stride, int pr
 int hmv_arena_build_multi(int ninst, const int *nd, const int *nlf, const void *const *leaves,
                           int leaf_stride, int precision, const hmv_tuning *tuning,
                           hmv_arena *out) {
-  if (!out || ninst < 1 || !nd || !nlf || !leaves || leaf_stride < (int)sizeof(hmv::RefLeaf)) {
-    set_error("hmv_arena_build_multi: invalid argument");
+  if (!out) {
+    set_error("hmv_arena_build_multi: null output handle");
     return HMV_ERR_ARG;
   }
-  // one block-diagonal leaf list: instance i's rows and columns shifted by
-  // the sizes of the instances before it
-  std::vector<int> seg(ninst + 1, 0);
-  int64_t total_lf = 0;
-  for (int i = 0; i < ninst; ++i) {
-    if (nd[i] < 1 || nlf[i] < 1 || !leaves[i]) {
-      set_error("hmv_arena_build_multi: empty instance " + std::to_string(i));
-      return HMV_ERR_ARG;
-    }
-    if ((int64_t)seg[i] + nd[i] > INT32_MAX) {
-      set_error("hmv_arena_build_multi: more than 2^31 unknowns");
-      return HMV_ERR_ARG;
-    }
-    seg[i + 1] = seg[i] + nd[i];
-    total_lf += nlf[i];
-  }
-  if (total_lf > INT32_MAX) {
-    set_error("hmv_arena_build_multi: too many leaves");
-    return HMV_ERR_ARG;
-  }
+  std::vector<int> seg;
   std::vector<hmv::RefLeaf> all;
-  all.reserve((size_t)total_lf);
-  for (int i = 0; i < ninst; ++i) {
-    const char *p = static_cast<const char *>(leaves[i]);
-    for (int l = 0; l < nlf[i]; ++l) {
-      hmv::RefLeaf r;
-      std::memcpy(&r, p + (size_t)l * leaf_stride, sizeof(r));
-      if (r.nstrtl < 1 || r.nstrtt < 1 || r.nstrtl - 1 + r.ndl > nd[i] ||
-          r.nstrtt - 1 + r.ndt > nd[i]) {
-        set_error("hmv_arena_build_multi: leaf outside its instance " + std::to_string(i));
-        return HMV_ERR_ARG;
-      }
-      r.nstrtl += seg[i];
-      r.nstrtt += seg[i];
-      all.push_back(r);
-    }
-  }
-  int rc = build(seg[ninst], (int)total_lf, all.data(), (int)sizeof(hmv::RefLeaf), precision,
-                 from_public(precision, tuning), out, &seg);
-  return rc;
+  if (!concat_instances("hmv_arena_build_multi", ninst, nd, nlf, leaves, leaf_stride, seg, all))
+    return HMV_ERR_ARG;
+  return build(seg[ninst], (int)all.size(), all.data(), (int)sizeof(hmv::RefLeaf), precision,
+               from_public(precision, tuning), out, &seg);
 }
 
 int hmv_arena_segments(hmv_arena a, int *offsets, int cap) {
@@ -646,78 +740,29 @@ int hmv_autotune(int nd, int nlf, const void *leaves, int leaf_stride, int preci
     set_error("hmv_autotune: invalid argument");
     return HMV_ERR_ARG;
   }
-  int rc = check_device();
-  if (rc) return rc;
-  if (reps < 3) reps = 3;
-  const size_t nb = sizeof(double) * (size_t)nd;
-  double *dx = nullptr, *dy = nullptr;
-  hipStream_t s = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  hipError_t e;
-  if ((e = hipMalloc(&dx, nb)) != hipSuccess || (e = hipMalloc(&dy, nb)) != hipSuccess ||
-      (e = hipMemset(dx, 0, nb)) != hipSuccess ||
-      (e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess ||
-      (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
-    rc = hip_fail(e, "hmv_autotune: setup");
+  auto make = [&](const hmv::PlanTuning &t, hmv_arena *a) {
+    return build(nd, nlf, leaves, leaf_stride, precision, t, a);
+  };
+  return autotune_search("hmv_autotune", nd, precision, reps, make, best, best_ms, default_ms);
+}
+
+int hmv_autotune_multi(int ninst, const int *nd, const int *nlf, const void *const *leaves,
+                       int leaf_stride, int precision, int reps, hmv_tuning *best, double *best_ms,
+                       double *default_ms) {
+  if (!best) {
+    set_error("hmv_autotune_multi: invalid argument");
+    return HMV_ERR_ARG;
   }
-  // time per apply of an arena built with t (the arena lives only for the measurement)
-  auto measure = [&](const hmv::PlanTuning &t, double *ms) -> int {
-    hmv_arena a = nullptr;
-    int r = build(nd, nlf, leaves, leaf_stride, precision, t, &a);
-    if (r) return r;
-    hipError_t he = hipSuccess;
-    for (int i = 0; i < 2 && he == hipSuccess; ++i) he = hmv::enqueue_apply(a, dx, dy, 0.0, nullptr, s);
-    if (he == hipSuccess) he = hipEventRecord(e0, s);
-    for (int i = 0; i < reps && he == hipSuccess; ++i)
-      he = hmv::enqueue_apply(a, dx, dy, 0.0, nullptr, s);
-    if (he == hipSuccess) he = hipEventRecord(e1, s);
-    if (he == hipSuccess) he = hipEventSynchronize(e1);
-    float m = 0.f;
-    if (he == hipSuccess) he = hipEventElapsedTime(&m, e0, e1);
-    release(a);
-    if (he != hipSuccess) return hip_fail(he, "hmv_autotune: timing");
-    *ms = (double)m / reps;
-    return HMV_OK;
+  std::vector<int> seg;
+  std::vector<hmv::RefLeaf> all;
+  if (!concat_instances("hmv_autotune_multi", ninst, nd, nlf, leaves, leaf_stride, seg, all))
+    return HMV_ERR_ARG;
+  auto make = [&](const hmv::PlanTuning &t, hmv_arena *a) {
+    return build(seg[ninst], (int)all.size(), all.data(), (int)sizeof(hmv::RefLeaf), precision, t,
+                 a, &seg);
   };
-  auto same = [](const hmv::PlanTuning &a, const hmv::PlanTuning &b) {
-    return a.p1_target_bytes == b.p1_target_bytes && a.p1_row_tile == b.p1_row_tile &&
-           a.p2_max_rows == b.p2_max_rows && a.p1_small_threads == b.p1_small_threads &&
-           a.p2_small_threads == b.p2_small_threads;
-  };
-  hmv::PlanTuning cur = hmv::default_tuning(precision);
-  double cur_ms = 0.0;
-  if (!rc) rc = measure(cur, &cur_ms);
-  if (!rc && default_ms) *default_ms = cur_ms;
-  // coordinate search, one tunable at a time; a candidate must beat the
-  // current best by more than 1% (timing noise) to replace it
-  auto search = [&](auto set, std::initializer_list<int64_t> values) {
-    for (int64_t v : values) {
-      if (rc) return;
-      hmv::PlanTuning t = cur;
-      set(t, v);
-      if (same(t, cur)) continue;
-      double ms = 0.0;
-      rc = measure(t, &ms);
-      if (!rc && ms < 0.99 * cur_ms) {
-        cur = t;
-        cur_ms = ms;
-      }
-    }
-  };
-  search([](hmv::PlanTuning &t, int64_t v) { t.p1_small_threads = (int)v; }, {-1, 64, 128});
-  search([](hmv::PlanTuning &t, int64_t v) { t.p2_small_threads = (int)v; }, {-1, 64, 128});
-  search([](hmv::PlanTuning &t, int64_t v) { t.p1_target_bytes = v; }, {48 << 10, 96 << 10, 192 << 10});
-  search([](hmv::PlanTuning &t, int64_t v) { t.p2_max_rows = (int)v; }, {128, 256});
-  search([](hmv::PlanTuning &t, int64_t v) { t.p1_row_tile = (int)v; }, {128, 256});
-  if (e1) (void)hipEventDestroy(e1);
-  if (e0) (void)hipEventDestroy(e0);
-  if (s) (void)hipStreamDestroy(s);
-  (void)hipFree(dx);
-  (void)hipFree(dy);
-  if (rc) return rc;
-  *best = to_public(cur);
-  if (best_ms) *best_ms = cur_ms;
-  return HMV_OK;
+  return autotune_search("hmv_autotune_multi", seg[ninst], precision, reps, make, best, best_ms,
+                         default_ms);
 }
 
 int hmv_stream_test(int kind, const void *src, void *dst, size_t bytes, int reps, double *ms) {

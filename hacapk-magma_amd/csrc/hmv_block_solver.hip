@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "hmv.h"
@@ -262,6 +263,7 @@ struct BlockWork {
   BlockState *bs = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  const double *graph_u = nullptr;  // the iterate the captured graph updates
   ~BlockWork() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -293,14 +295,14 @@ hipError_t enqueue_iteration(hmv_arena_s *a, BlockWork &w, double *u) {
   return hipGetLastError();
 }
 
-// per-column ||b - A u|| / ||b|| into rel[k] (uses w.r)
-int true_relres(hmv_arena_s *a, BlockWork &w, const double *b, const double *u,
-                std::vector<double> &rel) {
+// per-column ||b - A u|| / ||b|| into rel[k]; r (N x k) receives b - A u
+int block_residual(hmv_arena_s *a, BlockWork &w, const double *b, const double *u, double *r,
+                   std::vector<double> &rel) {
   const int g = grid_of(w.n), k = w.k;
-  hipError_t e = hmv::enqueue_apply_block(a, k, u, w.r, 0.0, nullptr, w.s);
+  hipError_t e = hmv::enqueue_apply_block(a, k, u, r, 0.0, nullptr, w.s);
   if (e != hipSuccess) return hmv::hip_fail(e, "block true residual");
-  hipLaunchKernelGGL(kb_residual, dim3(g), dim3(kT), 0, w.s, w.n, b, w.r);
-  hipLaunchKernelGGL(kb_dots, dim3(g), dim3(kT), 0, w.s, w.n, k, w.r, w.r, b, b, w.part,
+  hipLaunchKernelGGL(kb_residual, dim3(g), dim3(kT), 0, w.s, w.n, b, r);
+  hipLaunchKernelGGL(kb_dots, dim3(g), dim3(kT), 0, w.s, w.n, k, r, r, b, b, w.part,
                      (const BlockState *)nullptr);
   std::vector<double> part(2 * (size_t)g * k);
   if ((e = hipMemcpyAsync(part.data(), w.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
@@ -319,19 +321,31 @@ int true_relres(hmv_arena_s *a, BlockWork &w, const double *b, const double *u,
   return HMV_OK;
 }
 
-}  // namespace
+__global__ __launch_bounds__(kT) void kb_fill_n(int64_t n, double *__restrict__ x, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    x[i] = v;
+}
 
-extern "C" int hmv_block_bicgstab(hmv_arena a, int k, const double *b, double *u, double eps,
-                                  int maxit, int check_every, hmv_column_info *cols,
-                                  hmv_solve_info *info) {
-  if (!a || !b || !u || k < 1 || k > HMV_MAX_RHS || (k & (k - 1)) != 0 || maxit < 0) {
-    hmv::set_error("hmv_block_bicgstab: invalid argument (k must be a power of two <= 128)");
-    return HMV_ERR_ARG;
-  }
+// x = 0 in the stopped columns (element i is column i % k)
+__global__ __launch_bounds__(kT) void kb_zero_cols(int64_t n, int k, double *__restrict__ x,
+                                                   const int *__restrict__ stopped) {
+  const int j = threadIdx.x % k;
+  if (!stopped[j]) return;
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    x[i] = 0.0;
+}
+
+// u += d
+__global__ __launch_bounds__(kT) void kb_add_n(int64_t n, double *__restrict__ u,
+                                               const double *__restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    u[i] += d[i];
+}
+
+// Buffers of a k-column block solve on arena a.
+int block_setup(hmv_arena_s *a, BlockWork &w, int k, const char *who) {
   int rc = hmv::ensure_tk(a, k);
   if (rc) return rc;
-  if (check_every < 1) check_every = 1;
-  BlockWork w;
   w.k = k;
   w.n = (int64_t)a->nd * k;
   const int g = grid_of(w.n);
@@ -344,48 +358,94 @@ extern "C" int hmv_block_bicgstab(hmv_arena a, int k, const double *b, double *u
       (e = hipMalloc(&w.part, sizeof(double) * 2 * g * k)) != hipSuccess ||
       (e = hipMalloc(&w.cs, sizeof(ColState) * k)) != hipSuccess ||
       (e = hipMalloc(&w.bs, sizeof(BlockState))) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_block_bicgstab: workspace");
-  auto t0 = std::chrono::steady_clock::now();
+    return hmv::hip_fail(e, (std::string(who) + ": workspace").c_str());
+  return HMV_OK;
+}
+
+// k BiCGSTABs sharing one H-matmat per step, from u (zero_guess: u = 0 and
+// r = b without a product), until every column has stopped or maxit
+// iterations ran; cs receives the columns' final states, iters the
+// iterations run.  The iteration graph is captured on the first call with a
+// given u and reused by later calls.
+int block_solve(hmv_arena_s *a, BlockWork &w, const double *b, double *u, double eps, int maxit,
+                int check_every, bool zero_guess, std::vector<ColState> &cs, int &iters,
+                const char *who) {
+  const std::string name(who);
+  const int g = grid_of(w.n), k = w.k;
   BlockState init{};
   init.eps = eps;
   init.maxit = maxit;
-  if ((e = hipMemcpyAsync(w.bs, &init, sizeof(init), hipMemcpyHostToDevice, w.s)) != hipSuccess ||
-      (e = hmv::enqueue_apply_block(a, k, u, w.r, 0.0, nullptr, w.s)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_block_bicgstab: setup");
+  hipError_t e = hipMemcpyAsync(w.bs, &init, sizeof(init), hipMemcpyHostToDevice, w.s);
+  if (e != hipSuccess) return hmv::hip_fail(e, (name + ": setup").c_str());
+  if (zero_guess)
+    hipLaunchKernelGGL(kb_fill_n, dim3(g), dim3(kT), 0, w.s, w.n, w.r, 0.0);
+  else if ((e = hmv::enqueue_apply_block(a, k, u, w.r, 0.0, nullptr, w.s)) != hipSuccess)
+    return hmv::hip_fail(e, (name + ": setup").c_str());
   hipLaunchKernelGGL(kb_init, dim3(g), dim3(kT), 0, w.s, w.n, b, w.r, w.rhat, w.p, w.v);
   hipLaunchKernelGGL(kb_dots, dim3(g), dim3(kT), 0, w.s, w.n, k, w.r, w.r, b, b, w.part,
                      (const BlockState *)nullptr);
   hipLaunchKernelGGL(kb_final, dim3(1), dim3(kT), 0, w.s, g, k, w.part, w.cs, w.bs, 0);
-  if ((e = hipStreamSynchronize(w.s)) != hipSuccess ||
-      (e = hipStreamBeginCapture(w.s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_block_bicgstab: capture");
-  hipError_t ec = enqueue_iteration(a, w, u);
-  e = hipStreamEndCapture(w.s, &w.graph);
-  if (ec != hipSuccess) return hmv::hip_fail(ec, "hmv_block_bicgstab: capture");
-  if (e != hipSuccess || (e = hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_block_bicgstab: graph");
+  if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(w.s)) != hipSuccess)
+    return hmv::hip_fail(e, (name + ": setup").c_str());
+  if (!w.exec || w.graph_u != u) {
+    if (w.exec) (void)hipGraphExecDestroy(w.exec);
+    if (w.graph) (void)hipGraphDestroy(w.graph);
+    w.exec = nullptr;
+    w.graph = nullptr;
+    if ((e = hipStreamBeginCapture(w.s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hmv::hip_fail(e, (name + ": capture").c_str());
+    hipError_t ec = enqueue_iteration(a, w, u);
+    e = hipStreamEndCapture(w.s, &w.graph);
+    if (ec != hipSuccess) return hmv::hip_fail(ec, (name + ": capture").c_str());
+    if (e != hipSuccess ||
+        (e = hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0)) != hipSuccess)
+      return hmv::hip_fail(e, (name + ": graph").c_str());
+    w.graph_u = u;
+  }
   BlockState host{};
   int launched = 0;
   while (launched < maxit) {
     const int burst = std::min(check_every, maxit - launched);
     for (int i = 0; i < burst; ++i)
       if ((e = hipGraphLaunch(w.exec, w.s)) != hipSuccess)
-        return hmv::hip_fail(e, "hmv_block_bicgstab: launch");
+        return hmv::hip_fail(e, (name + ": launch").c_str());
     launched += burst;
     if ((e = hipMemcpyAsync(&host, w.bs, sizeof(host), hipMemcpyDeviceToHost, w.s)) != hipSuccess ||
         (e = hipStreamSynchronize(w.s)) != hipSuccess)
-      return hmv::hip_fail(e, "hmv_block_bicgstab: state");
+      return hmv::hip_fail(e, (name + ": state").c_str());
     if (host.all_done) break;
   }
   if ((e = hipMemcpyAsync(&host, w.bs, sizeof(host), hipMemcpyDeviceToHost, w.s)) != hipSuccess ||
       (e = hipStreamSynchronize(w.s)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_block_bicgstab: state");
-  auto t1 = std::chrono::steady_clock::now();
-  std::vector<ColState> cs(k);
+    return hmv::hip_fail(e, (name + ": state").c_str());
+  iters = host.iter;
+  cs.resize(k);
   if ((e = hipMemcpy(cs.data(), w.cs, sizeof(ColState) * k, hipMemcpyDeviceToHost)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_block_bicgstab: columns");
+    return hmv::hip_fail(e, (name + ": columns").c_str());
+  return HMV_OK;
+}
+
+}  // namespace
+
+extern "C" int hmv_block_bicgstab(hmv_arena a, int k, const double *b, double *u, double eps,
+                                  int maxit, int check_every, hmv_column_info *cols,
+                                  hmv_solve_info *info) {
+  if (!a || !b || !u || k < 1 || k > HMV_MAX_RHS || (k & (k - 1)) != 0 || maxit < 0) {
+    hmv::set_error("hmv_block_bicgstab: invalid argument (k must be a power of two <= 128)");
+    return HMV_ERR_ARG;
+  }
+  if (check_every < 1) check_every = 1;
+  const char *who = "hmv_block_bicgstab";
+  BlockWork w;
+  int rc = block_setup(a, w, k, who);
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<ColState> cs;
+  int iters = 0;
+  if ((rc = block_solve(a, w, b, u, eps, maxit, check_every, false, cs, iters, who))) return rc;
+  auto t1 = std::chrono::steady_clock::now();
   std::vector<double> rel;
-  if ((rc = true_relres(a, w, b, u, rel)) != HMV_OK) return rc;
+  if ((rc = block_residual(a, w, b, u, w.r, rel)) != HMV_OK) return rc;
   int maxstep = 0, worst = HMV_SOLVE_CONVERGED;
   double maxrel = 0.0;
   for (int j = 0; j < k; ++j) {
@@ -401,13 +461,125 @@ extern "C" int hmv_block_bicgstab(hmv_arena a, int k, const double *b, double *u
     if (status != HMV_SOLVE_CONVERGED) worst = status;
   }
   if (info) {
-    info->nstp = host.iter;
+    info->nstp = iters;
     info->status = worst;
     info->relres = maxrel;
     info->true_relres = maxrel;
     info->seconds = std::chrono::duration<double>(t1 - t0).count();
     info->outer = 0;
     info->inner_total = maxstep;
+  }
+  return HMV_OK;
+}
+
+extern "C" int hmv_block_bicgstab_ir(hmv_arena a64, hmv_arena alow, int k, const double *b,
+                                     double *u, double eps, double inner_eps, int max_outer,
+                                     int max_inner, int check_every, hmv_column_info *cols,
+                                     hmv_solve_info *info) {
+  const char *who = "hmv_block_bicgstab_ir";
+  if (!a64 || !alow || !b || !u || k < 1 || k > HMV_MAX_RHS || (k & (k - 1)) != 0 ||
+      max_outer < 1 || max_inner < 1 || a64->nd != alow->nd ||
+      !(inner_eps > 0.0 && inner_eps < 1.0)) {
+    hmv::set_error(std::string(who) + ": invalid argument (arenas of one matrix, k a power of "
+                   "two <= 128, 0 < inner_eps < 1)");
+    return HMV_ERR_ARG;
+  }
+  if (check_every < 1) check_every = 1;
+  int rc = hmv::ensure_tk(a64, k);
+  if (rc) return rc;
+  BlockWork w;  // inner solves over the low-precision arena
+  if ((rc = block_setup(alow, w, k, who))) return rc;
+  struct Buf {
+    double *r2 = nullptr, *d = nullptr;
+    int *stopped = nullptr;
+    ~Buf() {
+      (void)hipFree(r2);
+      (void)hipFree(d);
+      (void)hipFree(stopped);
+    }
+  } buf;
+  const size_t nbytes = sizeof(double) * (size_t)w.n;
+  hipError_t e;
+  if ((e = hipMalloc(&buf.r2, nbytes)) != hipSuccess || (e = hipMalloc(&buf.d, nbytes)) != hipSuccess ||
+      (e = hipMalloc(&buf.stopped, sizeof(int) * k)) != hipSuccess)
+    return hmv::hip_fail(e, "hmv_block_bicgstab_ir: workspace");
+  const int g = grid_of(w.n);
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> status(k, HMV_SOLVE_MAXITER), stopped(k, 0), steps(k, 0);
+  std::vector<double> rel(k, 0.0), now_rel;
+  std::vector<ColState> cs;
+  int outer = 0, inner_total = 0;
+  for (;;) {
+    // FP64 residual of every column with the FP64 arena
+    if ((rc = block_residual(a64, w, b, u, buf.r2, now_rel))) return rc;
+    bool all = true;
+    for (int j = 0; j < k; ++j) {
+      if (stopped[j]) continue;
+      rel[j] = now_rel[j];
+      if (!std::isfinite(rel[j]))
+        status[j] = HMV_SOLVE_NONFINITE;
+      else if (rel[j] < eps)
+        status[j] = HMV_SOLVE_CONVERGED;
+      else {
+        all = false;
+        continue;
+      }
+      stopped[j] = 1;
+    }
+    if (all || outer >= max_outer) break;
+    // inner solves A_low D = R on the columns still running (a zero column
+    // stops at its setup)
+    if ((e = hipMemcpyAsync(buf.stopped, stopped.data(), sizeof(int) * k, hipMemcpyHostToDevice,
+                            w.s)) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_block_bicgstab_ir: flags");
+    hipLaunchKernelGGL(kb_zero_cols, dim3(g), dim3(kT), 0, w.s, w.n, k, buf.r2, buf.stopped);
+    hipLaunchKernelGGL(kb_fill_n, dim3(g), dim3(kT), 0, w.s, w.n, buf.d, 0.0);
+    int iters = 0;
+    if ((rc = block_solve(alow, w, buf.r2, buf.d, inner_eps, max_inner, check_every, true, cs, iters,
+                          who)))
+      return rc;
+    inner_total += iters;
+    for (int j = 0; j < k; ++j) {
+      if (stopped[j]) continue;
+      steps[j] += cs[j].step;
+      if (cs[j].done && cs[j].status == HMV_SOLVE_NONFINITE) {
+        status[j] = HMV_SOLVE_NONFINITE;
+        stopped[j] = 1;
+      }
+    }
+    // U += D, with D = 0 in every stopped column
+    if ((e = hipMemcpyAsync(buf.stopped, stopped.data(), sizeof(int) * k, hipMemcpyHostToDevice,
+                            w.s)) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_block_bicgstab_ir: flags");
+    hipLaunchKernelGGL(kb_zero_cols, dim3(g), dim3(kT), 0, w.s, w.n, k, buf.d, buf.stopped);
+    hipLaunchKernelGGL(kb_add_n, dim3(g), dim3(kT), 0, w.s, w.n, u, buf.d);
+    if ((e = hipGetLastError()) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_block_bicgstab_ir: update");
+    ++outer;
+  }
+  if ((e = hipStreamSynchronize(w.s)) != hipSuccess)
+    return hmv::hip_fail(e, "hmv_block_bicgstab_ir: sync");
+  const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  int worst = HMV_SOLVE_CONVERGED;
+  double maxrel = 0.0;
+  for (int j = 0; j < k; ++j) {
+    if (cols) {
+      cols[j].nstp = steps[j];
+      cols[j].status = status[j];
+      cols[j].relres = rel[j];
+      cols[j].true_relres = rel[j];
+    }
+    if (std::isfinite(rel[j])) maxrel = std::max(maxrel, rel[j]);
+    if (status[j] != HMV_SOLVE_CONVERGED) worst = status[j];
+  }
+  if (info) {
+    info->nstp = inner_total;
+    info->status = worst;
+    info->relres = maxrel;
+    info->true_relres = maxrel;
+    info->seconds = total;
+    info->outer = outer;
+    info->inner_total = inner_total;
   }
   return HMV_OK;
 }
@@ -554,6 +726,7 @@ struct SegWork {
   int *flags = nullptr;  // per-instance done flags read by the H-matvec
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  const double *graph_u = nullptr;  // the iterate the captured graph updates
   ~SegWork() {
     (void)hipFree(flags);
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -566,6 +739,28 @@ struct SegWork {
     (void)hipFree(bs);
   }
 };
+
+__global__ __launch_bounds__(kT) void kb_fill(int64_t n, double *__restrict__ x, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    x[i] = v;
+}
+
+// u += d
+__global__ __launch_bounds__(kT) void kb_add(int64_t n, double *__restrict__ u,
+                                             const double *__restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT)
+    u[i] += d[i];
+}
+
+// r = 0 on the segments of stopped instances (their next inner solve then
+// stops at its setup and their products are skipped)
+__global__ __launch_bounds__(kT) void ks_zero_stopped(const SegBlock *__restrict__ tb,
+                                                      double *__restrict__ r,
+                                                      const int *__restrict__ stopped) {
+  const SegBlock B = tb[blockIdx.x];
+  if (!stopped[B.seg]) return;
+  for (int i = B.beg + threadIdx.x; i < B.end; i += kT) r[i] = 0.0;
+}
 
 hipError_t seg_iteration(hmv_arena_s *a, SegWork &w, double *u) {
   const int nb = w.nblk;
@@ -591,23 +786,15 @@ hipError_t seg_iteration(hmv_arena_s *a, SegWork &w, double *u) {
   return hipGetLastError();
 }
 
-}  // namespace
-
-extern "C" int hmv_multi_bicgstab(hmv_arena a, const double *b, double *u, double eps, int maxit,
-                                  int check_every, hmv_column_info *inst, double *inst_seconds,
-                                  hmv_solve_info *info) {
-  if (!a || !b || !u || maxit < 0 || a->seg.size() < 2) {
-    hmv::set_error("hmv_multi_bicgstab: invalid argument");
-    return HMV_ERR_ARG;
-  }
-  if (check_every < 1) check_every = 1;
-  SegWork w;
+// Tables and buffers of the per-instance solver over arena a's segments:
+// a block table whose blocks hold at most 16 elements per thread and never
+// cross an instance; bstart[s] = first block of instance s.
+int seg_setup(hmv_arena_s *a, SegWork &w, std::vector<int> &bstart, const char *who) {
   w.nseg = (int)a->seg.size() - 1;
   w.n = a->nd;
-  // block table: chunks of at most 16 elements per thread, never across instances
   constexpr int kChunk = 16 * kT;
   std::vector<SegBlock> tb;
-  std::vector<int> bstart(w.nseg + 1, 0);
+  bstart.assign(w.nseg + 1, 0);
   for (int sgi = 0; sgi < w.nseg; ++sgi) {
     bstart[sgi] = (int)tb.size();
     for (int i = a->seg[sgi]; i < a->seg[sgi + 1]; i += kChunk)
@@ -631,78 +818,146 @@ extern "C" int hmv_multi_bicgstab(hmv_arena a, const double *b, double *u, doubl
           hipSuccess ||
       (e = hipMemcpy(w.bstart, bstart.data(), sizeof(int) * bstart.size(), hipMemcpyHostToDevice)) !=
           hipSuccess)
-    return hmv::hip_fail(e, "hmv_multi_bicgstab: workspace");
-  auto t0 = std::chrono::steady_clock::now();
+    return hmv::hip_fail(e, (std::string(who) + ": workspace").c_str());
+  return HMV_OK;
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// One BiCGSTAB per instance from u (zero_guess: u = 0, and r = b without a
+// product), in lockstep until every instance has stopped or maxit
+// iterations ran.  cs receives the instances' final states; done_at (may be
+// null) the time since t0 at which each instance was first seen stopped;
+// iters the iterations run.  The iteration graph is captured on the first
+// call with a given u and reused by later calls.
+int seg_solve(hmv_arena_s *a, SegWork &w, const double *b, double *u, double eps, int maxit,
+              int check_every, bool zero_guess, std::chrono::steady_clock::time_point t0,
+              std::vector<ColState> &cs, std::vector<double> *done_at, int &iters,
+              const char *who) {
+  const std::string name(who);
   BlockState init{};
   init.eps = eps;
   init.maxit = maxit;
   const int g = grid_of(w.n);
-  if ((e = hipMemcpyAsync(w.bs, &init, sizeof(init), hipMemcpyHostToDevice, w.s)) != hipSuccess ||
-      (e = hmv::enqueue_apply(a, u, w.r, 0.0, nullptr, w.s)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_multi_bicgstab: setup");
+  hipError_t e = hipMemcpyAsync(w.bs, &init, sizeof(init), hipMemcpyHostToDevice, w.s);
+  if (e != hipSuccess) return hmv::hip_fail(e, (name + ": setup").c_str());
+  if (zero_guess)
+    hipLaunchKernelGGL(kb_fill, dim3(g), dim3(kT), 0, w.s, w.n, w.r, 0.0);
+  else if ((e = hmv::enqueue_apply(a, u, w.r, 0.0, nullptr, w.s)) != hipSuccess)
+    return hmv::hip_fail(e, (name + ": setup").c_str());
   hipLaunchKernelGGL(kb_init, dim3(g), dim3(kT), 0, w.s, w.n, b, w.r, w.rhat, w.p, w.v);
   hipLaunchKernelGGL(ks_dots, dim3(w.nblk), dim3(kT), 0, w.s, w.tb, w.r, w.r, b, b, w.part,
                      (const BlockState *)nullptr);
   hipLaunchKernelGGL(ks_final, dim3(1), dim3(kT), 0, w.s, w.nseg, w.bstart, w.part, w.cs, w.bs, 0,
                      w.flags);
-  if ((e = hipStreamSynchronize(w.s)) != hipSuccess ||
-      (e = hipStreamBeginCapture(w.s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_multi_bicgstab: capture");
-  hipError_t ec = seg_iteration(a, w, u);
-  e = hipStreamEndCapture(w.s, &w.graph);
-  if (ec != hipSuccess) return hmv::hip_fail(ec, "hmv_multi_bicgstab: capture");
-  if (e != hipSuccess || (e = hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_multi_bicgstab: graph");
-  // per-instance completion: the wall time of the burst in which it stopped
-  std::vector<ColState> cs(w.nseg);
-  std::vector<double> done_at(w.nseg, -1.0);
-  auto poll = [&](double now) -> hipError_t {
+  if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(w.s)) != hipSuccess)
+    return hmv::hip_fail(e, (name + ": setup").c_str());
+  if (!w.exec || w.graph_u != u) {
+    if (w.exec) (void)hipGraphExecDestroy(w.exec);
+    if (w.graph) (void)hipGraphDestroy(w.graph);
+    w.exec = nullptr;
+    w.graph = nullptr;
+    if ((e = hipStreamBeginCapture(w.s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hmv::hip_fail(e, (name + ": capture").c_str());
+    hipError_t ec = seg_iteration(a, w, u);
+    e = hipStreamEndCapture(w.s, &w.graph);
+    if (ec != hipSuccess) return hmv::hip_fail(ec, (name + ": capture").c_str());
+    if (e != hipSuccess ||
+        (e = hipGraphInstantiate(&w.exec, w.graph, nullptr, nullptr, 0)) != hipSuccess)
+      return hmv::hip_fail(e, (name + ": graph").c_str());
+    w.graph_u = u;
+  }
+  cs.resize(w.nseg);
+  auto poll = [&]() -> hipError_t {
     hipError_t pe = hipMemcpyAsync(cs.data(), w.cs, sizeof(ColState) * w.nseg,
                                    hipMemcpyDeviceToHost, w.s);
     if (pe == hipSuccess) pe = hipStreamSynchronize(w.s);
-    if (pe == hipSuccess)
+    if (pe == hipSuccess && done_at) {
+      const double now = seconds_since(t0);
       for (int sgi = 0; sgi < w.nseg; ++sgi)
-        if (cs[sgi].done && done_at[sgi] < 0.0) done_at[sgi] = now;
+        if (cs[sgi].done && (*done_at)[sgi] < 0.0) (*done_at)[sgi] = now;
+    }
     return pe;
   };
-  auto secs = [&]() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  };
-  if ((e = poll(secs())) != hipSuccess) return hmv::hip_fail(e, "hmv_multi_bicgstab: state");
+  if ((e = poll()) != hipSuccess) return hmv::hip_fail(e, (name + ": state").c_str());
   BlockState host{};
   int launched = 0;
   while (launched < maxit) {
     const int burst = std::min(check_every, maxit - launched);
     for (int i = 0; i < burst; ++i)
       if ((e = hipGraphLaunch(w.exec, w.s)) != hipSuccess)
-        return hmv::hip_fail(e, "hmv_multi_bicgstab: launch");
+        return hmv::hip_fail(e, (name + ": launch").c_str());
     launched += burst;
     if ((e = hipMemcpyAsync(&host, w.bs, sizeof(host), hipMemcpyDeviceToHost, w.s)) != hipSuccess ||
-        (e = poll(secs())) != hipSuccess)
-      return hmv::hip_fail(e, "hmv_multi_bicgstab: state");
+        (e = poll()) != hipSuccess)
+      return hmv::hip_fail(e, (name + ": state").c_str());
     if (host.all_done) break;
   }
-  const double total = secs();
-  // true residuals per instance: r = b - A u
-  if ((e = hmv::enqueue_apply(a, u, w.r, 0.0, nullptr, w.s)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_multi_bicgstab: true residual");
-  hipLaunchKernelGGL(kb_residual, dim3(g), dim3(kT), 0, w.s, w.n, b, w.r);
-  hipLaunchKernelGGL(ks_dots, dim3(w.nblk), dim3(kT), 0, w.s, w.tb, w.r, w.r, b, b, w.part,
+  if ((e = hipMemcpyAsync(&host, w.bs, sizeof(host), hipMemcpyDeviceToHost, w.s)) != hipSuccess ||
+      (e = hipStreamSynchronize(w.s)) != hipSuccess)
+    return hmv::hip_fail(e, (name + ": state").c_str());
+  iters = host.iter;
+  return HMV_OK;
+}
+
+// Per-instance ||b - A u||^2 and ||b||^2 (r receives b - A u), each summed
+// over the instance's blocks in block order.
+int seg_residuals(hmv_arena_s *a, SegWork &w, const double *b, const double *u, double *r,
+                  const std::vector<int> &bstart, std::vector<double> &rr, std::vector<double> &bb,
+                  const char *who) {
+  const int g = grid_of(w.n);
+  hipError_t e = hmv::enqueue_apply(a, u, r, 0.0, nullptr, w.s);
+  if (e != hipSuccess) return hmv::hip_fail(e, (std::string(who) + ": residual").c_str());
+  hipLaunchKernelGGL(kb_residual, dim3(g), dim3(kT), 0, w.s, w.n, b, r);
+  hipLaunchKernelGGL(ks_dots, dim3(w.nblk), dim3(kT), 0, w.s, w.tb, r, r, b, b, w.part,
                      (const BlockState *)nullptr);
-  std::vector<double> part(2 * tb.size());
-  if ((e = hipMemcpyAsync(part.data(), w.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+  std::vector<double> part(2 * (size_t)w.nblk);
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(part.data(), w.part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
                           w.s)) != hipSuccess ||
       (e = hipStreamSynchronize(w.s)) != hipSuccess)
-    return hmv::hip_fail(e, "hmv_multi_bicgstab: true residual");
+    return hmv::hip_fail(e, (std::string(who) + ": residual").c_str());
+  rr.assign(w.nseg, 0.0);
+  bb.assign(w.nseg, 0.0);
+  for (int sgi = 0; sgi < w.nseg; ++sgi)
+    for (int bi = bstart[sgi]; bi < bstart[sgi + 1]; ++bi) {
+      rr[sgi] += part[2 * bi];
+      bb[sgi] += part[2 * bi + 1];
+    }
+  return HMV_OK;
+}
+
+}  // namespace
+
+extern "C" int hmv_multi_bicgstab(hmv_arena a, const double *b, double *u, double eps, int maxit,
+                                  int check_every, hmv_column_info *inst, double *inst_seconds,
+                                  hmv_solve_info *info) {
+  if (!a || !b || !u || maxit < 0 || a->seg.size() < 2) {
+    hmv::set_error("hmv_multi_bicgstab: invalid argument");
+    return HMV_ERR_ARG;
+  }
+  if (check_every < 1) check_every = 1;
+  const char *who = "hmv_multi_bicgstab";
+  SegWork w;
+  std::vector<int> bstart;
+  int rc = seg_setup(a, w, bstart, who);
+  if (rc) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<ColState> cs;
+  std::vector<double> done_at(w.nseg, -1.0);
+  int iters = 0;
+  if ((rc = seg_solve(a, w, b, u, eps, maxit, check_every, false, t0, cs, &done_at, iters, who)))
+    return rc;
+  const double total = seconds_since(t0);
+  // true residuals per instance: r = b - A u
+  std::vector<double> rr, bb;
+  if ((rc = seg_residuals(a, w, b, u, w.r, bstart, rr, bb, who))) return rc;
   int maxstep = 0, worst = HMV_SOLVE_CONVERGED;
   double maxrel = 0.0;
   for (int sgi = 0; sgi < w.nseg; ++sgi) {
-    double rr = 0.0, bb = 0.0;
-    for (int bi = bstart[sgi]; bi < bstart[sgi + 1]; ++bi) {
-      rr += part[2 * bi];
-      bb += part[2 * bi + 1];
-    }
-    const double rel = std::sqrt(rr) / std::sqrt(bb);
+    const double rel = std::sqrt(rr[sgi]) / std::sqrt(bb[sgi]);
     const int status = cs[sgi].done ? cs[sgi].status : HMV_SOLVE_MAXITER;
     if (inst) {
       inst[sgi].nstp = cs[sgi].step;
@@ -716,13 +971,130 @@ extern "C" int hmv_multi_bicgstab(hmv_arena a, const double *b, double *u, doubl
     if (status != HMV_SOLVE_CONVERGED) worst = status;
   }
   if (info) {
-    info->nstp = host.iter;
+    info->nstp = iters;
     info->status = worst;
     info->relres = maxrel;
     info->true_relres = maxrel;
     info->seconds = total;
     info->outer = 0;
     info->inner_total = maxstep;
+  }
+  return HMV_OK;
+}
+
+extern "C" int hmv_multi_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u,
+                                     double eps, double inner_eps, int max_outer, int max_inner,
+                                     int check_every, hmv_column_info *inst,
+                                     double *inst_seconds, hmv_solve_info *info) {
+  const char *who = "hmv_multi_bicgstab_ir";
+  if (!a64 || !alow || !b || !u || max_outer < 1 || max_inner < 1 || a64->seg.size() < 2 ||
+      a64->seg != alow->seg || a64->nd != alow->nd || !(inner_eps > 0.0 && inner_eps < 1.0)) {
+    hmv::set_error(std::string(who) + ": invalid argument (two arenas of the same instances, "
+                   "0 < inner_eps < 1)");
+    return HMV_ERR_ARG;
+  }
+  if (check_every < 1) check_every = 1;
+  SegWork w;  // inner solves over the low-precision arena
+  std::vector<int> bstart;
+  int rc = seg_setup(alow, w, bstart, who);
+  if (rc) return rc;
+  const int nseg = w.nseg;
+  struct Buf {
+    double *r2 = nullptr, *d = nullptr;
+    int *stopped = nullptr;
+    ~Buf() {
+      (void)hipFree(r2);
+      (void)hipFree(d);
+      (void)hipFree(stopped);
+    }
+  } buf;
+  const size_t nbytes = sizeof(double) * (size_t)w.n;
+  hipError_t e;
+  if ((e = hipMalloc(&buf.r2, nbytes)) != hipSuccess || (e = hipMalloc(&buf.d, nbytes)) != hipSuccess ||
+      (e = hipMalloc(&buf.stopped, sizeof(int) * nseg)) != hipSuccess)
+    return hmv::hip_fail(e, "hmv_multi_bicgstab_ir: workspace");
+  const int g = grid_of(w.n);
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> status(nseg, HMV_SOLVE_MAXITER), stopped(nseg, 0), steps(nseg, 0);
+  std::vector<double> rel(nseg, 0.0), done_at(nseg, -1.0), rr, bb;
+  std::vector<ColState> cs;
+  int outer = 0, inner_total = 0;
+  for (;;) {
+    // FP64 residual of every instance on the FP64 arena
+    if ((rc = seg_residuals(a64, w, b, u, buf.r2, bstart, rr, bb, who))) return rc;
+    const double now = seconds_since(t0);
+    bool all = true;
+    for (int s = 0; s < nseg; ++s) {
+      if (stopped[s]) continue;
+      rel[s] = std::sqrt(rr[s]) / std::sqrt(bb[s]);
+      if (!std::isfinite(rel[s]))
+        status[s] = HMV_SOLVE_NONFINITE;
+      else if (rel[s] < eps)
+        status[s] = HMV_SOLVE_CONVERGED;
+      else {
+        all = false;
+        continue;
+      }
+      stopped[s] = 1;
+      done_at[s] = now;
+    }
+    if (all || outer >= max_outer) break;
+    // inner solves A_low d = r on the instances still running
+    if ((e = hipMemcpyAsync(buf.stopped, stopped.data(), sizeof(int) * nseg, hipMemcpyHostToDevice,
+                            w.s)) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_multi_bicgstab_ir: flags");
+    hipLaunchKernelGGL(ks_zero_stopped, dim3(w.nblk), dim3(kT), 0, w.s, w.tb, buf.r2, buf.stopped);
+    hipLaunchKernelGGL(kb_fill, dim3(g), dim3(kT), 0, w.s, w.n, buf.d, 0.0);
+    int iters = 0;
+    if ((rc = seg_solve(alow, w, buf.r2, buf.d, inner_eps, max_inner, check_every, true, t0, cs,
+                        nullptr, iters, who)))
+      return rc;
+    inner_total += iters;
+    for (int s = 0; s < nseg; ++s) {
+      if (stopped[s]) continue;
+      steps[s] += cs[s].step;
+      if (cs[s].done && cs[s].status == HMV_SOLVE_NONFINITE) {
+        // the correction of this instance is not usable: it stops here
+        status[s] = HMV_SOLVE_NONFINITE;
+        stopped[s] = 1;
+        done_at[s] = seconds_since(t0);
+      }
+    }
+    // u += d, with d = 0 on every stopped instance (those stopped before this
+    // solve never updated d; one stopped just now may hold a non-finite d)
+    if ((e = hipMemcpyAsync(buf.stopped, stopped.data(), sizeof(int) * nseg, hipMemcpyHostToDevice,
+                            w.s)) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_multi_bicgstab_ir: flags");
+    hipLaunchKernelGGL(ks_zero_stopped, dim3(w.nblk), dim3(kT), 0, w.s, w.tb, buf.d, buf.stopped);
+    hipLaunchKernelGGL(kb_add, dim3(g), dim3(kT), 0, w.s, w.n, u, buf.d);
+    if ((e = hipGetLastError()) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_multi_bicgstab_ir: update");
+    ++outer;
+  }
+  if ((e = hipStreamSynchronize(w.s)) != hipSuccess)
+    return hmv::hip_fail(e, "hmv_multi_bicgstab_ir: sync");
+  const double total = seconds_since(t0);
+  int worst = HMV_SOLVE_CONVERGED;
+  double maxrel = 0.0;
+  for (int s = 0; s < nseg; ++s) {
+    if (inst) {
+      inst[s].nstp = steps[s];
+      inst[s].status = status[s];
+      inst[s].relres = rel[s];
+      inst[s].true_relres = rel[s];
+    }
+    if (inst_seconds) inst_seconds[s] = done_at[s] >= 0.0 ? done_at[s] : total;
+    if (std::isfinite(rel[s])) maxrel = std::max(maxrel, rel[s]);
+    if (status[s] != HMV_SOLVE_CONVERGED) worst = status[s];
+  }
+  if (info) {
+    info->nstp = inner_total;
+    info->status = worst;
+    info->relres = maxrel;
+    info->true_relres = maxrel;
+    info->seconds = total;
+    info->outer = outer;
+    info->inner_total = inner_total;
   }
   return HMV_OK;
 }

@@ -126,6 +126,13 @@ int hmv_autotune(int nd, int nlf, const void *leaves, int leaf_stride, int preci
 int hmv_arena_build_multi(int ninst, const int *nd, const int *nlf, const void *const *leaves,
                           int leaf_stride, int precision, const hmv_tuning *tuning,
                           hmv_arena *out);
+/* Autotuner over a multi-instance arena: as hmv_autotune, each candidate
+ * built as hmv_arena_build_multi builds it (one launch per phase over every
+ * instance) and timed on the whole block-diagonal apply.  No reference
+ * counterpart. */
+int hmv_autotune_multi(int ninst, const int *nd, const int *nlf, const void *const *leaves,
+                       int leaf_stride, int precision, int reps, hmv_tuning *best, double *best_ms,
+                       double *default_ms);
 /* Number of instances of an arena (1 unless built by hmv_arena_build_multi);
  * offsets (may be NULL) receives up to cap of the ninst+1 row offsets. */
 int hmv_arena_segments(hmv_arena a, int *offsets, int cap);
@@ -257,6 +264,20 @@ typedef struct {
 int hmv_block_bicgstab(hmv_arena a, int k, const double *b, double *u, double eps, int maxit,
                        int check_every, hmv_column_info *cols, hmv_solve_info *info);
 
+/* Mixed-precision refinement of a block solve: a64 and alow are FP64 and
+ * low-precision arenas of one H-matrix.  Each outer step computes every
+ * running column's FP64 residual R = B - A64 U with an FP64 H-matmat; a
+ * column with ||r_j||/||b_j|| < eps stops (converged); the others solve
+ * A_low D = R by block BiCGSTAB (hmv_block_bicgstab's iteration, each
+ * column to inner_eps, at most max_inner iterations) and take U += D.  At
+ * most max_outer steps.  cols receives per column the inner iterations
+ * summed over the outer steps, the status and the final FP64 relative
+ * residual; info the outer steps, the inner iterations summed, the worst
+ * status and residual and the wall time.  No reference counterpart. */
+int hmv_block_bicgstab_ir(hmv_arena a64, hmv_arena alow, int k, const double *b, double *u,
+                          double eps, double inner_eps, int max_outer, int max_inner,
+                          int check_every, hmv_column_info *cols, hmv_solve_info *info);
+
 /* One BiCGSTAB per instance of a multi-instance arena (hmv_arena_build_multi),
  * all advancing together: one H-matvec launch per phase covers every
  * instance, scalars and stopping are per instance (a finished instance is
@@ -268,6 +289,24 @@ int hmv_block_bicgstab(hmv_arena a, int k, const double *b, double *u, double ep
 int hmv_multi_bicgstab(hmv_arena a, const double *b, double *u, double eps, int maxit,
                        int check_every, hmv_column_info *inst, double *inst_seconds,
                        hmv_solve_info *info);
+
+/* Mixed-precision refinement per instance: a64 and alow are FP64 and
+ * low-precision arenas of the same instances (hmv_arena_build_multi with
+ * the same leaves).  Each outer step computes every running instance's FP64
+ * residual r_i = b_i - A64_i u_i on a64; an instance with
+ * ||r_i||/||b_i|| < eps stops (converged); the others solve A_low d = r by
+ * one BiCGSTAB per instance in lockstep (hmv_multi_bicgstab's iteration,
+ * one hipGraph per iteration, each instance to inner_eps, at most
+ * max_inner iterations; stopped instances skip their products) and take
+ * u += d.  At most max_outer steps.  inst receives per instance the inner
+ * iterations summed over the outer steps, the status and the final FP64
+ * relative residual; inst_seconds the time at which each instance was seen
+ * converged; info the outer steps, the inner iterations summed over the
+ * outer steps, the worst status and residual and the wall time.  No
+ * reference counterpart. */
+int hmv_multi_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
+                          double inner_eps, int max_outer, int max_inner, int check_every,
+                          hmv_column_info *inst, double *inst_seconds, hmv_solve_info *info);
 
 const char *hmv_last_error(void);
 

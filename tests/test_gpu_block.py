@@ -96,3 +96,34 @@ def test_block_rejects_bad_k(gpu, sphere3):
     with pytest.raises(gpu.HmvError):
         a.matmat(np.ones((sphere3.nd, 129)))
     a.free()
+
+
+@pytest.mark.parametrize("prec,inner", [("f32", 1e-4), ("bf16", 1e-2)])
+def test_block_refinement(gpu, O, sphere4, prec, inner):
+    """hmv_block_bicgstab_ir: FP64 block residuals on the FP64 arena, inner
+    block BiCGSTAB on the low-precision arena; every column reaches a true
+    1e-8 (recomputed by the oracle) and the single FP64 solve's solution
+    within cond * 2e-8 (cond = 2 sqrt(N) + 1, the sphere single-layer
+    operator's eigenvalue spread); a zero column stops at once."""
+    hm = sphere4
+    a64, alow = gpu.Arena(hm, "f64"), gpu.Arena(hm, prec)
+    k = 8
+    B = np.stack([gpu.uniform(700 + j, hm.nd) for j in range(k)], axis=1)
+    B[:, 1] = 1.0
+    B[:, 5] = 0.0
+    U, cols, info = gpu.block_bicgstab_ir(a64, alow, B, eps=1e-8, inner_eps=inner, max_outer=30,
+                                          max_inner=500)
+    print(prec, info, [(c["nstp"], "%.1e" % c["true_relres"]) for c in cols], flush=True)
+    assert info["outer"] >= 2
+    assert cols[5]["status_name"] == "nonfinite" and cols[5]["nstp"] == 0
+    assert np.all(U[:, 5] == 0.0)
+    bound = (2.0 * np.sqrt(hm.nd) + 1.0) * 2e-8
+    for j in range(k):
+        if j == 5:
+            continue
+        tr = _rel(O.adot_body_lfmtx(hm, U[:, j]), B[:, j])
+        assert cols[j]["status_name"] == "converged" and tr < 1e-8
+        u1, i1, _ = a64.bicgstab(B[:, j], eps=1e-8, maxit=500)
+        assert _rel(U[:, j], u1) < bound
+    a64.free()
+    alow.free()

@@ -94,3 +94,66 @@ def test_multi_low_precision_apply(gpu, O, instances):
         lo, hi = m.offsets[i], m.offsets[i + 1]
         assert _rel(y[lo:hi], O.adot_body_lfmtx(hm, x[lo:hi])) <= 2e-2
     m.free()
+
+
+def _cond_bound(nd):
+    """cond * 2e-8 with cond = 2 sqrt(N) + 1, the spread of the sphere
+    single-layer operator's eigenvalues R/(2l+1) over (L+1)^2 ~ N modes."""
+    return (2.0 * np.sqrt(nd) + 1.0) * 2e-8
+
+
+@pytest.mark.parametrize("prec,inner", [("f32", 1e-4), ("bf16", 1e-2), ("bf16_all", 1e-2)])
+def test_multi_refinement_per_instance(gpu, O, instances, prec, inner):
+    """hmv_multi_bicgstab_ir: FP64 residuals on the FP64 arena, inner
+    per-instance BiCGSTAB on the low-precision arena.  Every instance reaches
+    a true 1e-8 (recomputed by the oracle) and its FP64 solution within the
+    conditioning bound; a zero right-hand side stops its instance at once
+    (non-finite relative residual) and leaves its iterate at zero."""
+    m64, mlow = gpu.MultiArena(instances, "f64"), gpu.MultiArena(instances, prec)
+    b = np.concatenate([np.ones(hm.nd) if i % 2 else gpu.uniform(40 + i, hm.nd)
+                        for i, hm in enumerate(instances)])
+    u, inst, secs, info = gpu.multi_bicgstab_ir(m64, mlow, b, eps=1e-8, inner_eps=inner,
+                                                max_outer=30, max_inner=500)
+    print(prec, info, [(d["nstp"], "%.1e" % d["true_relres"]) for d in inst], flush=True)
+    assert info["status_name"] == "converged" and info["outer"] >= 2
+    assert np.all(secs > 0) and np.all(secs <= info["seconds"] + 1e-9)
+    for i, hm in enumerate(instances):
+        lo, hi = m64.offsets[i], m64.offsets[i + 1]
+        tr = _rel(O.adot_body_lfmtx(hm, u[lo:hi]), b[lo:hi])
+        assert inst[i]["status_name"] == "converged" and tr < 1e-8
+        assert abs(tr - inst[i]["true_relres"]) <= 1e-3 * tr
+        u_ref, _, _ = O.bicgstab(hm, b[lo:hi], None, mstep=500, eps=1e-10)
+        assert _rel(u[lo:hi], u_ref) < _cond_bound(hm.nd)
+    b[m64.offsets[1]:m64.offsets[2]] = 0.0
+    u, inst, secs, info = gpu.multi_bicgstab_ir(m64, mlow, b, eps=1e-8, inner_eps=inner)
+    assert inst[1]["status_name"] == "nonfinite" and inst[1]["nstp"] == 0
+    assert np.all(u[m64.offsets[1]:m64.offsets[2]] == 0.0)
+    assert all(inst[i]["status_name"] == "converged" for i in (0, 2, 3, 4))
+    m64.free()
+    mlow.free()
+
+
+def test_multi_refinement_rejects_other_instances(gpu, instances):
+    m64 = gpu.MultiArena(instances, "f64")
+    other = gpu.MultiArena(instances[:3], "bf16")
+    with pytest.raises(gpu.HmvError, match="same instances"):
+        gpu.multi_bicgstab_ir(m64, other, np.ones(m64.nd))
+    m64.free()
+    other.free()
+
+
+def test_multi_autotune(gpu, O, instances):
+    """hmv_autotune_multi returns a tuning no slower than the defaults on the
+    whole block-diagonal apply; a multi arena built with it matches the
+    oracle per instance."""
+    best, best_ms, dflt_ms = gpu.autotune_multi(instances, "bf16", reps=5)
+    print("multi autotune", best, "%.4f ms (defaults %.4f ms)" % (best_ms, dflt_ms), flush=True)
+    assert 0 < best_ms <= dflt_ms
+    m = gpu.MultiArena(instances, "bf16", tuning=best)
+    assert m.tuning == best
+    x = gpu.uniform(9, m.nd)
+    y = m.matvec(x)
+    for i, hm in enumerate(instances):
+        lo, hi = m.offsets[i], m.offsets[i + 1]
+        assert _rel(y[lo:hi], O.adot_body_lfmtx(hm, x[lo:hi])) <= 2e-2
+    m.free()
