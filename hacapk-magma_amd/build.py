@@ -37,7 +37,8 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-march=x86-64-v3", "-ffp-contract=off"]
 
 HMV_SOURCES = ["hmv_plan.cpp", "hmv_api.cpp", "hmv_file.cpp", "hmv_kernels.hip", "hmv_matmat.hip", "hmv_solver.hip",
-               "hmv_block_solver.hip", "hmv_gcr.hip", "hmv_pipe.hip", "hmv_construct.hip"]
+               "hmv_block_solver.hip", "hmv_gcr.hip", "hmv_pipe.hip", "hmv_construct.hip",
+               "hmv_kstream.hip"]
 
 
 def run(cmd):

@@ -88,6 +88,17 @@ extern "C" hipError_t hmv_launch_stream_test(int kind, const void *src, void *ds
 extern "C" hipError_t hmv_launch_apply(const HmvLaunch *L, int phases, const double *x, double *y,
                                        double beta, const int *skip, const int *inst_done,
                                        hipStream_t stream);
+// H-matmats with at most this many right-hand sides run on the vector units
+// (hmv_kstream.hip), larger ones on MFMA (hmv_matmat.hip): measured at
+// 327,680 unknowns, the vector kernel is the faster one up to k = 8 with
+// FP64 factors and up to k = 4 with FP32 / BF16 factors (at k = 8 their
+// MFMA kernel, one 16-wide tile of right-hand sides, takes 0.92 ms against
+// 1.25 ms).
+#define HMV_KSTREAM_MAX_F64 8
+#define HMV_KSTREAM_MAX_NARROW 4
+extern "C" hipError_t hmv_launch_kstream(const HmvLaunch *L, int phase, int k, const double *x,
+                                         double *t, double *y, double beta, const int *skip,
+                                         hipStream_t stream);
 extern "C" hipError_t hmv_launch_block(const HmvLaunch *L, const HmvUnits *units, int k,
                                        const double *x, double *t, double *y, double beta,
                                        const int *skip, hipStream_t stream);

@@ -400,13 +400,21 @@ template <typename Acc, int DT0, int DT1>
 void launch_block(const HmvLaunch &L, const HmvUnits &U, int k, const double *x, double *t,
                   double *y, double beta, const int *skip, hipStream_t s) {
   const int kt = (k + 15) / 16;
-  launch_units<Acc, 1, DT1, DT1>(kt, L, U, L.work1, x, t, nullptr, k, 0.0, skip, s);
+  // few right-hand sides stream on the vector units (hmv_kstream.hip)
+  const bool vec = k <= (L.mode == 0 ? HMV_KSTREAM_MAX_F64 : HMV_KSTREAM_MAX_NARROW);
+  if (vec)
+    (void)hmv_launch_kstream(&L, 1, k, x, t, nullptr, 0.0, skip, s);
+  else
+    launch_units<Acc, 1, DT1, DT1>(kt, L, U, L.work1, x, t, nullptr, k, 0.0, skip, s);
   if (L.ntfin > 0) {
     const int64_t n = (int64_t)L.ntfin * k;
     hipLaunchKernelGGL(hmv_tfinal_block_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        L.tfin, L.ntfin, k, t, skip);
   }
-  launch_units<Acc, 2, DT0, DT1>(kt, L, U, L.work2, x, t, y, k, beta, skip, s);
+  if (vec)
+    (void)hmv_launch_kstream(&L, 2, k, x, t, y, beta, skip, s);
+  else
+    launch_units<Acc, 2, DT0, DT1>(kt, L, U, L.work2, x, t, y, k, beta, skip, s);
 }
 
 }  // namespace

@@ -1,4 +1,5 @@
-"""Multi-right-hand-side H-matmat (MFMA) and block BiCGSTAB through the C ABI.
+"""Multi-right-hand-side H-matmat (vector units for few right-hand sides,
+MFMA for more) and block BiCGSTAB through the C ABI.
 
 Bar: every column of the H-matmat equals the single-vector apply of that
 column (FP64: <= 1e-13 relative, FP32 / BF16 arenas: <= 1e-5 relative, both
@@ -32,8 +33,11 @@ def test_matmat_f64_columns(gpu, O, two_spheres4, k):
     a.free()
 
 
-@pytest.mark.parametrize("prec,k", [("f32", 8), ("f32", 32), ("bf16", 16), ("bf16_all", 8)])
+@pytest.mark.parametrize("prec,k", [("f32", 8), ("f32", 32), ("bf16", 16), ("bf16_all", 8),
+                                    ("f32", 4), ("bf16", 2), ("bf16_all", 3)])
 def test_matmat_low_precision(gpu, O, two_spheres4, prec, k):
+    """Narrow arenas: k <= 4 runs on the vector units (hmv_kstream.hip),
+    larger k on FP32 MFMAs."""
     hm = two_spheres4
     a = gpu.Arena(hm, prec)
     X = np.stack([gpu.uniform(200 + j, hm.nd) for j in range(k)], axis=1)
