@@ -98,9 +98,13 @@ struct Entry<HMV_DT_BF16> {
 // quads per prefetch group by RHS tiles and active tiles (two groups of
 // U * (NT + KT) loads per lane are live; measured on MI355X, deeper or
 // wider groups do not gain).
-template <int KT>
+// At KT = 8 (k > 64) two 16-row tiles share each B fragment with FP32
+// MFMAs (k = 128: 4.14 against 4.52 ms at 327,680 unknowns), while FP64
+// accumulators (twice the registers) are faster one tile per wave (5.2
+// against 5.9 ms).
+template <int KT, typename Acc>
 struct Shape {
-  static constexpr int TW = KT <= 1 ? 4 : KT <= 4 ? 2 : 1;
+  static constexpr int TW = KT <= 1 ? 4 : KT <= 4 ? 2 : sizeof(Acc) == 4 ? 2 : 1;
   static constexpr int LOG_TW = TW == 4 ? 2 : TW == 2 ? 1 : 0;
 };
 template <int KT, int NT>
@@ -285,7 +289,7 @@ __global__ __launch_bounds__(64) void hmv_matmat_kernel(
     const double *__restrict__ t_in, const int *__restrict__ outidx, double *__restrict__ t_out,
     double *__restrict__ y, int k, double beta, const int *__restrict__ skip) {
   if (skip && *skip) return;
-  constexpr int TW = Shape<KT>::TW;
+  constexpr int TW = Shape<KT, Acc>::TW;
   __shared__ int runs[2 * HMV_RUN_MAX];
   __shared__ gdouble *tbl[PHASE == 1 ? 1 : kWin + kPad];  // phase 1 reads its x run directly
   const int2 unit = units[blockIdx.x];
@@ -376,7 +380,7 @@ void launch_units(int kt, const HmvLaunch &L, const HmvUnits &U, const HmvWork *
                      y, k, beta, skip)
 #define HMV_MM_LAUNCH(KT)                              \
   do {                                                 \
-    const int w = Shape<KT>::LOG_TW, n = U.n[PHASE - 1][w]; \
+    const int w = Shape<KT, Acc>::LOG_TW, n = U.n[PHASE - 1][w]; \
     if (n > 0) {                                       \
       if (k == 16 * KT)                                \
         HMV_MM_GO(KT, true);                           \

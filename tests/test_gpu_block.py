@@ -34,10 +34,11 @@ def test_matmat_f64_columns(gpu, O, two_spheres4, k):
 
 
 @pytest.mark.parametrize("prec,k", [("f32", 8), ("f32", 32), ("bf16", 16), ("bf16_all", 8),
-                                    ("f32", 4), ("bf16", 2), ("bf16_all", 3)])
+                                    ("f32", 4), ("bf16", 2), ("bf16_all", 3), ("f32", 128),
+                                    ("bf16_all", 72)])
 def test_matmat_low_precision(gpu, O, two_spheres4, prec, k):
     """Narrow arenas: k <= 4 runs on the vector units (hmv_kstream.hip),
-    larger k on FP32 MFMAs."""
+    larger k on FP32 MFMAs (two 16-row tiles per wave above k = 64)."""
     hm = two_spheres4
     a = gpu.Arena(hm, prec)
     X = np.stack([gpu.uniform(200 + j, hm.nd) for j in range(k)], axis=1)
