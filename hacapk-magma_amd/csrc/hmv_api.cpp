@@ -2,16 +2,21 @@
 // the reference-named entry points a Fortran driver links against.
 #include <hip/hip_runtime.h>
 
+#include <sched.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstddef>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "hmv.h"
@@ -28,6 +33,79 @@ int hip_fail(hipError_t e, const char *what) {
   set_error(std::string(what) + ": " + hipGetErrorString(e));
   return (e == hipErrorOutOfMemory) ? HMV_ERR_MEMORY : HMV_ERR_DEVICE;
 }
+
+// Host threads for the host-vector apply's copies.  The caller's arrays are
+// pageable, so each vector crosses the CPU once (into or out of pinned
+// staging); the copy is split over up to 8 threads (half the CPUs this
+// process may use).  The workers sleep between calls and are joined at exit.
+class HostPool {
+ public:
+  static HostPool &get() {
+    static HostPool pool;
+    return pool;
+  }
+  // fn(lo, hi) over [0, n) in nparts() contiguous parts; the caller runs
+  // the first part itself.
+  void parallel_for(size_t n, const std::function<void(size_t, size_t)> &fn) {
+    const size_t parts = th_.size() + 1;
+    if (parts == 1 || n < 4096) {
+      fn(0, n);
+      return;
+    }
+    std::lock_guard<std::mutex> one_caller(call_mu_);
+    std::unique_lock<std::mutex> lk(mu_);
+    fn_ = &fn;
+    n_ = n;
+    pending_ = (int)th_.size();
+    ++gen_;
+    lk.unlock();
+    cv_.notify_all();
+    fn(0, n / parts);
+    lk.lock();
+    done_cv_.wait(lk, [&] { return pending_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  HostPool() {
+    cpu_set_t set;
+    int cpus = 1;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) cpus = CPU_COUNT(&set);
+    const int workers = std::max(0, std::min(8, cpus / 2) - 1);
+    for (int i = 0; i < workers; ++i) th_.emplace_back([this, i] { run(i + 1); });
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  void run(int part) {
+    uint64_t seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> lk(mu_);
+      cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      const std::function<void(size_t, size_t)> *fn = fn_;
+      const size_t n = n_, parts = th_.size() + 1;
+      lk.unlock();
+      (*fn)(n * part / parts, n * (part + 1) / parts);
+      lk.lock();
+      if (--pending_ == 0) done_cv_.notify_one();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex call_mu_, mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(size_t, size_t)> *fn_ = nullptr;
+  size_t n_ = 0;
+  int pending_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
 
 int ensure_tk(hmv_arena_s *a, int k) {
   const int64_t need = std::max<int64_t>(1, a->tpart_size) * k;
@@ -102,6 +180,9 @@ void release(hmv_arena_s *a) {
   (void)hipFree(a->d_t);
   (void)hipFree(a->d_hx);
   (void)hipFree(a->d_hy);
+  (void)hipHostFree(a->h_x);
+  (void)hipHostFree(a->h_y);
+  if (a->hstream) (void)hipStreamDestroy(a->hstream);
   delete a;
 }
 
@@ -622,19 +703,32 @@ int hmv_apply_host(hmv_arena a, const double *zu, double *zau) {
     return HMV_ERR_ARG;
   }
   hipError_t e;
-  const size_t nb = sizeof(double) * (size_t)a->nd;
+  const size_t n = (size_t)a->nd, nb = sizeof(double) * n;
   if (!a->d_hx) {
-    if ((e = hipMalloc(&a->d_hx, nb)) != hipSuccess) return hip_fail(e, "hmv_apply_host: alloc");
-    if ((e = hipMalloc(&a->d_hy, nb)) != hipSuccess) return hip_fail(e, "hmv_apply_host: alloc");
+    if ((e = hipMalloc(&a->d_hx, nb)) != hipSuccess || (e = hipMalloc(&a->d_hy, nb)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&a->h_x, nb, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc((void **)&a->h_y, nb, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&a->hstream, hipStreamNonBlocking)) != hipSuccess)
+      return hip_fail(e, "hmv_apply_host: setup");
   }
-  if ((e = hipMemcpy(a->d_hx, zu, nb, hipMemcpyHostToDevice)) != hipSuccess)
+  // two copies over PCIe, both from pinned staging: zu up, H zu down; the
+  // CPU's passes over the caller's arrays (zu into staging, zau += H zu out
+  // of it) are split over the pool's threads
+  hmv::HostPool &pool = hmv::HostPool::get();
+  double *hx = a->h_x, *hy = a->h_y;
+  pool.parallel_for(n, [&](size_t lo, size_t hi) {
+    std::memcpy(hx + lo, zu + lo, sizeof(double) * (hi - lo));
+  });
+  if ((e = hipMemcpyAsync(a->d_hx, hx, nb, hipMemcpyHostToDevice, a->hstream)) != hipSuccess)
     return hip_fail(e, "hmv_apply_host: copy x");
-  if ((e = hipMemcpy(a->d_hy, zau, nb, hipMemcpyHostToDevice)) != hipSuccess)
-    return hip_fail(e, "hmv_apply_host: copy y");
-  if ((e = hmv::enqueue_apply(a, a->d_hx, a->d_hy, 1.0, nullptr, nullptr)) != hipSuccess)
+  if ((e = hmv::enqueue_apply(a, a->d_hx, a->d_hy, 0.0, nullptr, a->hstream)) != hipSuccess)
     return hip_fail(e, "hmv_apply_host: launch");
-  if ((e = hipMemcpy(zau, a->d_hy, nb, hipMemcpyDeviceToHost)) != hipSuccess)
+  if ((e = hipMemcpyAsync(hy, a->d_hy, nb, hipMemcpyDeviceToHost, a->hstream)) != hipSuccess ||
+      (e = hipStreamSynchronize(a->hstream)) != hipSuccess)
     return hip_fail(e, "hmv_apply_host: copy back");
+  pool.parallel_for(n, [&](size_t lo, size_t hi) {
+    for (size_t i = lo; i < hi; ++i) zau[i] += hy[i];
+  });
   return HMV_OK;
 }
 
