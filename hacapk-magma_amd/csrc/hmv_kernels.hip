@@ -32,18 +32,21 @@ template <typename Acc, int NT>
 __device__ inline void stage(Acc *__restrict__ z, const Inputs &in, int c0, int cw,
                              const double *__restrict__ x, const int *__restrict__ codes,
                              const double *__restrict__ t) {
-  constexpr int K = 4;
-  for (int cb = threadIdx.x; cb < cw; cb += K * NT) {
-    int code[K];
+  with_codes(in, codes, [&](auto code_of) {
+    constexpr int K = 4;
+    for (int cb = threadIdx.x; cb < cw; cb += K * NT) {
+      // two code searches interleaved at a time: four would spill
+      int code[K];
+#pragma unroll 2
+      for (int k = 0; k < K; ++k) code[k] = code_of(c0 + min(cb + k * NT, cw - 1));
+      double val[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) code[k] = input_code(in, codes, c0 + min(cb + k * NT, cw - 1));
-    double val[K];
+      for (int k = 0; k < K; ++k) val[k] = code[k] >= 0 ? x[code[k]] : t[-code[k] - 1];
 #pragma unroll
-    for (int k = 0; k < K; ++k) val[k] = code[k] >= 0 ? x[code[k]] : t[-code[k] - 1];
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if (cb + k * NT < cw) z[cb + k * NT] = (Acc)val[k];
-  }
+      for (int k = 0; k < K; ++k)
+        if (cb + k * NT < cw) z[cb + k * NT] = (Acc)val[k];
+    }
+  });
 }
 
 // Accumulates the lane's n columns of one part, starting at byte offset vb
