@@ -35,13 +35,14 @@ constexpr int kcols() {
   return kbuf_elems<Acc, NT, VR, K>() / K - 1;
 }
 
-// Stages the inputs of item columns [c0, c0+cw) into z[c*K + j], j < K (0
-// for j >= k), and the zero column at c = cw.  One thread per column, its k
-// values loaded together.
+// Staging of the inputs of item columns [c0, c0+cw) into z[c*K + j], j < K
+// (0 for j >= k), and of the zero column at c = cw.
+//
+// kstage_rows: one thread per column, its K values loaded together.
 template <typename Acc, int NT, int K>
-__device__ inline void kstage(Acc *__restrict__ z, const Inputs &in, int c0, int cw,
-                              const double *__restrict__ x, const int *__restrict__ codes,
-                              const double *__restrict__ t, int k) {
+__device__ inline void kstage_rows(Acc *__restrict__ z, const Inputs &in, int c0, int cw,
+                                   const double *__restrict__ x, const int *__restrict__ codes,
+                                   const double *__restrict__ t, int k) {
   for (int c = threadIdx.x; c < cw; c += NT) {
     const int code = input_code(in, codes, c0 + c);
     const double *src = code >= 0 ? x + (size_t)code * k : t + (size_t)(-code - 1) * k;
@@ -52,6 +53,46 @@ __device__ inline void kstage(Acc *__restrict__ z, const Inputs &in, int c0, int
     for (int j = 0; j < K; ++j) z[c * K + j] = (Acc)v[j];
   }
   if ((int)threadIdx.x < K) z[cw * K + threadIdx.x] = Acc(0);
+}
+// kstage_cols: each wave takes 64 columns at a time; lane l finds the code
+// of column l, then K lanes per column load its K values (one wave load
+// covers 64/K whole columns, K*8 contiguous bytes each; B loads in flight
+// per lane) and write 64 consecutive z slots.
+template <typename Acc, int NT, int K, int B>
+__device__ inline void kstage_cols(Acc *__restrict__ z, const Inputs &in, int c0, int cw,
+                                   const double *__restrict__ x, const int *__restrict__ codes,
+                                   const double *__restrict__ t, int k) {
+  static_assert(B > 0 && K % B == 0, "whole load groups");
+  constexpr int CPI = 64 / K;  // columns per wave load
+  const int lane = threadIdx.x & 63, j = lane % K;
+  for (int b = (threadIdx.x >> 6) * 64; b < cw; b += NT) {
+    const int mine = b + lane < cw ? input_code(in, codes, c0 + b + lane) : 0;
+#pragma unroll 1
+    for (int i0 = 0; i0 < K; i0 += B) {
+      double v[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const int src_lane = (i0 + i) * CPI + lane / K;
+        const int code = __shfl(mine, src_lane);
+        const double *src = code >= 0 ? x + (size_t)code * k : t + (size_t)(-code - 1) * k;
+        v[i] = b + src_lane < cw && j < k ? src[j] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < B; ++i)
+        if (b + (i0 + i) * CPI + lane / K < cw) z[(b + (i0 + i) * CPI) * K + lane] = (Acc)v[i];
+    }
+  }
+  if ((int)threadIdx.x < K) z[cw * K + threadIdx.x] = Acc(0);
+}
+// B = 0: kstage_rows, else kstage_cols.
+template <typename Acc, int NT, int K, int B>
+__device__ inline void kstage(Acc *__restrict__ z, const Inputs &in, int c0, int cw,
+                              const double *__restrict__ x, const int *__restrict__ codes,
+                              const double *__restrict__ t, int k) {
+  if constexpr (B == 0)
+    kstage_rows<Acc, NT, K>(z, in, c0, cw, x, codes, t, k);
+  else
+    kstage_cols<Acc, NT, K, B>(z, in, c0, cw, x, codes, t, k);
 }
 
 // K FMAs per row of one loaded lane element: a[i][j] += e[i] * zc[j].
@@ -103,6 +144,14 @@ template <int VR, int K>
 constexpr int kprefetch() {
   return K >= 2 ? 1 : 2;
 }
+// Staging loads in flight per lane of kstage_cols, or 0 for kstage_rows.
+// The column-cooperative staging pays from K = 8 (FP64 only; 1.23 -> 1.19 ms
+// per H-matmat at 327,680 unknowns); at K <= 4 a thread's K*8 contiguous
+// bytes already make a short burst and the shuffles cost more than they save.
+template <typename Acc, int VR, int K>
+constexpr int kstage_batch() {
+  return K >= 8 ? 4 : 0;
+}
 
 template <typename Acc, int DT, int VR, int K>
 __device__ inline void krun_cols(const Part<DT, VR> &P, int lo, int hi, int cg, int m, int zshift,
@@ -152,7 +201,7 @@ __device__ inline void kstream_item(const HmvWork &w, Acc *__restrict__ z, int *
   for (int c0 = 0; c0 < ncols; c0 += ZC) {
     const int cw = min(ZC, ncols - c0);
     if (c0 > 0) __syncthreads();
-    kstage<Acc, NT, K>(z, in, c0, cw, x, codes, t_in, k);
+    kstage<Acc, NT, K, kstage_batch<Acc, vr, K>()>(z, in, c0, cw, x, codes, t_in, k);
     __syncthreads();
     if (active) {
       int lo0 = c0;
@@ -255,6 +304,8 @@ void klaunch_mode(const HmvLaunch &L, int phase, const double *x, double *t, dou
                   double beta, const int *skip, hipStream_t s) {
   if (L.mode == 0)
     klaunch_phase<double, HMV_DT_F64, HMV_DT_F64, K>(L, phase, x, t, y, k, beta, skip, s);
+  else if constexpr (K > HMV_KSTREAM_MAX_NARROW)
+    return;  // not instantiated: hmv_launch_kstream rejects these k
   else if (L.mode == 1)
     klaunch_phase<float, HMV_DT_F32, HMV_DT_F32, K>(L, phase, x, t, y, k, beta, skip, s);
   else if (L.mode == 2)
@@ -270,6 +321,8 @@ void klaunch_mode(const HmvLaunch &L, int phase, const double *x, double *t, dou
 extern "C" hipError_t hmv_launch_kstream(const HmvLaunch *L, int phase, int k, const double *x,
                                          double *t, double *y, double beta, const int *skip,
                                          hipStream_t stream) {
+  if (k < 1 || k > (L->mode == 0 ? HMV_KSTREAM_MAX_F64 : HMV_KSTREAM_MAX_NARROW))
+    return hipErrorInvalidValue;
   if (k <= 1)
     klaunch_mode<1>(*L, phase, x, t, y, k, beta, skip, stream);
   else if (k <= 2)
