@@ -150,7 +150,14 @@ def main():
     ap.add_argument("--construct", action="store_true",
                     help="only the device H-matrix fill and arena repack on 327,680 unknowns "
                          "(for profiler runs)")
+    ap.add_argument("--applies", action="store_true",
+                    help="only the single-vector applies on 327,680 unknowns in the four storage "
+                         "precisions (for profiler runs)")
     args = ap.parse_args()
+    if args.applies:
+        import __graft_entry__ as G
+        print(json.dumps({"applies_327680": applies_only(G.load_package(), log)}), flush=True)
+        return
     if args.hmatmat:
         import __graft_entry__ as G
         print(json.dumps({"hmatmat_fp64_327680": hmatmat_only(G.load_package(), log)}), flush=True)
@@ -557,6 +564,28 @@ def hmatmat_only(H, log, reps=5):
         X.free()
         Y.free()
         a.free()
+    return out
+
+
+def applies_only(H, log, reps=5):
+    """Single-vector applies of one icosphere s=7 in the four storage
+    precisions, one untimed and `reps` timed applies each."""
+    if H.hmv_lib().hmv_init(0) != 0:
+        raise SystemExit("no GPU: " + H.hmv_lib().hmv_last_error().decode())
+    hm = H.HMatrix(*H.icosphere(7), H.baseline_params(aca_eps=1e-5))
+    dx = H.DeviceBuffer.from_array(H.uniform(1, hm.nd))
+    dy = H.DeviceBuffer(8 * hm.nd)
+    out = {}
+    for prec in ("f64", "f32", "bf16", "bf16_all"):
+        a = H.Arena(hm, prec)
+        a.apply(dx.ptr, dy.ptr)
+        m1, m2, ma = a.apply_timed(dx.ptr, dy.ptr, reps)
+        out[prec] = {"ms": round(ma, 4), "phase1_ms": round(m1, 4), "phase2_ms": round(m2, 4),
+                     "storage_alg_GBs": round(a.stats["alg_bytes"] / ma / 1e6, 1)}
+        log("%s apply %.4f ms (p1 %.4f, p2 %.4f)" % (prec, ma, m1, m2))
+        a.free()
+    dx.free()
+    dy.free()
     return out
 
 
