@@ -105,7 +105,7 @@ __device__ inline void kfma(typename Lane<DT, VR>::raw v, const Acc *__restrict_
   for (int j = 0; j < K; ++j) {
     const Acc zj = zc[j];
 #pragma unroll
-    for (int i = 0; i < VR; ++i) a[i][j] = __builtin_fma(e[i], zj, a[i][j]);
+    for (int i = 0; i < VR; ++i) a[i][j] = acc_fma(e[i], zj, a[i][j]);
   }
 }
 

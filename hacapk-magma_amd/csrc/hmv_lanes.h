@@ -21,6 +21,12 @@ typedef float float2_nt __attribute__((ext_vector_type(2)));
 __device__ inline float bf16_lo(uint32_t v) { return __uint_as_float(v << 16); }
 __device__ inline float bf16_hi(uint32_t v) { return __uint_as_float(v & 0xffff0000u); }
 
+// Fused multiply-add in the accumulation type: one FP32 FMA for FP32
+// accumulators (__builtin_fma is the FP64 builtin: on floats it converts
+// all three operands, multiplies in FP64 and rounds back).
+__device__ inline float acc_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double acc_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Buffer resource over one column part (raw, byte-addressed).  A load whose
 // vector offset is at or past num_records returns zero without touching
 // memory; lanes with nothing to load use kOffNone.
@@ -57,7 +63,7 @@ struct Lane<HMV_DT_F64, 1> {
   }
   template <typename Acc>
   __device__ static void fma(raw v, Acc z, Acc (&a)[1]) {
-    a[0] = __builtin_fma((Acc)v, z, a[0]);
+    a[0] = acc_fma((Acc)v, z, a[0]);
   }
   template <typename Acc>
   __device__ static void unpack(raw v, Acc (&o)[1]) {
@@ -72,8 +78,8 @@ struct Lane<HMV_DT_F32, 2> {
   }
   template <typename Acc>
   __device__ static void fma(raw v, Acc z, Acc (&a)[2]) {
-    a[0] = __builtin_fma((Acc)v.x, z, a[0]);
-    a[1] = __builtin_fma((Acc)v.y, z, a[1]);
+    a[0] = acc_fma((Acc)v.x, z, a[0]);
+    a[1] = acc_fma((Acc)v.y, z, a[1]);
   }
   template <typename Acc>
   __device__ static void unpack(raw v, Acc (&o)[2]) {
@@ -89,8 +95,8 @@ struct Lane<HMV_DT_BF16, 2> {
   }
   template <typename Acc>
   __device__ static void fma(raw v, Acc z, Acc (&a)[2]) {
-    a[0] = __builtin_fma((Acc)bf16_lo(v), z, a[0]);
-    a[1] = __builtin_fma((Acc)bf16_hi(v), z, a[1]);
+    a[0] = acc_fma((Acc)bf16_lo(v), z, a[0]);
+    a[1] = acc_fma((Acc)bf16_hi(v), z, a[1]);
   }
   template <typename Acc>
   __device__ static void unpack(raw v, Acc (&o)[2]) {
@@ -106,10 +112,10 @@ struct Lane<HMV_DT_BF16, 4> {
   }
   template <typename Acc>
   __device__ static void fma(raw v, Acc z, Acc (&a)[4]) {
-    a[0] = __builtin_fma((Acc)bf16_lo(v.x), z, a[0]);
-    a[1] = __builtin_fma((Acc)bf16_hi(v.x), z, a[1]);
-    a[2] = __builtin_fma((Acc)bf16_lo(v.y), z, a[2]);
-    a[3] = __builtin_fma((Acc)bf16_hi(v.y), z, a[3]);
+    a[0] = acc_fma((Acc)bf16_lo(v.x), z, a[0]);
+    a[1] = acc_fma((Acc)bf16_hi(v.x), z, a[1]);
+    a[2] = acc_fma((Acc)bf16_lo(v.y), z, a[2]);
+    a[3] = acc_fma((Acc)bf16_hi(v.y), z, a[3]);
   }
   template <typename Acc>
   __device__ static void unpack(raw v, Acc (&o)[4]) {
