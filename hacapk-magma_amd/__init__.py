@@ -252,6 +252,9 @@ def hmv_lib():
         L.hmv_multi_bicgstab_ir.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_int,
                                             C.c_int, C.c_int, C.POINTER(ColumnInfo), _D,
                                             C.POINTER(SolveInfo)]
+        L.hmv_multi_gcr_ir.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.c_int, C.c_int, C.POINTER(ColumnInfo), _D,
+                                       C.POINTER(SolveInfo)]
         L.hmv_multi_bicgstab.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int,
                                          C.POINTER(ColumnInfo), _D, C.POINTER(SolveInfo)]
         L.hmv_apply_block_timed.argtypes = [vp, C.c_int, vp, vp, C.c_int, _D]
@@ -1014,6 +1017,30 @@ def multi_bicgstab_ir(m64, mlow, b, u0=None, eps=1e-8, inner_eps=1e-4, max_outer
                                                float(inner_eps), int(max_outer), int(max_inner),
                                                int(check_every), inst, _dptr(secs), C.byref(info)),
                "hmv_multi_bicgstab_ir")
+        u = du.download()
+    finally:
+        db.free()
+        du.free()
+    return u, [c.as_dict() for c in inst], secs, info.as_dict()
+
+
+def multi_gcr_ir(m64, mlow, b, u0=None, eps=1e-8, inner_eps=5e-3, m=32, max_outer=30,
+                 max_inner=2000, check_every=4):
+    """Mixed-precision refinement per instance with GCR(m) inner solves over
+    two MultiArenas of the same instances (hmv_multi_gcr_ir).  Returns (u,
+    per-instance info list, per-instance seconds, info)."""
+    b = np.ascontiguousarray(b, np.float64)
+    u0 = np.zeros_like(b) if u0 is None else np.ascontiguousarray(u0, np.float64)
+    n = len(m64.offsets) - 1
+    db, du = DeviceBuffer.from_array(b), DeviceBuffer.from_array(u0)
+    inst = (ColumnInfo * n)()
+    secs = np.zeros(n)
+    info = SolveInfo()
+    try:
+        _check(hmv_lib().hmv_multi_gcr_ir(m64.handle, mlow.handle, db.ptr, du.ptr, float(eps),
+                                          float(inner_eps), int(m), int(max_outer), int(max_inner),
+                                          int(check_every), inst, _dptr(secs), C.byref(info)),
+               "hmv_multi_gcr_ir")
         u = du.download()
     finally:
         db.free()

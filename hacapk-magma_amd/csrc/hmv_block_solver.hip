@@ -1098,3 +1098,509 @@ extern "C" int hmv_multi_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double
   }
   return HMV_OK;
 }
+
+// ---- many instances: GCR(m) per segment (inner solver of hmv_multi_gcr_ir) -
+//
+// One GCR(m) per instance (m_HACApK_solve.f90:506-571, as hmv_gcrm runs it),
+// over the same block table.  The running instances take their steps
+// together, so the slot of a step (ik) and of the next direction (ikn) are
+// common to all of them; each instance keeps its own alpha, betas and
+// (A p_l, A p_l).  A stopped instance's blocks skip every update, and its
+// H-matvec workgroups return at once (per-instance done flags).
+
+namespace {
+
+constexpr int kGcrM = HMV_GCR_MAX_M;
+constexpr int kGroup = 4;  // slots per block row of the beta dots
+
+struct SgInst {
+  double alpha, rnorm, bnorm;
+  double capap[kGcrM];
+  double beta[kGcrM];
+  int step, done, status, pad;
+};
+
+struct SgGlobal {
+  double eps;
+  int step, ik, ikn, m, maxit, all_done, pad;
+};
+
+// Fixed-order sum over the blocks [b0, b1) of part[stride * b + off]: thread
+// t takes blocks b0 + t, b0 + t + kT, ..., then the block sum.
+__device__ inline double inst_sum(const double *__restrict__ part, int b0, int b1, int stride,
+                                  int off, double *sh) {
+  double s = 0.0;
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += kT) s += part[(size_t)stride * b + off];
+  return blk_sum(s, sh);
+}
+
+// r = b, p_0 = b (zero initial guess)
+__global__ __launch_bounds__(kT) void sg_init(int64_t n, const double *__restrict__ b,
+                                              double *__restrict__ r, double *__restrict__ p) {
+  for (int64_t i = (int64_t)blockIdx.x * kT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kT) {
+    r[i] = b[i];
+    p[i] = b[i];
+  }
+}
+
+// (ap_ikn, r), (ap_ikn, ap_ikn) partials of each block (the first step's)
+__global__ __launch_bounds__(kT) void sg_dots_alpha(const SegBlock *__restrict__ tb, int64_t n,
+                                                    const double *__restrict__ ap,
+                                                    const double *__restrict__ r,
+                                                    double *__restrict__ part,
+                                                    const SgGlobal *gs) {
+  if (gs->all_done) return;
+  __shared__ double sh[4];
+  const SegBlock B = tb[blockIdx.x];
+  const double *q = ap + (size_t)gs->ikn * n;
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = B.beg + threadIdx.x; i < B.end; i += kT) {
+    s0 = __builtin_fma(q[i], r[i], s0);
+    s1 = __builtin_fma(q[i], q[i], s1);
+  }
+  const double r0 = blk_sum(s0, sh), r1 = blk_sum(s1, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r0;
+    part[2 * blockIdx.x + 1] = r1;
+  }
+}
+
+// alpha = (ap_ik, r) / (ap_ik, ap_ik) of the block's instance, from its
+// blocks' pairs in apart (every block of the instance forms it alike; the
+// instance's first block records it and capap_ik, and flags a breakdown),
+// then u += alpha p_ik, r -= alpha ap_ik and the block's (r, r) partial.
+__global__ __launch_bounds__(kT) void sg_update_ur(const SegBlock *__restrict__ tb,
+                                                   const int *__restrict__ bstart, int64_t n,
+                                                   const double *__restrict__ p,
+                                                   const double *__restrict__ ap,
+                                                   double *__restrict__ u, double *__restrict__ r,
+                                                   const double *__restrict__ apart,
+                                                   double *__restrict__ rpart, SgInst *inst,
+                                                   const SgGlobal *gs) {
+  if (gs->all_done) return;
+  __shared__ double sh[4];
+  const SegBlock B = tb[blockIdx.x];
+  const int s = B.seg;
+  if (inst[s].done) return;  // the stage-2 pass skips stopped instances
+  const int b0 = bstart[s], b1 = bstart[s + 1];
+  const double d0 = inst_sum(apart, b0, b1, 2, 0, sh), d1 = inst_sum(apart, b0, b1, 2, 1, sh);
+  const int ik = gs->ikn;  // the slot of this step
+  const double alpha = d0 / d1;
+  const bool bad = d1 == 0.0 || !isfinite(alpha);
+  if ((int)blockIdx.x == b0 && threadIdx.x == 0) {
+    inst[s].capap[ik] = d1;
+    inst[s].alpha = alpha;
+    if (bad) {
+      inst[s].status = isfinite(d1) ? HMV_SOLVE_BREAKDOWN : HMV_SOLVE_NONFINITE;
+      inst[s].done = 1;
+    }
+  }
+  if (bad) return;
+  const double *pk = p + (size_t)ik * n, *q = ap + (size_t)ik * n;
+  double s2 = 0.0;
+  for (int i = B.beg + threadIdx.x; i < B.end; i += kT) {
+    u[i] = u[i] + alpha * pk[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    s2 = __builtin_fma(ri, ri, s2);
+  }
+  const double r0 = blk_sum(s2, sh);
+  if (threadIdx.x == 0) rpart[blockIdx.x] = r0;
+}
+
+// Partials of (v_l, ar) for the slots l <= ik of the block row's group
+// (v_l = ar for the slot just overwritten, l == ikn, else ap_l):
+// part[blk * kGcrM + l].
+__global__ __launch_bounds__(kT) void sg_dots_beta(const SegBlock *__restrict__ tb, int64_t n,
+                                                   const double *__restrict__ ap,
+                                                   const double *__restrict__ ar,
+                                                   double *__restrict__ part,
+                                                   const SgInst *__restrict__ inst,
+                                                   const SgGlobal *gs) {
+  if (gs->all_done) return;
+  const SegBlock B = tb[blockIdx.x];
+  if (inst[B.seg].done) return;
+  const int ik = gs->ik, ikn = gs->ikn, l0 = blockIdx.y * kGroup;
+  if (l0 > ik) return;
+  __shared__ double sh[4];
+  const double *v[kGroup];
+#pragma unroll
+  for (int j = 0; j < kGroup; ++j) {
+    const int l = min(l0 + j, ik);
+    v[j] = l == ikn ? ar : ap + (size_t)l * n;
+  }
+  double s[kGroup] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = B.beg + threadIdx.x; i < B.end; i += kT) {
+    const double a = ar[i];
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) s[j] = __builtin_fma(v[j][i], a, s[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < kGroup; ++j) {
+    const double r0 = blk_sum(s[j], sh);
+    if (threadIdx.x == 0 && l0 + j <= ik) part[(size_t)blockIdx.x * kGcrM + l0 + j] = r0;
+  }
+}
+
+// Per-instance scalar stages, one thread per instance (its blocks summed in
+// block order):
+//   0: setup from the (r, r), (b, b) pairs;
+//   2: ||r|| from the (r, r) partials, the step and stopping tests, then the
+//      common slots (ik = the slot just used, ikn = the next one);
+//   3: beta_l = -(A p_l, A r) / capap_l for l <= ik (one thread per
+//      instance and slot).
+// done_flags mirrors each instance's done flag for the H-matvec.
+__global__ __launch_bounds__(kT) void sg_final(int nseg, const int *__restrict__ bstart,
+                                               const double *__restrict__ part, SgInst *inst,
+                                               SgGlobal *gs, int stage,
+                                               int *__restrict__ done_flags) {
+  if (gs->all_done && stage != 0) return;
+  __shared__ int sdone;
+  if (threadIdx.x == 0) sdone = 1;
+  __syncthreads();
+  if (stage == 3) {
+    const int ik = gs->ik;
+    for (int e = threadIdx.x; e < nseg * (ik + 1); e += kT) {
+      const int s = e / (ik + 1), l = e % (ik + 1);
+      if (inst[s].done) continue;
+      double v = 0.0;
+      for (int b = bstart[s]; b < bstart[s + 1]; ++b) v += part[(size_t)b * kGcrM + l];
+      inst[s].beta[l] = -v / inst[s].capap[l];
+    }
+    return;
+  }
+  for (int s = threadIdx.x; s < nseg; s += kT) {
+    SgInst &c = inst[s];
+    if (stage == 0) {
+      double d0 = 0.0, d1 = 0.0;
+      for (int b = bstart[s]; b < bstart[s + 1]; ++b) {
+        d0 += part[2 * b];
+        d1 += part[2 * b + 1];
+      }
+      c.rnorm = sqrt(d0);
+      c.bnorm = sqrt(d1);
+      c.alpha = 0.0;
+      c.step = 0;
+      c.done = 0;
+      c.status = HMV_SOLVE_MAXITER;
+      if (!isfinite(c.rnorm) || !isfinite(c.bnorm) || c.bnorm == 0.0) {
+        c.status = HMV_SOLVE_NONFINITE;
+        c.done = 1;
+      } else if (c.rnorm / c.bnorm < gs->eps) {
+        c.status = HMV_SOLVE_CONVERGED;
+        c.done = 1;
+      } else if (gs->maxit <= 0) {
+        c.done = 1;
+      }
+    } else if (!c.done) {
+      double d0 = 0.0;
+      for (int b = bstart[s]; b < bstart[s + 1]; ++b) d0 += part[b];
+      c.step += 1;
+      c.rnorm = sqrt(d0);
+      if (!isfinite(c.rnorm)) {
+        c.status = HMV_SOLVE_NONFINITE;
+        c.done = 1;
+      } else if (c.rnorm / c.bnorm < gs->eps) {
+        c.status = HMV_SOLVE_CONVERGED;
+        c.done = 1;
+      } else if (c.step >= gs->maxit) {
+        c.status = HMV_SOLVE_MAXITER;
+        c.done = 1;
+      }
+    }
+    done_flags[s] = c.done;
+    if (!c.done) atomicAnd(&sdone, 0);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (stage == 0) {
+      gs->step = 0;
+      gs->ik = 0;
+      gs->ikn = 0;
+    } else {
+      gs->ik = gs->ikn;
+      gs->step += 1;
+      gs->ikn = gs->step % gs->m;
+    }
+    gs->all_done = sdone;
+  }
+}
+
+// p_ikn = c0 r + sum_{l <= ik, l != ikn} beta_l p_l and ap_ikn likewise with
+// ar (c0 = 1 + beta_ikn when the slot is reused, else 1), per instance, and
+// the next step's alpha partials (ap_ikn, r), (ap_ikn, ap_ikn).
+__global__ __launch_bounds__(kT) void sg_update_dir(const SegBlock *__restrict__ tb, int64_t n,
+                                                    const double *__restrict__ r,
+                                                    const double *__restrict__ ar,
+                                                    double *__restrict__ p,
+                                                    double *__restrict__ ap,
+                                                    double *__restrict__ part,
+                                                    const SgInst *__restrict__ inst,
+                                                    const SgGlobal *gs) {
+  if (gs->all_done) return;
+  const SegBlock B = tb[blockIdx.x];
+  const SgInst &c = inst[B.seg];
+  if (c.done) return;
+  __shared__ double sh[4];
+  const int ik = gs->ik, ikn = gs->ikn;
+  const double c0 = ikn <= ik ? 1.0 + c.beta[ikn] : 1.0;
+  double *pn = p + (size_t)ikn * n, *apn = ap + (size_t)ikn * n;
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = B.beg + threadIdx.x; i < B.end; i += kT) {
+    const double ri = r[i];
+    double pv = c0 * ri, av = c0 * ar[i];
+    for (int l = 0; l <= ik; ++l) {
+      if (l == ikn) continue;
+      const double bl = c.beta[l];
+      pv = __builtin_fma(bl, p[(size_t)l * n + i], pv);
+      av = __builtin_fma(bl, ap[(size_t)l * n + i], av);
+    }
+    pn[i] = pv;
+    apn[i] = av;
+    s0 = __builtin_fma(av, ri, s0);
+    s1 = __builtin_fma(av, av, s1);
+  }
+  const double r0 = blk_sum(s0, sh), r1 = blk_sum(s1, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r0;
+    part[2 * blockIdx.x + 1] = r1;
+  }
+}
+
+struct SegGcrWork {
+  int m = 0;
+  double *r = nullptr, *ar = nullptr, *p = nullptr, *ap = nullptr, *part = nullptr;
+  SgInst *inst = nullptr;
+  SgGlobal *gs = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const double *graph_u = nullptr;
+  ~SegGcrWork() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (double *q : {r, ar, p, ap, part}) (void)hipFree(q);
+    (void)hipFree(inst);
+    (void)hipFree(gs);
+  }
+};
+
+int sg_setup(const SegWork &w, SegGcrWork &gw, int m, const char *who) {
+  gw.m = m;
+  const size_t nb = sizeof(double) * (size_t)w.n;
+  hipError_t e;
+  if ((e = hipMalloc(&gw.r, nb)) != hipSuccess || (e = hipMalloc(&gw.ar, nb)) != hipSuccess ||
+      (e = hipMalloc(&gw.p, nb * m)) != hipSuccess || (e = hipMalloc(&gw.ap, nb * m)) != hipSuccess ||
+      (e = hipMalloc(&gw.part, sizeof(double) * (size_t)w.nblk * kGcrM)) != hipSuccess ||
+      (e = hipMalloc(&gw.inst, sizeof(SgInst) * w.nseg)) != hipSuccess ||
+      (e = hipMalloc(&gw.gs, sizeof(SgGlobal))) != hipSuccess)
+    return hmv::hip_fail(e, (std::string(who) + ": GCR workspace").c_str());
+  return HMV_OK;
+}
+
+hipError_t sg_iteration(hmv_arena_s *a, const SegWork &w, SegGcrWork &gw, double *u) {
+  const int nb = w.nblk;
+  const int *skip = &gw.gs->all_done;
+  // (r, r) partials at [2 nb, 3 nb) of the nb * kGcrM partials, apart from the
+  // alpha pairs at [0, 2 nb) that sg_update_ur reads while writing them
+  double *rpart = gw.part + 2 * (size_t)nb;
+  hipLaunchKernelGGL(sg_update_ur, dim3(nb), dim3(kT), 0, w.s, w.tb, w.bstart, w.n, gw.p, gw.ap,
+                     u, gw.r, (const double *)gw.part, rpart, gw.inst, (const SgGlobal *)gw.gs);
+  hipLaunchKernelGGL(sg_final, dim3(1), dim3(kT), 0, w.s, w.nseg, w.bstart, (const double *)rpart,
+                     gw.inst, gw.gs, 2, w.flags);
+  hipError_t e = hmv::enqueue_apply(a, gw.r, gw.ar, 0.0, skip, w.s, w.flags);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(sg_dots_beta, dim3(nb, (gw.m + kGroup - 1) / kGroup), dim3(kT), 0, w.s, w.tb,
+                     w.n, gw.ap, gw.ar, gw.part, (const SgInst *)gw.inst, (const SgGlobal *)gw.gs);
+  hipLaunchKernelGGL(sg_final, dim3(1), dim3(kT), 0, w.s, w.nseg, w.bstart, (const double *)gw.part,
+                     gw.inst, gw.gs, 3, w.flags);
+  hipLaunchKernelGGL(sg_update_dir, dim3(nb), dim3(kT), 0, w.s, w.tb, w.n, gw.r, gw.ar, gw.p,
+                     gw.ap, gw.part, (const SgInst *)gw.inst, (const SgGlobal *)gw.gs);
+  return hipGetLastError();
+}
+
+// One GCR(m) per instance for A d = b from d = 0 (the caller zeroes d), in
+// lockstep until every instance has stopped or maxit steps ran.  inst
+// receives the instances' final states, iters the steps run.  The
+// iteration graph is captured on the first call with a given d.
+int sg_solve(hmv_arena_s *a, const SegWork &w, SegGcrWork &gw, const double *b, double *d,
+             double eps, int maxit, int check_every, std::vector<SgInst> &inst, int &iters,
+             const char *who) {
+  const std::string name(who);
+  SgGlobal init{};
+  init.eps = eps;
+  init.m = gw.m;
+  init.maxit = maxit;
+  const int g = grid_of(w.n);
+  hipError_t e = hipMemcpyAsync(gw.gs, &init, sizeof(init), hipMemcpyHostToDevice, w.s);
+  if (e != hipSuccess) return hmv::hip_fail(e, (name + ": setup").c_str());
+  hipLaunchKernelGGL(sg_init, dim3(g), dim3(kT), 0, w.s, w.n, b, gw.r, gw.p);
+  hipLaunchKernelGGL(ks_dots, dim3(w.nblk), dim3(kT), 0, w.s, w.tb, gw.r, gw.r, b, b, gw.part,
+                     (const BlockState *)nullptr);
+  hipLaunchKernelGGL(sg_final, dim3(1), dim3(kT), 0, w.s, w.nseg, w.bstart,
+                     (const double *)gw.part, gw.inst, gw.gs, 0, w.flags);
+  // ap_0 = A p_0 (the stopped instances' workgroups return) and the first
+  // step's alpha partials
+  if ((e = hmv::enqueue_apply(a, gw.p, gw.ap, 0.0, &gw.gs->all_done, w.s, w.flags)) != hipSuccess)
+    return hmv::hip_fail(e, (name + ": setup").c_str());
+  hipLaunchKernelGGL(sg_dots_alpha, dim3(w.nblk), dim3(kT), 0, w.s, w.tb, w.n, gw.ap, gw.r,
+                     gw.part, (const SgGlobal *)gw.gs);
+  if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(w.s)) != hipSuccess)
+    return hmv::hip_fail(e, (name + ": setup").c_str());
+  if (!gw.exec || gw.graph_u != d) {
+    if (gw.exec) (void)hipGraphExecDestroy(gw.exec);
+    if (gw.graph) (void)hipGraphDestroy(gw.graph);
+    gw.exec = nullptr;
+    gw.graph = nullptr;
+    if ((e = hipStreamBeginCapture(w.s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+      return hmv::hip_fail(e, (name + ": capture").c_str());
+    hipError_t ec = sg_iteration(a, w, gw, d);
+    e = hipStreamEndCapture(w.s, &gw.graph);
+    if (ec != hipSuccess) return hmv::hip_fail(ec, (name + ": capture").c_str());
+    if (e != hipSuccess ||
+        (e = hipGraphInstantiate(&gw.exec, gw.graph, nullptr, nullptr, 0)) != hipSuccess)
+      return hmv::hip_fail(e, (name + ": graph").c_str());
+    gw.graph_u = d;
+  }
+  SgGlobal host{};
+  int launched = 0;
+  for (;;) {
+    if ((e = hipMemcpyAsync(&host, gw.gs, sizeof(host), hipMemcpyDeviceToHost, w.s)) != hipSuccess ||
+        (e = hipStreamSynchronize(w.s)) != hipSuccess)
+      return hmv::hip_fail(e, (name + ": state").c_str());
+    if (host.all_done || launched >= maxit) break;
+    const int burst = std::min(check_every, maxit - launched);
+    for (int i = 0; i < burst; ++i)
+      if ((e = hipGraphLaunch(gw.exec, w.s)) != hipSuccess)
+        return hmv::hip_fail(e, (name + ": launch").c_str());
+    launched += burst;
+  }
+  inst.resize(w.nseg);
+  if ((e = hipMemcpyAsync(inst.data(), gw.inst, sizeof(SgInst) * w.nseg, hipMemcpyDeviceToHost,
+                          w.s)) != hipSuccess ||
+      (e = hipStreamSynchronize(w.s)) != hipSuccess)
+    return hmv::hip_fail(e, (name + ": state").c_str());
+  iters = host.step;
+  return HMV_OK;
+}
+
+}  // namespace
+
+extern "C" int hmv_multi_gcr_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u,
+                                double eps, double inner_eps, int m, int max_outer, int max_inner,
+                                int check_every, hmv_column_info *inst, double *inst_seconds,
+                                hmv_solve_info *info) {
+  const char *who = "hmv_multi_gcr_ir";
+  if (!a64 || !alow || !b || !u || max_outer < 1 || max_inner < 1 || m < 1 || m > kGcrM ||
+      a64->seg.size() < 2 || a64->seg != alow->seg || a64->nd != alow->nd ||
+      !(inner_eps > 0.0 && inner_eps < 1.0)) {
+    hmv::set_error(std::string(who) + ": invalid argument (two arenas of the same instances, "
+                   "1 <= m <= 32, 0 < inner_eps < 1)");
+    return HMV_ERR_ARG;
+  }
+  if (check_every < 1) check_every = 1;
+  SegWork w;  // block table, stream and FP64 residuals
+  std::vector<int> bstart;
+  int rc = seg_setup(alow, w, bstart, who);
+  if (rc) return rc;
+  SegGcrWork gw;
+  if ((rc = sg_setup(w, gw, m, who))) return rc;
+  const int nseg = w.nseg;
+  struct Buf {
+    double *r2 = nullptr, *d = nullptr;
+    int *stopped = nullptr;
+    ~Buf() {
+      (void)hipFree(r2);
+      (void)hipFree(d);
+      (void)hipFree(stopped);
+    }
+  } buf;
+  const size_t nbytes = sizeof(double) * (size_t)w.n;
+  hipError_t e;
+  if ((e = hipMalloc(&buf.r2, nbytes)) != hipSuccess || (e = hipMalloc(&buf.d, nbytes)) != hipSuccess ||
+      (e = hipMalloc(&buf.stopped, sizeof(int) * nseg)) != hipSuccess)
+    return hmv::hip_fail(e, "hmv_multi_gcr_ir: workspace");
+  const int g = grid_of(w.n);
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> status(nseg, HMV_SOLVE_MAXITER), stopped(nseg, 0), steps(nseg, 0);
+  std::vector<double> rel(nseg, 0.0), done_at(nseg, -1.0), rr, bb;
+  std::vector<SgInst> cs;
+  int outer = 0, inner_total = 0;
+  for (;;) {
+    // FP64 residual of every instance on the FP64 arena
+    if ((rc = seg_residuals(a64, w, b, u, buf.r2, bstart, rr, bb, who))) return rc;
+    const double now = seconds_since(t0);
+    bool all = true;
+    for (int s = 0; s < nseg; ++s) {
+      if (stopped[s]) continue;
+      rel[s] = std::sqrt(rr[s]) / std::sqrt(bb[s]);
+      if (!std::isfinite(rel[s]))
+        status[s] = HMV_SOLVE_NONFINITE;
+      else if (rel[s] < eps)
+        status[s] = HMV_SOLVE_CONVERGED;
+      else {
+        all = false;
+        continue;
+      }
+      stopped[s] = 1;
+      done_at[s] = now;
+    }
+    if (all || outer >= max_outer) break;
+    // inner solves A_low d = r on the instances still running (a stopped
+    // instance's r is zero, so its inner solve stops at the setup)
+    if ((e = hipMemcpyAsync(buf.stopped, stopped.data(), sizeof(int) * nseg, hipMemcpyHostToDevice,
+                            w.s)) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_multi_gcr_ir: flags");
+    hipLaunchKernelGGL(ks_zero_stopped, dim3(w.nblk), dim3(kT), 0, w.s, w.tb, buf.r2, buf.stopped);
+    hipLaunchKernelGGL(kb_fill, dim3(g), dim3(kT), 0, w.s, w.n, buf.d, 0.0);
+    int iters = 0;
+    if ((rc = sg_solve(alow, w, gw, buf.r2, buf.d, inner_eps, max_inner, check_every, cs, iters,
+                       who)))
+      return rc;
+    inner_total += iters;
+    for (int s = 0; s < nseg; ++s) {
+      if (stopped[s]) continue;
+      steps[s] += cs[s].step;
+      if (cs[s].done && cs[s].status == HMV_SOLVE_NONFINITE) {
+        // the correction of this instance is not usable: it stops here
+        status[s] = HMV_SOLVE_NONFINITE;
+        stopped[s] = 1;
+        done_at[s] = seconds_since(t0);
+      }
+    }
+    // u += d, with d = 0 on every stopped instance
+    if ((e = hipMemcpyAsync(buf.stopped, stopped.data(), sizeof(int) * nseg, hipMemcpyHostToDevice,
+                            w.s)) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_multi_gcr_ir: flags");
+    hipLaunchKernelGGL(ks_zero_stopped, dim3(w.nblk), dim3(kT), 0, w.s, w.tb, buf.d, buf.stopped);
+    hipLaunchKernelGGL(kb_add, dim3(g), dim3(kT), 0, w.s, w.n, u, buf.d);
+    if ((e = hipGetLastError()) != hipSuccess)
+      return hmv::hip_fail(e, "hmv_multi_gcr_ir: update");
+    ++outer;
+  }
+  if ((e = hipStreamSynchronize(w.s)) != hipSuccess)
+    return hmv::hip_fail(e, "hmv_multi_gcr_ir: sync");
+  const double total = seconds_since(t0);
+  int worst = HMV_SOLVE_CONVERGED;
+  double maxrel = 0.0;
+  for (int s = 0; s < nseg; ++s) {
+    if (inst) {
+      inst[s].nstp = steps[s];
+      inst[s].status = status[s];
+      inst[s].relres = rel[s];
+      inst[s].true_relres = rel[s];
+    }
+    if (inst_seconds) inst_seconds[s] = done_at[s] >= 0.0 ? done_at[s] : total;
+    if (std::isfinite(rel[s])) maxrel = std::max(maxrel, rel[s]);
+    if (status[s] != HMV_SOLVE_CONVERGED) worst = status[s];
+  }
+  if (info) {
+    info->nstp = inner_total;
+    info->status = worst;
+    info->relres = maxrel;
+    info->true_relres = maxrel;
+    info->seconds = total;
+    info->outer = outer;
+    info->inner_total = inner_total;
+  }
+  return HMV_OK;
+}

@@ -308,6 +308,16 @@ int hmv_multi_bicgstab_ir(hmv_arena a64, hmv_arena alow, const double *b, double
                           double inner_eps, int max_outer, int max_inner, int check_every,
                           hmv_column_info *inst, double *inst_seconds, hmv_solve_info *info);
 
+/* The same refinement per instance with GCR(m) inner solves (1 <= m <=
+ * HMV_GCR_MAX_M): one GCR(m) per running instance in lockstep, the steps of
+ * hmv_gcrm (m_HACApK_solve.f90:506-571) with each instance's own alpha,
+ * betas and projections, one hipGraph per step, each instance to
+ * inner_eps, at most max_inner steps.  inst's nstp counts the inner steps
+ * (one product each).  No reference counterpart. */
+int hmv_multi_gcr_ir(hmv_arena a64, hmv_arena alow, const double *b, double *u, double eps,
+                     double inner_eps, int m, int max_outer, int max_inner, int check_every,
+                     hmv_column_info *inst, double *inst_seconds, hmv_solve_info *info);
+
 const char *hmv_last_error(void);
 
 /* ---- device plumbing (buffers, copies, timing) ------------------------- */

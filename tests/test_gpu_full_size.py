@@ -307,19 +307,25 @@ def test_batch48_bicgstab(gpu, O, batch48, batch48_f64):
         assert err < _cond_bound(hm.nd)
 
 
-@pytest.mark.parametrize("prec,inner", [("bf16", 1e-2), ("f32", 1e-4)])
-def test_batch48_mixed_refinement(gpu, O, batch48, batch48_f64, prec, inner):
+@pytest.mark.parametrize("prec,method,inner", [("bf16", "bicgstab", 1e-2), ("f32", "bicgstab", 1e-4),
+                                               ("bf16_all", "gcr", 5e-3)])
+def test_batch48_mixed_refinement(gpu, O, batch48, batch48_f64, prec, method, inner):
     """The batch with mixed-precision factors: per-instance FP64 refinement
-    (hmv_multi_bicgstab_ir) over one low-precision multi-instance arena;
-    every instance reaches a true 1e-8, sampled instances checked by the
-    oracle's apply and against the oracle's FP64 solution."""
+    with BiCGSTAB (hmv_multi_bicgstab_ir) or GCR(32) (hmv_multi_gcr_ir) inner
+    solves over one low-precision multi-instance arena; every instance
+    reaches a true 1e-8, sampled instances checked by the oracle's apply and
+    against the oracle's FP64 solution."""
     mlow = gpu.MultiArena(batch48, prec)
     b = np.ones(batch48_f64.nd)
-    u, inst, secs, info = gpu.multi_bicgstab_ir(batch48_f64, mlow, b, eps=1e-8, inner_eps=inner,
-                                                max_outer=40, max_inner=3000)
+    if method == "gcr":
+        u, inst, secs, info = gpu.multi_gcr_ir(batch48_f64, mlow, b, eps=1e-8, inner_eps=inner, m=32,
+                                               max_outer=40, max_inner=3000)
+    else:
+        u, inst, secs, info = gpu.multi_bicgstab_ir(batch48_f64, mlow, b, eps=1e-8, inner_eps=inner,
+                                                    max_outer=40, max_inner=3000)
     mlow.free()
-    print("48 instances %s refinement: %.3f s, %d outer steps, worst relres %.2e, p50 %.3f s "
-          "p99 %.3f s" % (prec, info["seconds"], info["outer"], info["true_relres"],
+    print("48 instances %s %s refinement: %.3f s, %d outer steps, worst relres %.2e, p50 %.3f s "
+          "p99 %.3f s" % (prec, method, info["seconds"], info["outer"], info["true_relres"],
                           np.percentile(secs, 50), np.percentile(secs, 99)), flush=True)
     assert info["status_name"] == "converged"
     assert all(d["status_name"] == "converged" and d["true_relres"] < 1e-8 for d in inst)

@@ -1,5 +1,5 @@
 """Many independent instances in one arena (one launch per phase) and their
-per-instance BiCGSTAB solves, through the C ABI.
+per-instance BiCGSTAB and GCR(m) solves, through the C ABI.
 
 Bar: the concatenated apply and H-matmat equal each instance's own apply
 (FP64 <= 1e-13 relative per instance) and the oracle; each instance's solve
@@ -133,11 +133,51 @@ def test_multi_refinement_per_instance(gpu, O, instances, prec, inner):
     mlow.free()
 
 
+@pytest.mark.parametrize("prec,m", [("bf16_all", 32), ("f32", 32), ("bf16", 4)])
+def test_multi_gcr_refinement_per_instance(gpu, O, instances, prec, m):
+    """hmv_multi_gcr_ir: the same refinement with one GCR(m) per instance.
+    Every instance reaches a true 1e-8 (recomputed by the oracle) and its
+    FP64 solution within the conditioning bound; each instance takes the
+    inner steps of its own single-instance hmv_gcr_ir (the per-instance
+    dots sum other partial groupings, so up to 3 steps apart over all outer
+    steps); m = 4 reuses direction slots (the restart step).  A zero
+    right-hand side stops its instance at once and leaves it at zero."""
+    m64, mlow = gpu.MultiArena(instances, "f64"), gpu.MultiArena(instances, prec)
+    b = np.concatenate([np.ones(hm.nd) if i % 2 else gpu.uniform(40 + i, hm.nd)
+                        for i, hm in enumerate(instances)])
+    u, inst, secs, info = gpu.multi_gcr_ir(m64, mlow, b, eps=1e-8, inner_eps=5e-3, m=m,
+                                           max_outer=30, max_inner=500)
+    print(prec, m, info, [(d["nstp"], "%.1e" % d["true_relres"]) for d in inst], flush=True)
+    assert info["status_name"] == "converged" and info["outer"] >= 2
+    assert np.all(secs > 0) and np.all(secs <= info["seconds"] + 1e-9)
+    for i, hm in enumerate(instances):
+        lo, hi = m64.offsets[i], m64.offsets[i + 1]
+        tr = _rel(O.adot_body_lfmtx(hm, u[lo:hi]), b[lo:hi])
+        assert inst[i]["status_name"] == "converged" and tr < 1e-8
+        assert abs(tr - inst[i]["true_relres"]) <= 1e-3 * tr
+        u_ref, _, _ = O.bicgstab(hm, b[lo:hi], None, mstep=500, eps=1e-10)
+        assert _rel(u[lo:hi], u_ref) < _cond_bound(hm.nd)
+        a64, alow = gpu.Arena(hm, "f64"), gpu.Arena(hm, prec)
+        _, inf1, _ = gpu.gcr_ir(a64, alow, b[lo:hi], eps=1e-8, inner_eps=5e-3, m=m)
+        assert abs(inf1["inner_total"] - inst[i]["nstp"]) <= 3, (i, inf1, inst[i])
+        a64.free()
+        alow.free()
+    b[m64.offsets[1]:m64.offsets[2]] = 0.0
+    u, inst, secs, info = gpu.multi_gcr_ir(m64, mlow, b, eps=1e-8, inner_eps=5e-3, m=m)
+    assert inst[1]["status_name"] == "nonfinite" and inst[1]["nstp"] == 0
+    assert np.all(u[m64.offsets[1]:m64.offsets[2]] == 0.0)
+    assert all(inst[i]["status_name"] == "converged" for i in (0, 2, 3, 4))
+    m64.free()
+    mlow.free()
+
+
 def test_multi_refinement_rejects_other_instances(gpu, instances):
     m64 = gpu.MultiArena(instances, "f64")
     other = gpu.MultiArena(instances[:3], "bf16")
     with pytest.raises(gpu.HmvError, match="same instances"):
         gpu.multi_bicgstab_ir(m64, other, np.ones(m64.nd))
+    with pytest.raises(gpu.HmvError, match="same instances"):
+        gpu.multi_gcr_ir(m64, other, np.ones(m64.nd))
     m64.free()
     other.free()
 
