@@ -704,12 +704,20 @@ int hmv_apply_host(hmv_arena a, const double *zu, double *zau) {
   }
   hipError_t e;
   const size_t n = (size_t)a->nd, nb = sizeof(double) * n;
-  if (!a->d_hx) {
+  if (!a->hstream) {  // created last: set only when every buffer exists
     if ((e = hipMalloc(&a->d_hx, nb)) != hipSuccess || (e = hipMalloc(&a->d_hy, nb)) != hipSuccess ||
         (e = hipHostMalloc((void **)&a->h_x, nb, hipHostMallocDefault)) != hipSuccess ||
         (e = hipHostMalloc((void **)&a->h_y, nb, hipHostMallocDefault)) != hipSuccess ||
-        (e = hipStreamCreateWithFlags(&a->hstream, hipStreamNonBlocking)) != hipSuccess)
+        (e = hipStreamCreateWithFlags(&a->hstream, hipStreamNonBlocking)) != hipSuccess) {
+      // leave nothing half set up: the next call starts again
+      (void)hipFree(a->d_hx);
+      (void)hipFree(a->d_hy);
+      (void)hipHostFree(a->h_x);
+      (void)hipHostFree(a->h_y);
+      a->d_hx = a->d_hy = a->h_x = a->h_y = nullptr;
+      a->hstream = nullptr;
       return hip_fail(e, "hmv_apply_host: setup");
+    }
   }
   // two copies over PCIe, both from pinned staging: zu up, H zu down; the
   // CPU's passes over the caller's arrays (zu into staging, zau += H zu out
