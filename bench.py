@@ -486,6 +486,17 @@ def study_327680(H, O, log):
             "peak; floor %.3f ms (%s) -> %.0f%% of roofline"
             % (k, ms, ms / k, single_ms * k / ms, tf, 100 * tf / FP64_PEAK_TFLOPS, floor,
                mm[str(k)]["bound"], 100 * floor / ms))
+        # the path autotuner (vector units vs MFMA, MFMA tiles per unit); the
+        # block solves below run with what it picks
+        dflt = a64.block_tuning(k)
+        best, best_ms, dflt_ms = a64.autotune_block(k, reps=5)
+        tuned_ms = a64.apply_block_timed(X.ptr, Y.ptr, k, 5)
+        mm[str(k)]["autotune"] = {"default_path": dflt, "best_path": best,
+                                  "default_ms": round(dflt_ms, 4), "best_ms": round(best_ms, 4),
+                                  "tuned_ms": round(tuned_ms, 4),
+                                  "tuned_roofline_frac": round(floor / tuned_ms, 3)}
+        log("s=7 H-matmat k=%d autotuned: %s -> %s, %.3f ms (default path %.3f ms)"
+            % (k, dflt, best, tuned_ms, dflt_ms))
         X.free()
         Y.free()
     out["hmatmat_fp64"] = mm

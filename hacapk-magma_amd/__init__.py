@@ -131,6 +131,14 @@ class Tuning(C.Structure):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
 
 
+class BlockTuning(C.Structure):
+    """hmv_block_tuning: the H-matmat path for a class of k."""
+    _fields_ = [("vector", C.c_int), ("tiles_per_unit", C.c_int)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
 WORK_DTYPE = np.dtype([("off0", "<i8"), ("off1", "<i8"), ("rows", "<i4"), ("rows_p", "<i4"),
                        ("ncol0", "<i4"), ("ncol1", "<i4"), ("dt0", "<i2"), ("dt1", "<i2"),
                        ("zseg_begin", "<i4"), ("zseg_end", "<i4"), ("out_base", "<i4")])
@@ -258,6 +266,9 @@ def hmv_lib():
         L.hmv_multi_bicgstab.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int,
                                          C.POINTER(ColumnInfo), _D, C.POINTER(SolveInfo)]
         L.hmv_apply_block_timed.argtypes = [vp, C.c_int, vp, vp, C.c_int, _D]
+        L.hmv_block_tuning_get.argtypes = [vp, C.c_int, C.POINTER(BlockTuning)]
+        L.hmv_block_tuning_set.argtypes = [vp, C.c_int, C.POINTER(BlockTuning)]
+        L.hmv_autotune_block.argtypes = [vp, C.c_int, C.c_int, C.POINTER(BlockTuning), _D, _D]
         L.hmv_block_bicgstab.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_int, C.c_int,
                                          C.POINTER(ColumnInfo), C.POINTER(SolveInfo)]
         L.hmv_arena_tuning_get.argtypes = [vp, C.POINTER(Tuning)]
@@ -794,6 +805,26 @@ class Arena:
         dx, dy = DeviceBuffer.from_array(X.reshape(-1)), DeviceBuffer.from_array(np.zeros(X.size))
         self.apply_block(dx.ptr, dy.ptr, k)
         return dy.download().reshape(self.nd, k)
+
+    def block_tuning(self, k):
+        """The H-matmat path for k right-hand sides (hmv_block_tuning_get)."""
+        t = BlockTuning()
+        _check(hmv_lib().hmv_block_tuning_get(self._a, int(k), C.byref(t)), "hmv_block_tuning_get")
+        return t.as_dict()
+
+    def set_block_tuning(self, k, vector, tiles_per_unit=0):
+        """Sets the H-matmat path of k's classes (hmv_block_tuning_set)."""
+        t = BlockTuning(int(vector), int(tiles_per_unit))
+        _check(hmv_lib().hmv_block_tuning_set(self._a, int(k), C.byref(t)), "hmv_block_tuning_set")
+
+    def autotune_block(self, k, reps=5):
+        """hmv_autotune_block: sets and returns the fastest H-matmat path for
+        k right-hand sides, with its ms per product and the default's."""
+        t = BlockTuning()
+        best, dflt = C.c_double(), C.c_double()
+        _check(hmv_lib().hmv_autotune_block(self._a, int(k), int(reps), C.byref(t), C.byref(best),
+                                            C.byref(dflt)), "hmv_autotune_block")
+        return t.as_dict(), best.value, dflt.value
 
     def apply_block_timed(self, x_ptr, y_ptr, k, reps):
         ms = C.c_double()

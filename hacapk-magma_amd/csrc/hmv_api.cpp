@@ -119,14 +119,22 @@ int ensure_tk(hmv_arena_s *a, int k) {
   return HMV_OK;
 }
 
-hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
-                               const int *skip, hipStream_t s) {
-  HmvUnits u;
+void fill_units(const hmv_arena_s *a, HmvUnits &u) {
   for (int p = 0; p < 2; ++p)
     for (int w = 0; w < HMV_MM_WIDTHS; ++w) {
       u.u[p][w] = a->d_units[p][w];
       u.n[p][w] = a->nunits[p][w];
     }
+  for (int c = 0; c < 4; ++c) {
+    u.vec[c] = a->bt_vec[c];
+    u.log_tw[c] = a->bt_log_tw[c];
+  }
+}
+
+hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
+                               const int *skip, hipStream_t s) {
+  HmvUnits u;
+  fill_units(a, u);
   return hmv_launch_block(&a->launch, &u, k, x, a->d_tk, y, beta, skip, s);
 }
 
@@ -694,6 +702,108 @@ int hmv_apply_block_timed(hmv_arena a, int k, const double *x, double *y, int re
   if (s) (void)hipStreamDestroy(s);
   if (e != hipSuccess) return hip_fail(e, "hmv_apply_block_timed");
   if (ms) *ms = (double)m / reps;
+  return HMV_OK;
+}
+
+namespace {
+// right-hand-side classes of the H-matmat path choices
+int rhs_class(int k) { return k <= 1 ? 0 : k <= 2 ? 1 : k <= 4 ? 2 : 3; }
+int tile_class(int k) {
+  const int kt = (k + 15) / 16;
+  return kt == 1 ? 0 : kt == 2 ? 1 : kt <= 4 ? 2 : 3;
+}
+int vec_limit(const hmv_arena_s *a) {
+  return a->mode == 0 ? HMV_KSTREAM_MAX_F64 : HMV_KSTREAM_MAX_NARROW;
+}
+// tiles per unit instantiated for a tile class, the default first (the
+// MFMA kernel's dispatch in hmv_matmat.hip)
+std::vector<int> tw_choices(const hmv_arena_s *a, int tc) {
+  if (tc == 0) return {4, 2};
+  if (tc < 3) return {2, 1};
+  return a->mode == 0 ? std::vector<int>{1, 2} : std::vector<int>{2, 1};  // 5-8 tiles: FP64 1, else 2
+}
+int tw_of(const hmv_arena_s *a, int tc) {
+  return a->bt_log_tw[tc] < 0 ? tw_choices(a, tc)[0] : 1 << a->bt_log_tw[tc];
+}
+bool block_tuning_args(hmv_arena a, int k, const void *t, const char *who) {
+  if (!a || !t || k < 1 || k > HMV_MAX_RHS) {
+    set_error(std::string(who) + ": invalid argument (k must be 1..128)");
+    return false;
+  }
+  return true;
+}
+}  // namespace
+
+int hmv_block_tuning_get(hmv_arena a, int k, hmv_block_tuning *t) {
+  if (!block_tuning_args(a, k, t, "hmv_block_tuning_get")) return HMV_ERR_ARG;
+  t->vector = k <= vec_limit(a) && a->bt_vec[rhs_class(k)] ? 1 : 0;
+  t->tiles_per_unit = tw_of(a, tile_class(k));
+  return HMV_OK;
+}
+
+int hmv_block_tuning_set(hmv_arena a, int k, const hmv_block_tuning *t) {
+  if (!block_tuning_args(a, k, t, "hmv_block_tuning_set")) return HMV_ERR_ARG;
+  if (t->vector && k > vec_limit(a)) {
+    set_error("hmv_block_tuning_set: the vector path takes k <= " + std::to_string(vec_limit(a)) +
+              " on this arena");
+    return HMV_ERR_ARG;
+  }
+  const int tc = tile_class(k);
+  const std::vector<int> ch = tw_choices(a, tc);
+  if (t->tiles_per_unit != 0 && std::find(ch.begin(), ch.end(), t->tiles_per_unit) == ch.end()) {
+    set_error("hmv_block_tuning_set: tiles_per_unit must be 0, " + std::to_string(ch[0]) + " or " +
+              std::to_string(ch[1]) + " for k = " + std::to_string(k));
+    return HMV_ERR_ARG;
+  }
+  if (k <= vec_limit(a)) a->bt_vec[rhs_class(k)] = t->vector ? 1 : 0;
+  a->bt_log_tw[tc] = t->tiles_per_unit == 0 ? -1 : t->tiles_per_unit == 4 ? 2 : t->tiles_per_unit / 2;
+  return HMV_OK;
+}
+
+int hmv_autotune_block(hmv_arena a, int k, int reps, hmv_block_tuning *best, double *best_ms,
+                       double *default_ms) {
+  if (!block_tuning_args(a, k, best, "hmv_autotune_block")) return HMV_ERR_ARG;
+  if (reps < 1) {
+    set_error("hmv_autotune_block: reps must be >= 1");
+    return HMV_ERR_ARG;
+  }
+  const size_t bytes = sizeof(double) * (size_t)a->nd * k;
+  double *x = nullptr, *y = nullptr;
+  hipError_t e;
+  if ((e = hipMalloc(&x, bytes)) != hipSuccess || (e = hipMalloc(&y, bytes)) != hipSuccess ||
+      (e = hipMemset(x, 0, bytes)) != hipSuccess) {
+    (void)hipFree(x);
+    (void)hipFree(y);
+    return hip_fail(e, "hmv_autotune_block: buffers");
+  }
+  // candidates: the defaults first, then every other choice of k's classes
+  const int tc = tile_class(k);
+  const std::vector<int> ch = tw_choices(a, tc);
+  std::vector<hmv_block_tuning> cand;
+  const bool vec_ok = k <= vec_limit(a);
+  cand.push_back(hmv_block_tuning{vec_ok ? 1 : 0, ch[0]});
+  if (vec_ok) cand.push_back(hmv_block_tuning{0, ch[0]});
+  cand.push_back(hmv_block_tuning{0, ch[1]});
+  int rc = HMV_OK, bi = -1;
+  double bms = 0.0, dms = 0.0;
+  for (size_t i = 0; i < cand.size() && rc == HMV_OK; ++i) {
+    double ms = 0.0;
+    if ((rc = hmv_block_tuning_set(a, k, &cand[i])) != HMV_OK ||
+        (rc = hmv_apply_block_timed(a, k, x, y, reps, &ms)) != HMV_OK)
+      break;
+    if (i == 0) dms = ms;
+    if (bi < 0 || ms < bms) {
+      bi = (int)i;
+      bms = ms;
+    }
+  }
+  (void)hipFree(x);
+  (void)hipFree(y);
+  if (rc != HMV_OK) return rc;
+  if ((rc = hmv_block_tuning_set(a, k, &cand[bi])) != HMV_OK) return rc;
+  *best = cand[bi];
+  if (best_ms) *best_ms = bms;
+  if (default_ms) *default_ms = dms;
   return HMV_OK;
 }
 

@@ -20,6 +20,13 @@ struct HmvSolveWork;
 struct HmvUnits {
   const int2 *u[2][HMV_MM_WIDTHS];
   int n[2][HMV_MM_WIDTHS];
+  // Path choices per right-hand-side class (set by hmv_block_tuning_set /
+  // hmv_autotune_block): vec[c] = 1 runs k <= 8 in class c (k <= 1, 2, 4,
+  // 8) on the vector units; log_tw[c] = log2 of the 16-row tiles per MFMA
+  // unit for KT class c (1, 2, 3-4, 5-8 tiles of 16 right-hand sides), -1
+  // for the kernel's default.
+  int vec[4];
+  int log_tw[4];
 };
 
 struct hmv_arena_s {
@@ -43,6 +50,8 @@ struct hmv_arena_s {
   int nunits[2][HMV_MM_WIDTHS] = {};
   double *d_tk = nullptr;  // t buffer of the H-matmat (tpart_size * k doubles)
   int64_t tk_cap = 0;      // its capacity in doubles
+  int bt_vec[4] = {1, 1, 1, 1};      // H-matmat path choices (HmvUnits::vec, log_tw)
+  int bt_log_tw[4] = {-1, -1, -1, -1};
   std::vector<int> seg;    // instance row offsets (ninst + 1) of a multi-instance arena
   int *d_winst1 = nullptr, *d_winst2 = nullptr;  // instance of each work item (multi only)
   hmv_tuning tuning{};     // the tuning the arena was built with
@@ -69,6 +78,8 @@ hipError_t enqueue_apply(hmv_arena_s *a, const double *x, double *y, double beta
                          hipStream_t s, const int *inst_done = nullptr);
 void free_solve_work(hmv_arena_s *a);
 void free_gcr_work(hmv_arena_s *a);
+// The arena's H-matmat unit lists and path choices.
+void fill_units(const hmv_arena_s *a, HmvUnits &u);
 // Enqueues Y = H X + beta*Y for k interleaved right-hand sides (1..128).
 hipError_t enqueue_apply_block(hmv_arena_s *a, int k, const double *x, double *y, double beta,
                                const int *skip, hipStream_t s);

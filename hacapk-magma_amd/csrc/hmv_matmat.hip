@@ -103,7 +103,7 @@ struct Entry<HMV_DT_BF16> {
 // accumulators (twice the registers) are faster one tile per wave (5.2
 // against 5.9 ms).
 template <int KT, typename Acc>
-struct Shape {
+struct Shape {  // default tiles per unit (TW) by RHS tile count and accumulator type
   static constexpr int TW = KT <= 1 ? 4 : KT <= 4 ? 2 : sizeof(Acc) == 4 ? 2 : 1;
   static constexpr int LOG_TW = TW == 4 ? 2 : TW == 2 ? 1 : 0;
 };
@@ -282,14 +282,14 @@ __device__ inline void unit_mma(typename Mfma<Acc>::acc (&acc)[TW][KT], const Hm
 // One wave per unit {item, tile0 << 8 | nt}: tiles tile0 .. tile0+nt-1 of
 // the item.  PHASE 1: t[outidx[out_base + row] * k + j];  PHASE 2:
 // y[(out_base + row) * k + j].
-template <typename Acc, int PHASE, int DT0, int DT1, int KT, bool EXACT>
+template <typename Acc, int PHASE, int DT0, int DT1, int KT, int TW, bool EXACT>
 __global__ __launch_bounds__(64) void hmv_matmat_kernel(
     const char *__restrict__ arena, const HmvWork *__restrict__ works, const int2 *__restrict__ units,
     const HmvZSeg *__restrict__ zsegs, const double *__restrict__ x, const int *__restrict__ codes,
     const double *__restrict__ t_in, const int *__restrict__ outidx, double *__restrict__ t_out,
     double *__restrict__ y, int k, double beta, const int *__restrict__ skip) {
   if (skip && *skip) return;
-  constexpr int TW = Shape<KT, Acc>::TW;
+  static_assert(TW == 1 || TW == 2 || TW == 4, "tiles per unit");
   __shared__ int runs[2 * HMV_RUN_MAX];
   __shared__ gdouble *tbl[PHASE == 1 ? 1 : kWin + kPad];  // phase 1 reads its x run directly
   const int2 unit = units[blockIdx.x];
@@ -374,28 +374,44 @@ void launch_units(int kt, const HmvLaunch &L, const HmvUnits &U, const HmvWork *
                   hipStream_t s) {
   const double *t_in = PHASE == 2 ? t : nullptr;
   double *t_out = PHASE == 1 ? t : nullptr;
-#define HMV_MM_GO(KT, EX)                                                                      \
-  hipLaunchKernelGGL((hmv_matmat_kernel<Acc, PHASE, DT0, DT1, KT, EX>), dim3(n), dim3(64), 0, s, \
-                     L.arena, works, U.u[PHASE - 1][w], L.zseg, x, L.codes, t_in, L.outidx, t_out,  \
-                     y, k, beta, skip)
-#define HMV_MM_LAUNCH(KT)                              \
-  do {                                                 \
-    const int w = Shape<KT, Acc>::LOG_TW, n = U.n[PHASE - 1][w]; \
-    if (n > 0) {                                       \
-      if (k == 16 * KT)                                \
-        HMV_MM_GO(KT, true);                           \
-      else                                             \
-        HMV_MM_GO(KT, false);                          \
-    }                                                  \
+#define HMV_MM_GO(KT, TW, EX)                                                                  \
+  hipLaunchKernelGGL((hmv_matmat_kernel<Acc, PHASE, DT0, DT1, KT, TW, EX>), dim3(n), dim3(64), 0, \
+                     s, L.arena, works, U.u[PHASE - 1][w], L.zseg, x, L.codes, t_in, L.outidx,      \
+                     t_out, y, k, beta, skip)
+#define HMV_MM_LAUNCH(KT, TW)                                      \
+  do {                                                             \
+    const int w = TW == 4 ? 2 : TW == 2 ? 1 : 0, n = U.n[PHASE - 1][w]; \
+    if (n > 0) {                                                   \
+      if (k == 16 * KT)                                            \
+        HMV_MM_GO(KT, TW, true);                                   \
+      else                                                         \
+        HMV_MM_GO(KT, TW, false);                                  \
+    }                                                              \
   } while (0)
-  if (kt == 1)
-    HMV_MM_LAUNCH(1);
-  else if (kt == 2)
-    HMV_MM_LAUNCH(2);
-  else if (kt <= 4)
-    HMV_MM_LAUNCH(4);
-  else
-    HMV_MM_LAUNCH(8);
+  // tiles per unit: the tuned choice of the KT class when it is one of the
+  // two instantiated for it, else the default (Shape)
+  const int lt = U.log_tw[kt == 1 ? 0 : kt == 2 ? 1 : kt <= 4 ? 2 : 3];
+  if (kt == 1) {
+    if (lt == 1 || (lt < 0 && Shape<1, Acc>::TW == 2))
+      HMV_MM_LAUNCH(1, 2);
+    else
+      HMV_MM_LAUNCH(1, 4);
+  } else if (kt == 2) {
+    if (lt == 0 || (lt < 0 && Shape<2, Acc>::TW == 1))
+      HMV_MM_LAUNCH(2, 1);
+    else
+      HMV_MM_LAUNCH(2, 2);
+  } else if (kt <= 4) {
+    if (lt == 0 || (lt < 0 && Shape<4, Acc>::TW == 1))
+      HMV_MM_LAUNCH(4, 1);
+    else
+      HMV_MM_LAUNCH(4, 2);
+  } else {
+    if (lt == 1 || (lt < 0 && Shape<8, Acc>::TW == 2))
+      HMV_MM_LAUNCH(8, 2);
+    else
+      HMV_MM_LAUNCH(8, 1);
+  }
 #undef HMV_MM_LAUNCH
 #undef HMV_MM_GO
 }
@@ -404,8 +420,10 @@ template <typename Acc, int DT0, int DT1>
 void launch_block(const HmvLaunch &L, const HmvUnits &U, int k, const double *x, double *t,
                   double *y, double beta, const int *skip, hipStream_t s) {
   const int kt = (k + 15) / 16;
-  // few right-hand sides stream on the vector units (hmv_kstream.hip)
-  const bool vec = k <= (L.mode == 0 ? HMV_KSTREAM_MAX_F64 : HMV_KSTREAM_MAX_NARROW);
+  // few right-hand sides stream on the vector units (hmv_kstream.hip) unless
+  // the arena's tuning sends their class to MFMA
+  const bool vec = k <= (L.mode == 0 ? HMV_KSTREAM_MAX_F64 : HMV_KSTREAM_MAX_NARROW) &&
+                   U.vec[k <= 1 ? 0 : k <= 2 ? 1 : k <= 4 ? 2 : 3];
   if (vec)
     (void)hmv_launch_kstream(&L, 1, k, x, t, nullptr, 0.0, skip, s);
   else
@@ -511,11 +529,7 @@ extern "C" int hmv_apply_block_static(hmv_arena a, int k, const double *x, doubl
   int rc = hmv::static_scratch(&t);
   if (rc) return rc;
   HmvUnits u;
-  for (int p = 0; p < 2; ++p)
-    for (int w = 0; w < HMV_MM_WIDTHS; ++w) {
-      u.u[p][w] = a->d_units[p][w];
-      u.n[p][w] = a->nunits[p][w];
-    }
+  hmv::fill_units(a, u);
   e = hmv_launch_block(&a->launch, &u, k, x, t, y, beta, nullptr, (hipStream_t)stream);
   if (e != hipSuccess) return hmv::hip_fail(e, "hmv_apply_block_static: launch");
   return HMV_OK;

@@ -117,6 +117,28 @@ int hmv_arena_tuning_get(hmv_arena a, hmv_tuning *t);
 int hmv_autotune(int nd, int nlf, const void *leaves, int leaf_stride, int precision, int reps,
                  hmv_tuning *best, double *best_ms, double *default_ms);
 
+/* Path of the H-matmat (hmv_apply_block) for k right-hand sides, per
+ * arena: vector = 1 streams k <= 8 (FP64 arenas) or k <= 4 (FP32 / BF16) on
+ * the vector units, else MFMA; tiles_per_unit = 16-row tiles one MFMA wave
+ * takes per work unit (1, 2 or 4; the choices for ceil(k/16) = 1 are 2 and 4,
+ * else 1 and 2; 0 = the default).  A setting holds for every k of the same
+ * class: the vector choice per k <= 1, 2, 4, 8, the tiles per unit per
+ * ceil(k/16) = 1, 2, 3-4, 5-8.  Results of the paths agree to rounding.  No
+ * reference counterpart. */
+typedef struct {
+  int vector;
+  int tiles_per_unit;
+} hmv_block_tuning;
+int hmv_block_tuning_get(hmv_arena a, int k, hmv_block_tuning *t);
+int hmv_block_tuning_set(hmv_arena a, int k, const hmv_block_tuning *t);
+/* Autotuner of the H-matmat path for k right-hand sides: times the
+ * default path and every other choice of k's classes (`reps` products each
+ * after one untimed, HIP events, zero inputs), sets the fastest on the
+ * arena and returns it; *best_ms and *default_ms are the times per product.
+ * Needs device memory for 2 x nd x k doubles. */
+int hmv_autotune_block(hmv_arena a, int k, int reps, hmv_block_tuning *best, double *best_ms,
+                       double *default_ms);
+
 /* Many independent instances in one arena: instance i (nd[i] unknowns,
  * nlf[i] leaves at leaves[i], all with the same record stride) occupies rows
  * and columns [off_i, off_i + nd[i]) of one block-diagonal H-matrix, off_i =

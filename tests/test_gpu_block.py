@@ -49,6 +49,68 @@ def test_matmat_low_precision(gpu, O, two_spheres4, prec, k):
     a.free()
 
 
+TOL = {"f64": 1e-12, "f32": 1e-5, "bf16": 2e-2, "bf16_all": 2e-2}
+
+
+def _paths(prec, k):
+    """Every H-matmat path choice of k's classes: (vector, tiles per unit)."""
+    vec_max = 8 if prec == "f64" else 4
+    kt = (k + 15) // 16
+    tiles = [4, 2] if kt == 1 else [2, 1] if kt <= 4 else ([1, 2] if prec == "f64" else [2, 1])
+    out = [(1, tiles[0])] if k <= vec_max else []
+    return out + [(0, t) for t in tiles]
+
+
+@pytest.mark.parametrize("prec,k", [("f64", 3), ("f64", 8), ("f64", 16), ("f64", 40), ("f64", 128),
+                                    ("f32", 4), ("f32", 72), ("bf16_all", 128)])
+def test_matmat_every_path(gpu, O, two_spheres4, prec, k):
+    """Each path the block tuning can select (vector units, MFMA with each
+    instantiated tiles-per-unit width) gives the oracle's product; the
+    arena reports the path it was set to."""
+    hm = two_spheres4
+    a = gpu.Arena(hm, prec)
+    X = np.stack([gpu.uniform(300 + j, hm.nd) for j in range(k)], axis=1)
+    for vec, tiles in _paths(prec, k):
+        a.set_block_tuning(k, vec, tiles)
+        assert a.block_tuning(k) == {"vector": vec, "tiles_per_unit": tiles}
+        Y = a.matmat(X)
+        for j in (0, k - 1):
+            assert _rel(Y[:, j], O.adot_body_lfmtx(hm, X[:, j])) <= TOL[prec], (vec, tiles, j)
+    a.free()
+
+
+@pytest.mark.parametrize("prec,k", [("f64", 8), ("f64", 32), ("f32", 4)])
+def test_block_autotune(gpu, O, sphere4, prec, k):
+    """hmv_autotune_block sets the fastest path of k's classes: no slower
+    than the defaults, reported back by the arena, and its product matches
+    the oracle."""
+    a = gpu.Arena(sphere4, prec)
+    dflt = a.block_tuning(k)
+    best, best_ms, dflt_ms = a.autotune_block(k, reps=3)
+    print(prec, k, "default", dflt, "best", best, "%.4f ms (default %.4f ms)" % (best_ms, dflt_ms),
+          flush=True)
+    assert 0 < best_ms <= dflt_ms
+    assert a.block_tuning(k) == best
+    X = np.stack([gpu.uniform(400 + j, sphere4.nd) for j in range(k)], axis=1)
+    Y = a.matmat(X)
+    assert _rel(Y[:, k - 1], O.adot_body_lfmtx(sphere4, X[:, k - 1])) <= TOL[prec]
+    a.free()
+
+
+def test_block_tuning_rejects(gpu, sphere3):
+    a, n = gpu.Arena(sphere3, "f64"), gpu.Arena(sphere3, "f32")
+    with pytest.raises(gpu.HmvError, match="vector path"):
+        a.set_block_tuning(16, 1)
+    with pytest.raises(gpu.HmvError, match="vector path"):
+        n.set_block_tuning(8, 1)
+    with pytest.raises(gpu.HmvError, match="tiles_per_unit"):
+        a.set_block_tuning(32, 0, 4)
+    with pytest.raises(gpu.HmvError):
+        a.block_tuning(129)
+    a.free()
+    n.free()
+
+
 def test_matmat_beta_accumulates(gpu, sphere4):
     a = gpu.Arena(sphere4, "f64")
     k = 8
