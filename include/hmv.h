@@ -133,7 +133,7 @@ int hmv_block_tuning_get(hmv_arena a, int k, hmv_block_tuning *t);
 int hmv_block_tuning_set(hmv_arena a, int k, const hmv_block_tuning *t);
 /* Autotuner of the H-matmat path for k right-hand sides: times the
  * default path and every other choice of k's classes (`reps` products each
- * after one untimed, HIP events, zero inputs), sets the fastest on the
+ * after one untimed, HIP events, inputs uniform in [-1, 1)), sets the fastest on the
  * arena and returns it; *best_ms and *default_ms are the times per product.
  * Needs device memory for 2 x nd x k doubles. */
 int hmv_autotune_block(hmv_arena a, int k, int reps, hmv_block_tuning *best, double *best_ms,
