@@ -704,6 +704,7 @@ int hmv_apply_block_timed(hmv_arena a, int k, const double *x, double *y, int re
   if (ms) *ms = (double)m / reps;
   return HMV_OK;
 }
+}  // extern "C"
 
 namespace {
 // right-hand-side classes of the H-matmat path choices
@@ -734,6 +735,7 @@ bool block_tuning_args(hmv_arena a, int k, const void *t, const char *who) {
 }
 }  // namespace
 
+extern "C" {
 int hmv_block_tuning_get(hmv_arena a, int k, hmv_block_tuning *t) {
   if (!block_tuning_args(a, k, t, "hmv_block_tuning_get")) return HMV_ERR_ARG;
   t->vector = k <= vec_limit(a) && a->bt_vec[rhs_class(k)] ? 1 : 0;
