@@ -55,6 +55,11 @@ def ref_lib():
         L.ref_minabsvalloc_d.argtypes = [C.c_int, _D, _D, C.POINTER(C.c_int)]
         L.ref_face_integral.argtypes = [_D, _D, _D, C.c_double, C.c_double, C.c_double]
         L.ref_face_integral.restype = C.c_double
+        if hasattr(L, "ref_acaplus"):  # HACApK_acaplus (ref_aca_bind.f90)
+            _I = C.POINTER(C.c_int)
+            L.ref_acaplus.argtypes = [C.c_int, _D, C.c_int, _I, _I, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, _D, _D, _D]
+            L.ref_acaplus.restype = C.c_int
         _ref = L
     return _ref
 

@@ -11,7 +11,12 @@
 #    plus oracle/ref_bind.f90, a bind(C) binding that lets the tests call the
 #    assumed-shape routines of HACApK_lib.f90 from C.  These four files
 #    compile unmodified with flang; the rest of the reference needs MPI or
-#    MAGMA and is not built.
+#    MAGMA and is not built, except:
+#      ACA+: HACApK_calc_vec and HACApK_acaplus (m_HACApK_base.f90:677-699,
+#      783-975), which use no MPI, are cut out of the preprocessed module
+#      together with its header and interface block into a temporary
+#      module ref_aca_mod (text unmodified; only the module name differs),
+#      and oracle/ref_aca_bind.f90 forwards a C call to HACApK_acaplus.
 #
 # 2. ref_layout_magma.txt, ref_layout_plain.txt: the byte layout flang gives
 #    the reference's declarations of st_HACApK_leafmtx / st_HACApK_leafmtxp
@@ -32,16 +37,41 @@ OUT="$HERE/_ref"
 LIBDIR="$HERE/../hacapk-magma_amd/lib"
 FLANG="${FLANG:-/opt/rocm/lib/llvm/bin/flang}"
 mkdir -p "$OUT/mod"
+TMP="$(mktemp -d)"
+trap 'rm -rf "$TMP"' EXIT
+: >"$OUT/build.log"
+
+# ACA+ routines of m_HACApK_base.f90 as module ref_aca_mod: the module
+# header (use, implicit), its interface block, HACApK_calc_vec and
+# HACApK_acaplus, from the preprocessed source (plain build).
+"$FLANG" -E -cpp "$REF/m_HACApK_base.f90" 2>>"$OUT/build.log" |
+  awk '
+    /^[ \t]*module[ \t]+m_HACApK_base[ \t]*$/ { print "module ref_aca_mod"; hdr = 1; next }
+    hdr && /^[ \t]*type[ \t]*::/ { hdr = 0 }
+    hdr && !/^#/ { print; next }
+    !done_if && /^[ \t]*interface[ \t]*$/ { inif = 1 }
+    inif && !/^#/ { print }
+    inif && /^[ \t]*end[ \t]*interface[ \t]*$/ { inif = 0; done_if = 1; print "contains" }
+    /^[ \t]*subroutine[ \t]+HACApK_calc_vec[ \t]*\(/ { cv = 1 }
+    cv && !/^#/ { print }
+    cv && /^[ \t]*end[ \t]*subroutine/ { cv = 0 }
+    /^[ \t]*integer[ \t]+function[ \t]+HACApK_acaplus[ \t]*\(/ { ap = 1 }
+    ap && !/^#/ { print }
+    ap && /^[ \t]*end[ \t]*function/ { ap = 0 }
+    END { print "end module ref_aca_mod" }
+  ' >"$TMP/ref_aca_mod.f90"
+grep -q "function HACApK_acaplus" "$TMP/ref_aca_mod.f90"
+grep -q "subroutine HACApK_calc_vec" "$TMP/ref_aca_mod.f90"
+
 "$FLANG" -O2 -fPIC -shared -J "$OUT/mod" -o "$OUT/libhacapk_ref.so.tmp" \
   "$REF/m_ppohBEM_user_func.f90" \
   "$REF/m_ppohBEM_matrix_element_ij.f90" \
   "$REF/m_HACApK_calc_entry_ij.f90" \
   "$REF/HACApK_lib.f90" \
-  "$HERE/ref_bind.f90" 2>"$OUT/build.log"
+  "$HERE/ref_bind.f90" \
+  "$TMP/ref_aca_mod.f90" \
+  "$HERE/ref_aca_bind.f90" 2>>"$OUT/build.log"
 mv "$OUT/libhacapk_ref.so.tmp" "$OUT/libhacapk_ref.so"
-
-TMP="$(mktemp -d)"
-trap 'rm -rf "$TMP"' EXIT
 # the three derived types of m_HACApK_base.f90 as the given flags declare them
 extract_types() {  # $1 = output file, rest = preprocessor flags
   local out="$1"

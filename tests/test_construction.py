@@ -1,7 +1,8 @@
 """Pins the host input generator (libhmvgen) against the reference.
 
-* entry function and ACA helpers: bit-for-bit against the reference's own
-  Fortran sources compiled with flang (oracle/_ref), plus closed forms;
+* entry function, ACA helpers and ACA+ itself: bit-for-bit against the
+  reference's own Fortran sources compiled with flang (oracle/_ref), plus
+  closed forms;
 * cluster tree / block tree: the counts of the reference's recorded run on
   input_10ts.pbs (ivb/job_cpu.sh.o78632, ivb/log0000.txt), and invariants;
 * ACA+: error against the dense block within the tolerance.
@@ -152,6 +153,61 @@ def test_aca_error_within_tolerance(H):
             if len(errs) >= 60:
                 break
         assert np.median(errs) < eps and max(errs) < 20 * eps, (eps, np.median(errs), max(errs))
+
+
+def _ref_param(hm):
+    """The reference's param() slots that HACApK_acaplus reads (1-based)."""
+    p = hm.params.as_dict()
+    param = np.zeros(100)
+    param[60 - 1] = p["aca_kind"]
+    param[61 - 1] = p["aca_norm"]
+    param[63 - 1] = p["kmax"]
+    param[64 - 1] = p["kmin"]
+    param[71 - 1] = p["aca_eps"]
+    param[72 - 1] = p["aca_eps_factor"]
+    return param
+
+
+@pytest.mark.parametrize("mesh,eps", [("input_10ts", 1e-6), ("icosphere3", 1e-4)])
+def test_acaplus_bitexact_vs_reference(H, ref, mesh, eps):
+    """Every low-rank leaf of the host fill equals the reference's own
+    HACApK_acaplus (m_HACApK_base.f90:783-975, compiled from its source by
+    oracle/ref_build.sh) on the same block: the same rank and the same U and
+    V factors bit for bit.  The input_10ts mesh with leaf 15 / eta 2 / ACA+
+    1e-6 is the configuration of the reference's recorded run."""
+    if not hasattr(ref, "ref_acaplus"):
+        pytest.skip("oracle/_ref built without the ACA+ routine")
+    if mesh == "input_10ts":
+        d = np.load(os.path.join(HERE, "golden", "input_10ts_mesh.npz"))
+        nodes, faces = d["nodes"], d["faces"]
+        hm = H.HMatrix(nodes, faces, H.HParams(leaf_size=15, aca_eps=eps, kernel="ppohbem"))
+    else:
+        nodes, faces = H.icosphere(3)
+        hm = H.HMatrix(nodes, faces, H.baseline_params(aca_eps=eps, kernel="ppohbem"))
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    faces = np.ascontiguousarray(faces, dtype=np.int32)
+    lod = np.ascontiguousarray(hm.lod, dtype=np.int32)
+    param = _ref_param(hm)
+    kmax = int(param[62])
+    I = C.POINTER(C.c_int)
+    nlr = 0
+    kts = []
+    for l in hm.leaves():
+        if l.ltmtx != 1:
+            continue
+        zaa = np.zeros((kmax, l.ndl))
+        zab = np.zeros((kmax, l.ndt))
+        kt = ref.ref_acaplus(nodes.shape[0], _p(nodes), faces.shape[0], faces.ctypes.data_as(I),
+                             lod.ctypes.data_as(I), l.ndl, l.ndt, l.nstrtl, l.nstrtt, _p(param),
+                             _p(zaa), _p(zab))
+        assert kt == l.kt, (l.nstrtl, l.nstrtt, kt, l.kt)
+        V = np.ctypeslib.as_array(l.a1, shape=(l.kt * l.ndt,)).reshape(l.kt, l.ndt)
+        U = np.ctypeslib.as_array(l.a2, shape=(l.kt * l.ndl,)).reshape(l.kt, l.ndl)
+        assert np.array_equal(zab[:kt], V) and np.array_equal(zaa[:kt], U), (l.nstrtl, l.nstrtt)
+        nlr += 1
+        kts.append(kt)
+    assert nlr == hm.stats["nlfkt"] and nlr > 0
+    print(mesh, "low-rank leaves", nlr, "ranks", min(kts), "-", max(kts), flush=True)
 
 
 def test_fill_independent_of_thread_count(H):
