@@ -55,6 +55,11 @@ def test_apply_is_deterministic(gpu, sphere4):
     y1 = a.matvec(x)
     y2 = a.matvec(x)
     assert np.array_equal(y1, y2)
+    # into a caller-owned array (overwritten, as hmv_apply_host writes H x)
+    out = np.full(sphere4.nd, 7.0)
+    assert a.matvec(x, out=out) is out and np.array_equal(out, y1)
+    with pytest.raises(ValueError):
+        a.matvec(x, out=np.zeros(sphere4.nd + 1))
     a.free()
 
 
