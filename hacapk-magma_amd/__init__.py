@@ -785,16 +785,20 @@ class Arena:
         """y = H x + beta y on device pointers (asynchronous)."""
         _check(hmv_lib().hmv_apply(self._a, x_ptr, y_ptr, float(beta), stream), "hmv_apply")
 
-    def matvec(self, x, out=None):
-        """Host convenience: returns H x (x, result in cluster order); `out`
-        (a contiguous float64 array of x's length) receives it when given."""
+    def matvec(self, x):
+        """Host convenience: returns H x (x, result in cluster order)."""
         x = np.ascontiguousarray(x, np.float64)
-        if out is None:
-            out = np.zeros_like(x)
-        elif out.dtype != np.float64 or out.shape != x.shape or not out.flags.c_contiguous:
-            raise ValueError("out must be a contiguous float64 array of shape %s" % (x.shape,))
-        _check(hmv_lib().hmv_apply_host(self._a, _dptr(x), _dptr(out)), "hmv_apply_host")
-        return out
+        y = np.zeros_like(x)
+        _check(hmv_lib().hmv_apply_host(self._a, _dptr(x), _dptr(y)), "hmv_apply_host")
+        return y
+
+    def apply_host(self, zu, zau):
+        """zau += H zu on host arrays, in place (hmv_apply_host, the reference's
+        zau += H*zu); zu and zau contiguous float64 arrays of length nd."""
+        for v in (zu, zau):
+            if v.dtype != np.float64 or v.shape != (self.nd,) or not v.flags.c_contiguous:
+                raise ValueError("zu and zau must be contiguous float64 arrays of length %d" % self.nd)
+        _check(hmv_lib().hmv_apply_host(self._a, _dptr(zu), _dptr(zau)), "hmv_apply_host")
 
     def apply_block(self, x_ptr, y_ptr, k, beta=0.0, stream=None):
         """Y = H X + beta Y for k row-interleaved right-hand sides on device (asynchronous)."""
