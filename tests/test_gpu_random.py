@@ -1,0 +1,39 @@
+"""The random block structures of test_plan_random.py (ragged sizes, dense and
+low-rank leaves of every rank from 0, rows covered by no leaf) through the
+GPU kernels: the single-vector apply in every storage precision, with the
+default and a tight repack tuning, and the H-matmat on every path its
+tuning can select, against the oracle's apply (HACApK_MAGMA.c:5-57)."""
+import numpy as np
+import pytest
+
+from test_gpu_block import _paths
+from test_plan_random import TUNINGS, _leaf_set
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64": 1e-12, "f32": 1e-5, "bf16": 2e-2, "bf16_all": 2e-2}
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("seed", range(0, 24, 2))
+def test_random_structures_on_gpu(gpu, O, seed):
+    hm = _leaf_set(gpu, seed)
+    x = gpu.uniform(70 + seed, hm.nd)
+    y_ref = O.adot_body_lfmtx(hm, x)
+    for prec in TOL:
+        for tune in (None, TUNINGS[1]):
+            a = gpu.Arena(hm, prec, tuning=tune)
+            assert _rel(a.matvec(x), y_ref) <= TOL[prec], (seed, prec, tune)
+            if tune is None and prec in ("f64", "f32"):
+                for k in (3, 20):
+                    X = np.stack([gpu.uniform(400 + 10 * seed + j, hm.nd) for j in range(k)], axis=1)
+                    refs = {j: O.adot_body_lfmtx(hm, X[:, j]) for j in (0, k - 1)}
+                    for vec, tiles in _paths(prec, k):
+                        a.set_block_tuning(k, vec, tiles)
+                        Y = a.matmat(X)
+                        for j, r in refs.items():
+                            assert _rel(Y[:, j], r) <= TOL[prec], (seed, prec, k, vec, tiles, j)
+            a.free()
