@@ -79,6 +79,26 @@ def test_matmat_every_path(gpu, O, two_spheres4, prec, k):
     a.free()
 
 
+@pytest.mark.parametrize("prec,k", [("f64", 7), ("f64", 8), ("f64", 64), ("f64", 128),
+                                    ("f32", 128)])
+def test_matmat_inputs_at_any_alignment(gpu, sphere4, prec, k):
+    """Inputs that start 8 bytes past a 16-byte boundary (a caller's x may
+    start anywhere) take the paths that read inputs one value at a time
+    instead of in 16-byte pairs; the product is bit-equal to the aligned
+    one (the same values reach the same FMAs), and odd k uses the unpaired
+    staging too."""
+    a = gpu.Arena(sphere4, prec)
+    X = np.stack([gpu.uniform(700 + j, sphere4.nd) for j in range(k)], axis=1)
+    Y_al = a.matmat(X)
+    buf = gpu.DeviceBuffer.from_array(np.concatenate([[0.0], X.reshape(-1)]))
+    dy = gpu.DeviceBuffer.from_array(np.zeros(X.size))
+    a.apply_block(buf.ptr + 8, dy.ptr, k)
+    assert np.array_equal(dy.download().reshape(sphere4.nd, k), Y_al)
+    buf.free()
+    dy.free()
+    a.free()
+
+
 @pytest.mark.parametrize("prec,k", [("f64", 8), ("f64", 32), ("f32", 4)])
 def test_block_autotune(gpu, O, sphere4, prec, k):
     """hmv_autotune_block sets the fastest path of k's classes: no slower
