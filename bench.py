@@ -476,7 +476,8 @@ def study_327680(H, O, log):
     for k in (8, 32, 128):
         X = H.DeviceBuffer.from_array(rng.uniform(-1, 1, nd * k))
         Y = H.DeviceBuffer.from_array(np.zeros(nd * k))
-        ms = a64.apply_block_timed(X.ptr, Y.ptr, k, 5)
+        # median of 5 rounds of 5 products (each round after one untimed)
+        ms = float(np.median([a64.apply_block_timed(X.ptr, Y.ptr, k, 5) for _ in range(5)]))
         tf = 2.0 * E * k / ms / 1e9
         hbm_ms = (8.0 * E + 16.0 * nd * k) / (HBM_PEAK_GBS * 1e6)
         mfma_ms = 2.0 * E * k / (FP64_PEAK_TFLOPS * 1e9)
