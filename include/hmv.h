@@ -168,9 +168,12 @@ int hmv_apply(hmv_arena a, const double *x, double *y, double beta, void *stream
 
 /* H-matmat: Y = H X + beta*Y for k right-hand sides (1 <= k <= HMV_MAX_RHS)
  * on device arrays stored row-interleaved (N x k row-major: row i's k values
- * contiguous).  Runs on MFMA (FP64, or FP32 for FP32/BF16 arenas); each
- * column equals hmv_apply of that column to rounding.  No reference
- * counterpart (the reference applies one vector per call). */
+ * contiguous).  Few right-hand sides run on the vector units, more on MFMA
+ * (FP64, or FP32 for FP32/BF16 arenas; hmv_block_tuning); each column
+ * equals hmv_apply of that column to rounding.  x needs 8-byte alignment;
+ * a 16-byte aligned x (as hipMalloc returns) lets the kernels read inputs
+ * in 16-byte pairs, with the same result.  No reference counterpart (the
+ * reference applies one vector per call). */
 #define HMV_MAX_RHS 128
 int hmv_apply_block(hmv_arena a, int k, const double *x, double *y, double beta, void *stream);
 /* H-matmat that allocates nothing: as hmv_apply_block (same kernels and
