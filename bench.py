@@ -61,7 +61,8 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 
 def cpu_threads():
     """Threads of this process's CPU share: OMP_NUM_THREADS when set (the
-    share a GPU box gives one GPU), else the CPUs this process may run on."""
+    host cores set aside for one GPU on a shared node), else the CPUs this
+    process may run on."""
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
         return int(env)
