@@ -289,9 +289,9 @@ __device__ inline void kstream_item(const HmvWork &w, Acc *__restrict__ z, int *
 }
 
 // The kernels are fitted to 64 VGPRs (8 waves per SIMD, as the
-// single-vector kernel): with K >= 4 accumulators per row a lane keeps two
-// loads per batch and one prefetched column (kbatch, kprefetch), which
-// leaves every instantiation without spills.
+// single-vector kernel): the loads per batch, prefetched columns and
+// staging loads in flight (kbatch, kprefetch, kstage_batch) are chosen so
+// that every instantiation runs without spills.
 template <typename Acc, int PHASE, int DT0, int DT1, int NT, int K, bool PAIRS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 hmv_kstream_kernel(const char *__restrict__ arena, const HmvWork *__restrict__ works,

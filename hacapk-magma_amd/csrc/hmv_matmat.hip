@@ -127,8 +127,8 @@ __device__ __attribute__((aligned(16))) double g_zero_inputs[128];
 // tiles; from four tiles on the tiles go in pairs, tile 2s + b holding
 // 32 s + 2 j0 + b, so that a lane's inputs of the two tiles of a pair are
 // adjacent and one 16-byte load brings both (for k = 128 five loads per
-// column quad instead of nine: one for A, four for B; 4.64 against 5.02 ms
-// at 327,680 unknowns, 2.57 against 2.75 at k = 64, products bit-equal).
+// column quad instead of nine: one for A, four for B; 4.62 against 4.96 ms
+// at 327,680 unknowns, 2.53 against 2.72 at k = 64, products bit-equal).
 template <int KT>
 constexpr bool bpaired() {
   return KT >= 4;
