@@ -37,3 +37,30 @@ def test_random_structures_on_gpu(gpu, O, seed):
                         for j, r in refs.items():
                             assert _rel(Y[:, j], r) <= TOL[prec], (seed, prec, k, vec, tiles, j)
             a.free()
+
+
+@pytest.mark.parametrize("seed", range(1, 24, 4))
+def test_random_structures_through_reference_entry_points(gpu, O, seed):
+    """The same structures through the reference-named entry points on the
+    HAVE_MAGMA container (its MAGMA block filled with sentinel bytes):
+    lfcpy_batch_sorted_ + lfmtx_batch_ (zau += H zu), lfmtx_ and lfmtx_gpu_
+    (HACApK_MAGMA.c:5-57, 79-227), then lfdel_batch_; each product within
+    1e-12 of the oracle's."""
+    hm = _leaf_set(gpu, seed)
+    zu = gpu.uniform(90 + seed, hm.nd)
+    z0 = gpu.uniform(95 + seed, hm.nd)
+    ref = O.adot_body_lfmtx(hm, zu, z0.copy())
+    prev = gpu.set_ref_layout("magma")
+    try:
+        lp = gpu.make_leafmtxp(hm, "magma", fill=0x5A)
+        gpu.adot_body_lfcpy_batch_sorted(hm.nd, lp)
+        zau = z0.copy()
+        gpu.adot_body_lfmtx_batch(zau, lp, zu)
+        assert _rel(zau, ref) <= TOL["f64"], seed
+        for variant in ("lfmtx", "lfmtx_gpu"):
+            zau = z0.copy()
+            gpu.adot_body_lfmtx_host(zau, lp, zu, variant)
+            assert _rel(zau, ref) <= TOL["f64"], (seed, variant)
+        gpu.adot_body_lfdel_batch(lp)
+    finally:
+        gpu.set_ref_layout(prev)
