@@ -25,9 +25,13 @@ uneven instances in one arena, solved in FP64 and with autotuned BF16
 factors under per-instance FP64 refinement.
 """
 import argparse
+import csv
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -54,6 +58,89 @@ def load_traffic(kernel_phase):
         if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel_phase:
             return float(parts[2])
     return None
+
+
+# what the counter passes time: the two FP64 apply kernels and the
+# calibration kernels of known traffic
+_APPLY_KERNELS = {"phase1": "hmv_stream_kernel<double, 1, 0, 0,",
+                  "phase2": "hmv_stream_kernel<double, 2, 0, 0,"}
+CAL_READ_BYTES = 2 << 30   # hmv_read_kernel reads this many bytes
+CAL_COPY_BYTES = 1 << 30   # hmv_copy_kernel writes this many bytes
+
+
+def traffic_pass(H, subdiv):
+    """Counter pass (run by measure_traffic under rocprofv3): the benchmark's
+    arena, the calibration kernels and six applies, nothing timed."""
+    L = H.hmv_lib()
+    if L.hmv_init(0) != 0:
+        raise SystemExit("no usable GPU: " + L.hmv_last_error().decode())
+    mesh = H.merge(H.icosphere(subdiv), H.icosphere(subdiv, 1.0, (3.0, 0.0, 0.0)))
+    hm = H.HMatrix(*mesh, H.baseline_params(aca_eps=1e-5))
+    a64 = H.Arena(hm, "f64")
+    dx = H.DeviceBuffer.from_array(H.uniform(1, hm.nd))
+    dy = H.DeviceBuffer(8 * hm.nd)
+    H.stream_test(0, CAL_READ_BYTES, reps=2)
+    H.stream_test(1, CAL_COPY_BYTES, reps=2)
+    for _ in range(6):
+        a64.apply(dx.ptr, dy.ptr, 0.0)
+    L.hmv_device_synchronize()
+
+
+def measure_traffic(subdiv, log):
+    """HBM bytes per launch of each apply kernel, measured in this run: two
+    `rocprofv3 --pmc` passes over `bench.py --traffic-pass` (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch), each counter scaled by what it reads
+    for the calibration kernel of known size in the same pass.  Returns
+    ({phase: bytes}, {"fetch": f, "write": w}) or (None, reason)."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not found"
+    per = {}
+    scale = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        for counter, cal, cal_bytes in (("FETCH_SIZE", "hmv_read_kernel", CAL_READ_BYTES),
+                                        ("WRITE_SIZE", "hmv_copy_kernel", CAL_COPY_BYTES)):
+            d = os.path.join(td, counter)
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+                   sys.executable, os.path.abspath(__file__), "--traffic-pass", "--subdiv", str(subdiv)]
+            log("traffic: rocprofv3 --pmc %s pass" % counter)
+            try:
+                p = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
+                                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=300)
+            except subprocess.TimeoutExpired:
+                return None, "%s pass timed out" % counter
+            path = os.path.join(d, "run_counter_collection.csv")
+            if not os.path.exists(path):
+                return None, "%s pass wrote no counters (exit %d)" % (counter, p.returncode)
+            got = counter_bytes(path, counter, cal, cal_bytes)
+            if isinstance(got, str):
+                return None, "%s pass: %s" % (counter, got)
+            scale[counter], pb = got
+            for ph, b in pb.items():
+                per[ph] = per.get(ph, 0.0) + b
+    if any(v <= 0 for v in per.values()):
+        return None, "no apply dispatch counted"
+    return per, {"fetch": round(scale["FETCH_SIZE"], 4), "write": round(scale["WRITE_SIZE"], 4)}
+
+
+def counter_bytes(path, counter, cal, cal_bytes):
+    """From one rocprofv3 counter CSV: the scale that turns `counter` (KiB
+    per dispatch) into bytes, from the calibration kernel `cal` of known
+    traffic `cal_bytes`, and the scaled bytes per launch of each apply phase
+    (the mean over dispatches of each kernel template, summed over the
+    phase's templates).  Returns (scale, {phase: bytes}) or a reason."""
+    vals = {}
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    cal_kib = [v for n, vs in vals.items() if cal in n for v in vs]
+    if not cal_kib or min(cal_kib) <= 0:
+        return "no %s dispatch" % cal
+    scale = cal_bytes / (1024.0 * float(np.median(cal_kib)))
+    per = {ph: 1024.0 * scale * sum(float(np.mean(vs)) for n, vs in vals.items() if key in n)
+           for ph, key in _APPLY_KERNELS.items()}
+    return scale, per
 
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
@@ -165,7 +252,17 @@ def main():
     ap.add_argument("--applies", action="store_true",
                     help="only the single-vector applies on 327,680 unknowns in the four storage "
                          "precisions (for profiler runs)")
+    ap.add_argument("--traffic-pass", action="store_true",
+                    help="counter pass over the benchmark's applies and calibration kernels "
+                         "(bench.py runs it under rocprofv3 to measure HBM traffic)")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="take roofline.traffic from the committed PMC summary instead of "
+                         "measuring it with two rocprofv3 passes")
     args = ap.parse_args()
+    if args.traffic_pass:
+        import __graft_entry__ as G
+        traffic_pass(G.load_package(), args.subdiv)
+        return
     if args.applies:
         import __graft_entry__ as G
         print(json.dumps({"applies_327680": applies_only(G.load_package(), log)}), flush=True)
@@ -240,12 +337,27 @@ def main():
     dom = max(phases, key=lambda k: phases[k][0])
     dms, dbytes, dname = phases[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9
-    traffic = load_traffic(dom)
+    # HBM traffic per launch: measured by two counter passes of this workload
+    # (or, with --no-pmc / --quick or when they fail, the committed summary)
+    measured, how = (None, "not requested") if (args.quick or args.no_pmc) else \
+        measure_traffic(args.subdiv, log)
+    if measured:
+        traffic = measured[dom]
+        tsrc = ("measured in this run: rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over "
+                "bench.py --traffic-pass (same workload), each scaled by the calibration "
+                "kernels' counts (fetch x%.4g, write x%.4g)" % (how["fetch"], how["write"]))
+        log("traffic: phase1 %.4g B, phase2 %.4g B per launch (%.3fx / %.3fx algorithmic)"
+            % (measured["phase1"], measured["phase2"], measured["phase1"] / bytes_p1,
+               measured["phase2"] / bytes_p2))
+    else:
+        traffic = load_traffic(dom)
+        tsrc = ("profiles/pmc_traffic_two_spheres_163840.txt (rocprofv3 --pmc FETCH_SIZE and "
+                "WRITE_SIZE of this kernel on this workload, scaled by the calibration kernels' "
+                "counts; not measured in this run: %s)" % how)
     roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": "profiles/pmc_traffic_two_spheres_163840.txt (rocprofv3 --pmc "
-                                  "FETCH_SIZE and WRITE_SIZE of this kernel on this workload, scaled "
-                                  "by the calibration kernels' counts; not measured in this run)",
+                "traffic_source": tsrc,
+                "traffic_per_phase": ({k: round(v) for k, v in measured.items()} if measured else None),
                 "alg_bytes_per_launch": dbytes, "avg_launch_ms": round(dms, 5),
                 "phases_ms": {"phase1": round(ms1, 5), "phase2": round(ms2, 5), "apply": round(msa, 5)},
                 "apply_event_GBs": round(alg_bytes / (msa * 1e-3) / 1e9, 1)}
