@@ -1,7 +1,7 @@
 """Writes the golden fixtures under tests/golden/ from the reference's data files.
 
-Run here (where /root/reference exists); the GPU box uses the committed
-outputs.  Nothing from the reference's source code is stored: only its mesh
+Runs only where the reference tree exists at /root/reference; the tests read
+the committed outputs.  Nothing from the reference's source code is stored: only its mesh
 data files (converted to .npz) and the figures its recorded run printed.
 
   input_sample_mesh.npz  input_sample.pbf            (304 nodes, 600 triangles)
