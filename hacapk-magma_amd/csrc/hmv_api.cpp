@@ -187,7 +187,6 @@ void release(hmv_arena_s *a) {
   (void)hipFree(a->d_codes);
   (void)hipFree(a->d_t);
   (void)hipFree(a->d_hx);
-  (void)hipFree(a->d_hy);
   (void)hipHostFree(a->h_x);
   (void)hipHostFree(a->h_y);
   if (a->hstream) (void)hipStreamDestroy(a->hstream);
@@ -825,23 +824,25 @@ int hmv_apply_host(hmv_arena a, const double *zu, double *zau) {
   hipError_t e;
   const size_t n = (size_t)a->nd, nb = sizeof(double) * n;
   if (!a->hstream) {  // created last: set only when every buffer exists
-    if ((e = hipMalloc(&a->d_hx, nb)) != hipSuccess || (e = hipMalloc(&a->d_hy, nb)) != hipSuccess ||
+    if ((e = hipMalloc(&a->d_hx, nb)) != hipSuccess ||
         (e = hipHostMalloc((void **)&a->h_x, nb, hipHostMallocDefault)) != hipSuccess ||
         (e = hipHostMalloc((void **)&a->h_y, nb, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostGetDevicePointer((void **)&a->h_y_dev, a->h_y, 0)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&a->hstream, hipStreamNonBlocking)) != hipSuccess) {
       // leave nothing half set up: the next call starts again
       (void)hipFree(a->d_hx);
-      (void)hipFree(a->d_hy);
       (void)hipHostFree(a->h_x);
       (void)hipHostFree(a->h_y);
-      a->d_hx = a->d_hy = a->h_x = a->h_y = nullptr;
+      a->d_hx = a->h_x = a->h_y = a->h_y_dev = nullptr;
       a->hstream = nullptr;
       return hip_fail(e, "hmv_apply_host: setup");
     }
   }
-  // two copies over PCIe, both from pinned staging: zu up, H zu down; the
-  // CPU's passes over the caller's arrays (zu into staging, zau += H zu out
-  // of it) are split over the pool's threads
+  // one copy over PCIe, from pinned staging: zu up; the apply's phase 2
+  // writes H zu straight into pinned host memory as its rows finish (0.48
+  // against 0.52 ms with a download after the apply at 163,840 unknowns).
+  // The CPU's passes over the caller's arrays (zu into staging, zau += H zu
+  // out of it) are split over the pool's threads.
   hmv::HostPool &pool = hmv::HostPool::get();
   double *hx = a->h_x, *hy = a->h_y;
   pool.parallel_for(n, [&](size_t lo, size_t hi) {
@@ -849,11 +850,10 @@ int hmv_apply_host(hmv_arena a, const double *zu, double *zau) {
   });
   if ((e = hipMemcpyAsync(a->d_hx, hx, nb, hipMemcpyHostToDevice, a->hstream)) != hipSuccess)
     return hip_fail(e, "hmv_apply_host: copy x");
-  if ((e = hmv::enqueue_apply(a, a->d_hx, a->d_hy, 0.0, nullptr, a->hstream)) != hipSuccess)
+  if ((e = hmv::enqueue_apply(a, a->d_hx, a->h_y_dev, 0.0, nullptr, a->hstream)) != hipSuccess)
     return hip_fail(e, "hmv_apply_host: launch");
-  if ((e = hipMemcpyAsync(hy, a->d_hy, nb, hipMemcpyDeviceToHost, a->hstream)) != hipSuccess ||
-      (e = hipStreamSynchronize(a->hstream)) != hipSuccess)
-    return hip_fail(e, "hmv_apply_host: copy back");
+  if ((e = hipStreamSynchronize(a->hstream)) != hipSuccess)
+    return hip_fail(e, "hmv_apply_host: sync");
   pool.parallel_for(n, [&](size_t lo, size_t hi) {
     for (size_t i = lo; i < hi; ++i) zau[i] += hy[i];
   });

@@ -55,9 +55,10 @@ struct hmv_arena_s {
   std::vector<int> seg;    // instance row offsets (ninst + 1) of a multi-instance arena
   int *d_winst1 = nullptr, *d_winst2 = nullptr;  // instance of each work item (multi only)
   hmv_tuning tuning{};     // the tuning the arena was built with
-  // hmv_apply_host: device x / y, pinned host staging of both, its stream
-  // (made on the first call)
-  double *d_hx = nullptr, *d_hy = nullptr, *h_x = nullptr, *h_y = nullptr;
+  // hmv_apply_host: device x, pinned host staging of x and y, y's device
+  // address (the apply writes H zu there directly), its stream (made on the
+  // first call)
+  double *d_hx = nullptr, *h_x = nullptr, *h_y = nullptr, *h_y_dev = nullptr;
   hipStream_t hstream = nullptr;
   HmvSolveWork *solve = nullptr;             // solver workspace (lazy)
   struct HmvGcrWork *gcr = nullptr;          // GCR(m) workspace (lazy)
