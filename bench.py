@@ -104,9 +104,11 @@ def measure_traffic(subdiv, log):
             cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
                    sys.executable, os.path.abspath(__file__), "--traffic-pass", "--subdiv", str(subdiv)]
             log("traffic: rocprofv3 --pmc %s pass" % counter)
+            # a pass takes about 10 s; a stuck profiler costs at most 2 minutes
+            # and leaves the committed figure in place
             try:
                 p = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
-                                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=300)
+                                   stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=120)
             except subprocess.TimeoutExpired:
                 return None, "%s pass timed out" % counter
             path = os.path.join(d, "run_counter_collection.csv")
