@@ -193,7 +193,10 @@ int hmv_apply_block_static(hmv_arena a, int k, const double *x, double *y, doubl
 int hmv_apply_block_timed(hmv_arena a, int k, const double *x, double *y, int reps, double *ms);
 
 /* zau += H*zu on host vectors (the accumulate semantics of
- * c_hacapk_adot_body_lfmtx_batch_, HACApK_BATCH.c:257-271); synchronous. */
+ * c_hacapk_adot_body_lfmtx_batch_, HACApK_BATCH.c:257-271); synchronous.
+ * zu goes up through pinned staging and the apply writes H*zu into pinned
+ * host memory, both held by the arena from the first call on; calls on one
+ * arena from several threads at once race on that staging. */
 int hmv_apply_host(hmv_arena a, const double *zu, double *zau);
 
 /* Bytes of the device t-partial buffer owned by the arena. */
