@@ -64,3 +64,27 @@ def test_random_structures_through_reference_entry_points(gpu, O, seed):
         gpu.adot_body_lfdel_batch(lp)
     finally:
         gpu.set_ref_layout(prev)
+
+
+@pytest.mark.parametrize("prec", ["f64", "bf16_all"])
+def test_random_structures_as_instances_of_one_arena(gpu, O, prec):
+    """Five random structures (ragged sizes, rank-0 leaves, a row with no
+    leaf) as the instances of one multi-instance arena: every instance's rows
+    of the apply and of the H-matmat (k = 3 on the vector units, k = 20 on
+    MFMA) equal the oracle's apply of that instance alone."""
+    hms = [_leaf_set(gpu, seed) for seed in (3, 5, 8, 13, 21)]
+    m = gpu.MultiArena(hms, prec)
+    assert m.offsets[-1] == sum(hm.nd for hm in hms)
+    x = gpu.uniform(123, m.nd)
+    y = m.matvec(x)
+    for i, hm in enumerate(hms):
+        lo, hi = m.offsets[i], m.offsets[i + 1]
+        assert _rel(y[lo:hi], O.adot_body_lfmtx(hm, x[lo:hi])) <= TOL[prec], (i, prec)
+    for k in (3, 20):
+        X = np.stack([gpu.uniform(600 + j, m.nd) for j in range(k)], axis=1)
+        Y = m.matmat(X)
+        for i, hm in enumerate(hms):
+            lo, hi = m.offsets[i], m.offsets[i + 1]
+            for j in (0, k - 1):
+                assert _rel(Y[lo:hi, j], O.adot_body_lfmtx(hm, X[lo:hi, j])) <= TOL[prec], (i, k, j)
+    m.free()
