@@ -88,3 +88,24 @@ def test_random_structures_as_instances_of_one_arena(gpu, O, prec):
             for j in (0, k - 1):
                 assert _rel(Y[lo:hi, j], O.adot_body_lfmtx(hm, X[lo:hi, j])) <= TOL[prec], (i, k, j)
     m.free()
+
+
+@pytest.mark.parametrize("seed", [2, 9, 15])
+def test_random_structures_arena_file_round_trip(gpu, O, seed, tmp_path):
+    """An arena of a random structure saved to an arena file and loaded back
+    (no leaves, no repack) applies bit for bit as the original does, in every
+    storage precision, and within the precision's bar of the oracle."""
+    hm = _leaf_set(gpu, seed)
+    x = gpu.uniform(140 + seed, hm.nd)
+    y_ref = O.adot_body_lfmtx(hm, x)
+    for prec in TOL:
+        a = gpu.Arena(hm, prec, tuning=TUNINGS[2] if seed % 2 else None)
+        path = str(tmp_path / ("%s_%d.hmv" % (prec, seed)))
+        a.save(path)
+        b = gpu.Arena.load(path)
+        assert b.nd == hm.nd and b.precision == prec
+        ya, yb = a.matvec(x), b.matvec(x)
+        assert np.array_equal(ya, yb), (seed, prec)
+        assert _rel(yb, y_ref) <= TOL[prec], (seed, prec)
+        a.free()
+        b.free()
