@@ -168,7 +168,8 @@ def _ref_param(hm):
     return param
 
 
-@pytest.mark.parametrize("mesh,eps", [("input_10ts", 1e-6), ("icosphere3", 1e-4)])
+@pytest.mark.parametrize("mesh,eps", [("input_10ts", 1e-6), ("icosphere3", 1e-4),
+                                      ("input_1ts", 1e-5), ("icosphere4", 1e-8)])
 def test_acaplus_bitexact_vs_reference(H, ref, mesh, eps):
     """Every low-rank leaf of the host fill equals the reference's own
     HACApK_acaplus (m_HACApK_base.f90:783-975, compiled from its source by
@@ -177,12 +178,12 @@ def test_acaplus_bitexact_vs_reference(H, ref, mesh, eps):
     1e-6 is the configuration of the reference's recorded run."""
     if not hasattr(ref, "ref_acaplus"):
         pytest.skip("oracle/_ref built without the ACA+ routine")
-    if mesh == "input_10ts":
-        d = np.load(os.path.join(HERE, "golden", "input_10ts_mesh.npz"))
+    if mesh.startswith("input_"):
+        d = np.load(os.path.join(HERE, "golden", mesh + "_mesh.npz"))
         nodes, faces = d["nodes"], d["faces"]
         hm = H.HMatrix(nodes, faces, H.HParams(leaf_size=15, aca_eps=eps, kernel="ppohbem"))
     else:
-        nodes, faces = H.icosphere(3)
+        nodes, faces = H.icosphere(int(mesh[-1]))
         hm = H.HMatrix(nodes, faces, H.baseline_params(aca_eps=eps, kernel="ppohbem"))
     nodes = np.ascontiguousarray(nodes, dtype=np.float64)
     faces = np.ascontiguousarray(faces, dtype=np.int32)
